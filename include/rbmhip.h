@@ -1,0 +1,166 @@
+/*
+ * rbmhip.h -- C ABI of the MI355X (gfx950) variational-Monte-Carlo library
+ * librbmhip.so.  Plain pointers and sizes only; every function returns
+ * RBMHIP_OK (0) or a negative error code, and rbmhip_last_error() returns the
+ * message of the calling thread's last failure.  No exception crosses the ABI.
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) doubles;
+ *   - a spin configuration is two uint64 words, spin i = +1 iff bit i is set
+ *     (reference include/Spins.h:125-129), N <= 128;
+ *   - model, operator and ensemble handles are host objects; device copies are
+ *     made on first use on the current device and refreshed after a setter;
+ *   - every compute call runs on the library's stream of the current device and
+ *     returns after the results are on the host, like the reference's blocking
+ *     calls; calls from several host threads must not share a handle;
+ *   - compute calls need a GPU and fail with RBMHIP_ERR_NO_DEVICE otherwise.
+ *
+ * Each declaration cites the reference interface it replaces (file:line,
+ * relative to the reference repository root).
+ */
+#ifndef RBMHIP_H
+#define RBMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBMHIP_OK 0
+#define RBMHIP_ERR_ARG (-1)
+#define RBMHIP_ERR_NO_DEVICE (-2)
+#define RBMHIP_ERR_HIP (-3)
+#define RBMHIP_ERR_UNSUPPORTED (-4)
+#define RBMHIP_ERR_NUMERIC (-5)
+
+#define RBMHIP_FP32 32 /* complex64 kernels, FP64 reductions */
+#define RBMHIP_FP64 64 /* complex128 kernels */
+
+typedef struct rbmhip_psi rbmhip_psi;
+typedef struct rbmhip_op rbmhip_op;
+typedef struct rbmhip_ens rbmhip_ens;
+
+/* ---- library / device ---------------------------------------------------- */
+const char* rbmhip_last_error(void);
+int rbmhip_version(void);
+/* setDevice: source/RBMonGPU.cu:7-9 (bound at pyRBMonGPU/main.cpp:611) */
+int rbmhip_set_device(int device);
+int rbmhip_device_count(int* count);
+/* start_profiling / stop_profiling: source/RBMonGPU.cu:11-17 (main.cpp:612-613) */
+int rbmhip_start_profiling(void);
+int rbmhip_stop_profiling(void);
+/* activation_function (my_logcosh), pyRBMonGPU/main.cpp:607-609;
+ * derivative = 1 evaluates my_tanh (include/quantum_state/psi_functions.hpp:97-114) */
+int rbmhip_activation(const double* z, int64_t n, int derivative, double* out);
+
+/* ---- wavefunctions ------------------------------------------------------- */
+/* Psi(alpha, beta, b, W, prefactor, free_quantum_axis, gpu): include/quantum_state/Psi.hpp:222-239.
+ * b: M complex, W: N x M complex row-major, a: optional visible bias (N complex,
+ * an extension with no counterpart in the reference; NULL keeps the reference model). */
+int rbmhip_rbm_create(int N, int M, const double* b, const double* W, const double* a, double prefactor,
+                      int precision, rbmhip_psi** out);
+/* Psi::set_params (source/quantum_state/Psi.cu:80-92), device side */
+int rbmhip_rbm_set_params(rbmhip_psi* psi, const double* b, const double* W, const double* a);
+/* PsiDeep(alpha, beta, biases, connections, W, final_weights, prefactor, ...):
+ * include/quantum_state/PsiDeep.hpp:339-439.  Layers are flattened: sizes[L],
+ * conn_counts[L] (C_l), conns = concatenated C_l x size_l uint32 tables,
+ * weights = concatenated C_l x size_l complex, biases = concatenated size_l
+ * complex, final_weights = size_{L-1} complex.  ref_compat_final = 1 keeps the
+ * reference's final-weight derivative log_psi * x_j (PsiDeep.hpp:285-291);
+ * 0 uses dlog psi/df_j = x_j. */
+int rbmhip_deep_create(int N, int num_layers, const int32_t* sizes, const int32_t* conn_counts,
+                       const uint32_t* conns, const double* weights, const double* biases,
+                       const double* final_weights, double prefactor, int translational_invariance,
+                       int ref_compat_final, int precision, rbmhip_psi** out);
+/* PsiDeepT::set_params: source/quantum_state/PsiDeep.cu:143-177 */
+int rbmhip_deep_set_params(rbmhip_psi* psi, const double* weights, const double* biases, const double* final_weights);
+/* PsiDeep.translational_invariance (pyRBMonGPU/main.cpp:110) */
+int rbmhip_deep_set_translational_invariance(rbmhip_psi* psi, int on);
+/* prefactor property (main.cpp:64, 109) */
+int rbmhip_psi_set_prefactor(rbmhip_psi* psi, double prefactor);
+/* num_params (main.cpp:89, 114): 2N + M + NM (+ N with visible bias) for the RBM,
+ * source/quantum_state/PsiDeep.cu:36-63 for PsiDeep */
+int rbmhip_psi_num_params(const rbmhip_psi* psi, int64_t* out);
+int rbmhip_psi_destroy(rbmhip_psi* psi);
+
+/* ---- operators ----------------------------------------------------------- */
+/* Operator(coefficients, pauli_types, pauli_indices, gpu): include/operator/Operator.hpp:309-324.
+ * types: S x L int32 (0 = I, 1 = X, 2 = Y, 3 = Z); idx: S x L int32, each
+ * string ends at the first -1 (a string without one continues into the next
+ * row, as in Operator.hpp:148-152). */
+int rbmhip_op_create(int S, int L, const double* coefficients, const int32_t* types, const int32_t* indices,
+                     rbmhip_op** out);
+/* Canonical form: strings grouped by flip mask.  hermitian = 1 when the
+ * operator's matrix equals its conjugate transpose. */
+int rbmhip_op_info(const rbmhip_op* op, int* num_diagonal, int* num_flip_masks, int* hermitian, int* max_index);
+int rbmhip_op_destroy(rbmhip_op* op);
+
+/* ---- spin ensembles ------------------------------------------------------ */
+/* ExactSummation(N, gpu): source/spin_ensembles/ExactSummation.cu:16-24 */
+int rbmhip_exact_create(int N, rbmhip_ens** out);
+/* ExactSummation::set_total_z_symmetry: ExactSummation.cu:26-45 */
+int rbmhip_exact_set_total_z(rbmhip_ens* ens, int sector);
+/* MonteCarloLoop(num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains, gpu):
+ * source/spin_ensembles/MonteCarloLoop.cu:20-42.  seed keys the Philox stream
+ * (the reference seeds curand with 0, MonteCarloLoop.cu:15). */
+int rbmhip_mc_create(uint32_t num_samples, uint32_t num_sweeps, uint32_t num_thermalization_sweeps,
+                     uint32_t num_markov_chains, uint64_t seed, rbmhip_ens** out);
+/* MonteCarloLoop(MonteCarloLoop&): MonteCarloLoop.cu:44-66 */
+int rbmhip_mc_copy(const rbmhip_ens* other, rbmhip_ens** out);
+/* set_total_z_symmetry / set_fast_sweep: include/spin_ensembles/MonteCarloLoop.hpp:283-291 */
+int rbmhip_mc_set_total_z(rbmhip_ens* ens, int sector);
+int rbmhip_mc_set_fast_sweep(rbmhip_ens* ens, uint32_t num_tries);
+/* num_steps (main.cpp:280, 290) */
+int rbmhip_ens_num_steps(const rbmhip_ens* ens, uint64_t* out);
+/* acceptance_rate counters of the last call (main.cpp:282-284) */
+int rbmhip_mc_acceptance(const rbmhip_ens* ens, uint64_t* acceptances, uint64_t* rejections);
+/* Per-sample records of the last Monte Carlo call (no reference counterpart:
+ * the reference keeps no samples).  Any output may be NULL. */
+int rbmhip_mc_last_samples(const rbmhip_ens* ens, uint64_t* spins, double* log_psi, double* e_loc);
+int rbmhip_ens_destroy(rbmhip_ens* ens);
+
+/* ---- batched evaluation -------------------------------------------------- */
+/* log psi of n configurations (Psi::log_psi_s_std, source/quantum_state/Psi.cu:36-69) */
+int rbmhip_log_psi(rbmhip_psi* psi, const uint64_t* spins, int64_t n, double* out);
+/* local energy of n configurations (Operator::local_energy, include/operator/Operator.hpp:90-115) */
+int rbmhip_local_energy(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, int64_t n, double* out);
+/* psi_O_k_vector(psi, spins): source/network_functions/PsiOkVector.cu:9-44;
+ * writes num_params complex values, entries below 2N are zero */
+int rbmhip_o_k_vector(rbmhip_psi* psi, const uint64_t* spins, double* out);
+/* psi_vector: source/network_functions/PsiVector.cu:13-51 (2^N complex values) */
+int rbmhip_psi_vector(rbmhip_psi* psi, double* out);
+/* psi_norm: source/network_functions/PsiNorm.cu:13-44 (FP64 log-sum-exp) */
+int rbmhip_psi_norm(rbmhip_psi* psi, rbmhip_ens* exact, double* out);
+
+/* ---- network functions --------------------------------------------------- */
+/* ExpectationValue::operator(): source/network_functions/ExpectationValue.cu:17-73 */
+int rbmhip_expectation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* out);
+/* list version: ExpectationValue.cu:448-518; out has 2 * n_ops doubles */
+int rbmhip_expectation_list(rbmhip_psi* psi, rbmhip_op* const* ops, int n_ops, rbmhip_ens* ens, double* out);
+/* ExpectationValue::fluctuation: ExpectationValue.cu:130-175 */
+int rbmhip_fluctuation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* fluctuation, double* expectation);
+/* ExpectationValue::gradient: ExpectationValue.cu:177-256; grad has num_params
+ * complex values.  The reference's term conj<E_k> equals <E conj(O_k)> for a
+ * Hermitian operator; such operators use 2<E conj O> - 2<E> conj<O>. */
+int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad, double* expectation);
+/* get_S_matrix(Psi, ExactSummation): source/network_functions/S_matrix.cu:10-58
+ * (num_params x num_params complex, unweighted and uncentred as in the reference) */
+int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out);
+/* HilbertSpaceDistance::distance / gradient (free_quantum_axis = false):
+ * source/network_functions/HilbertSpaceDistance.cu:193-269.  grad (num_params
+ * of psi_prime) is written when want_grad != 0. */
+int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
+                       int want_grad, double nu, double* distance, double* grad);
+
+/* ---- kernel timing (HIP events on the library stream) -------------------- */
+int rbmhip_timing_enable(int on);
+int rbmhip_timing_reset(void);
+/* total milliseconds and launch count of a kernel family ("mc", "eval", "ok_reduce", ...) */
+int rbmhip_timing_query(const char* name, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBMHIP_H */

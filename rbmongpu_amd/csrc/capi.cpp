@@ -1,0 +1,1313 @@
+// Host side of librbmhip.so: the C ABI declared in include/rbmhip.h.
+//
+// Objects (wavefunctions, operators, ensembles) are host descriptors; device
+// copies are made lazily per device and refreshed after setters.  Each call
+// runs its kernels on one stream of the current device, reduces in FP64 on
+// the device and synchronises before returning (the reference's calls block on
+// a device-to-host copy, include/Array.hpp / source/Array.cu:96-100).
+#include "../../include/rbmhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rbm_kernels.hpp"
+#include "deep_kernels.hpp"
+
+using cd = std::complex<double>;
+
+namespace rbmhip {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define RBM_HIPCHK(x)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::rbmhip::Error(RBMHIP_ERR_HIP, std::string(#x) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RBM_REQUIRE(cond, msg)                                              \
+    do {                                                                    \
+        if (!(cond)) throw ::rbmhip::Error(RBMHIP_ERR_ARG, std::string(msg)); \
+    } while (0)
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return RBMHIP_OK;
+    } catch (Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return RBMHIP_ERR_ARG;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return RBMHIP_ERR_ARG;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device memory and per-device context
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    void* ensure(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (bytes > n) {
+            if (p) RBM_HIPCHK(hipFree(p));
+            p = nullptr;
+            n = 0;
+            RBM_HIPCHK(hipMalloc(&p, bytes));
+            n = bytes;
+        }
+        return p;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t max_lds = 65536;
+    int num_cus = 256;
+    std::map<std::string, DevBuf> ws;
+    bool timing = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::map<std::string, std::pair<double, int64_t>> totals;
+
+    void* buf(const char* name, size_t bytes) { return ws[name].ensure(bytes); }
+
+    template <class F>
+    void launch(const char* name, F&& f) {
+        if (!timing) {
+            RBM_HIPCHK(f());
+            return;
+        }
+        Pending p{name, nullptr, nullptr};
+        RBM_HIPCHK(hipEventCreate(&p.a));
+        RBM_HIPCHK(hipEventCreate(&p.b));
+        RBM_HIPCHK(hipEventRecord(p.a, stream));
+        RBM_HIPCHK(f());
+        RBM_HIPCHK(hipEventRecord(p.b, stream));
+        pending.push_back(p);
+    }
+
+    void sync() {
+        RBM_HIPCHK(hipStreamSynchronize(stream));
+        for (auto& p : pending) {
+            float ms = 0.f;
+            RBM_HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+            auto& t = totals[p.name];
+            t.first += ms;
+            t.second += 1;
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        pending.clear();
+    }
+
+    void d2h(void* dst, const void* src, size_t bytes) {
+        RBM_HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    }
+    void h2d(void* dst, const void* src, size_t bytes) {
+        RBM_HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    }
+};
+
+static std::mutex g_mu;
+static int g_device = 0;
+static std::map<int, std::unique_ptr<Ctx>> g_ctx;
+
+static Ctx& ctx() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        throw Error(RBMHIP_ERR_NO_DEVICE, "no HIP device is visible; librbmhip runs on the GPU only");
+    RBM_REQUIRE(g_device >= 0 && g_device < count, "device index out of range");
+    RBM_HIPCHK(hipSetDevice(g_device));
+    auto it = g_ctx.find(g_device);
+    if (it != g_ctx.end()) return *it->second;
+    auto c = std::make_unique<Ctx>();
+    c->device = g_device;
+    RBM_HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, g_device) == hipSuccess && v > 0)
+        c->max_lds = (size_t)v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, g_device) == hipSuccess && v > 0)
+        c->num_cus = v;
+    Ctx& ref = *c;
+    g_ctx[g_device] = std::move(c);
+    return ref;
+}
+
+}  // namespace rbmhip
+
+using namespace rbmhip;
+
+// ---------------------------------------------------------------------------
+// wavefunction descriptors
+// ---------------------------------------------------------------------------
+struct DevModel {
+    uint64_t version = ~(uint64_t)0;
+    int precision = 0;
+    DevBuf W, b, a;
+    // PsiDeep
+    DevBuf conns, weights, biases, final_w, rhs_begin, rhs_entries;
+};
+
+struct rbmhip_psi {
+    int kind = 0;  // 0 RBM, 1 PsiDeep
+    int N = 0, M = 0;
+    int precision = RBMHIP_FP32;
+    double prefactor = 1.0;
+    std::vector<cd> b, W, a;
+    bool has_a = false;
+    // PsiDeep
+    std::vector<int32_t> sizes, conn_counts;
+    std::vector<uint32_t> conns;
+    std::vector<cd> weights, biases, final_weights;
+    std::vector<int> rhs_begin, rhs_entries, rb_off;
+    int translational_invariance = 0;
+    int ref_compat_final = 1;
+    uint64_t version = 0;
+    std::map<int, DevModel> dev;
+
+    int64_t num_params() const {
+        if (kind == 0) return 2 * (int64_t)N + M + (int64_t)N * M + (has_a ? N : 0);
+        int64_t p = 2 * (int64_t)N;
+        for (size_t l = 0; l < sizes.size(); l++) p += sizes[l] + (int64_t)conn_counts[l] * sizes[l];
+        return p + sizes.back();
+    }
+    ~rbmhip_psi() {
+        for (auto& kv : dev) {
+            kv.second.W.release(); kv.second.b.release(); kv.second.a.release();
+            kv.second.conns.release(); kv.second.weights.release(); kv.second.biases.release();
+            kv.second.final_w.release(); kv.second.rhs_begin.release(); kv.second.rhs_entries.release();
+        }
+    }
+};
+
+template <typename T>
+static void upload_cx(Ctx& c, DevBuf& buf, const std::vector<cd>& v) {
+    std::vector<cx<T>> h(v.size());
+    for (size_t i = 0; i < v.size(); i++) h[i] = mk<T>((T)v[i].real(), (T)v[i].imag());
+    buf.ensure(sizeof(cx<T>) * h.size());
+    if (!h.empty()) RBM_HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cx<T>) * h.size(), hipMemcpyHostToDevice));
+    (void)c;
+}
+
+template <typename T>
+static RbmDev<T> rbm_dev(Ctx& c, rbmhip_psi* psi) {
+    DevModel& d = psi->dev[c.device];
+    if (d.version != psi->version || d.precision != psi->precision) {
+        upload_cx<T>(c, d.W, psi->W);
+        upload_cx<T>(c, d.b, psi->b);
+        if (psi->has_a) upload_cx<T>(c, d.a, psi->a);
+        d.version = psi->version;
+        d.precision = psi->precision;
+    }
+    RbmDev<T> md;
+    md.N = psi->N;
+    md.M = psi->M;
+    md.W = reinterpret_cast<const cx<T>*>(d.W.p);
+    md.b = reinterpret_cast<const cx<T>*>(d.b.p);
+    md.a = psi->has_a ? reinterpret_cast<const cx<T>*>(d.a.p) : nullptr;
+    return md;
+}
+
+template <typename T>
+static DeepDev<T> deep_dev(Ctx& c, rbmhip_psi* psi) {
+    DevModel& d = psi->dev[c.device];
+    const int L = (int)psi->sizes.size();
+    if (d.version != psi->version || d.precision != psi->precision) {
+        upload_cx<T>(c, d.weights, psi->weights);
+        upload_cx<T>(c, d.biases, psi->biases);
+        upload_cx<T>(c, d.final_w, psi->final_weights);
+        d.conns.ensure(sizeof(uint32_t) * psi->conns.size());
+        RBM_HIPCHK(hipMemcpy(d.conns.p, psi->conns.data(), sizeof(uint32_t) * psi->conns.size(), hipMemcpyHostToDevice));
+        d.rhs_begin.ensure(sizeof(int) * psi->rhs_begin.size());
+        if (!psi->rhs_begin.empty())
+            RBM_HIPCHK(hipMemcpy(d.rhs_begin.p, psi->rhs_begin.data(), sizeof(int) * psi->rhs_begin.size(), hipMemcpyHostToDevice));
+        d.rhs_entries.ensure(sizeof(int) * psi->rhs_entries.size());
+        if (!psi->rhs_entries.empty())
+            RBM_HIPCHK(hipMemcpy(d.rhs_entries.p, psi->rhs_entries.data(), sizeof(int) * psi->rhs_entries.size(), hipMemcpyHostToDevice));
+        d.version = psi->version;
+        d.precision = psi->precision;
+    }
+    DeepDev<T> md;
+    md.N = psi->N;
+    md.num_layers = L;
+    int64_t woff = 0, boff = 0;
+    int width = psi->N, units = 0;
+    for (int l = 0; l < L; l++) {
+        md.size[l] = psi->sizes[l];
+        md.conn[l] = psi->conn_counts[l];
+        md.w_off[l] = woff;
+        md.b_off[l] = boff;
+        woff += (int64_t)psi->conn_counts[l] * psi->sizes[l];
+        boff += psi->sizes[l];
+        width = std::max(width, (int)psi->sizes[l]);
+        units += psi->sizes[l];
+        md.rb_off[l] = l < (int)psi->rb_off.size() ? psi->rb_off[l] : 0;
+    }
+    md.rhs_begin = reinterpret_cast<const int*>(d.rhs_begin.p);
+    md.rhs_entries = reinterpret_cast<const int*>(d.rhs_entries.p);
+    md.width = width;
+    md.units = units;
+    md.conns = reinterpret_cast<const uint32_t*>(d.conns.p);
+    md.weights = reinterpret_cast<const cx<T>*>(d.weights.p);
+    md.biases = reinterpret_cast<const cx<T>*>(d.biases.p);
+    md.final_w = reinterpret_cast<const cx<T>*>(d.final_w.p);
+    md.translational_invariance = psi->translational_invariance;
+    md.ref_compat_final = psi->ref_compat_final;
+    return md;
+}
+
+// ---------------------------------------------------------------------------
+// operators: canonical flip-mask form of the Pauli table
+// ---------------------------------------------------------------------------
+struct DevOp {
+    DevBuf diag_coef, diag_zy, flip_begin, flips, ent_begin, ent_coef, ent_zy;
+    bool ready = false;
+};
+
+struct rbmhip_op {
+    int S = 0, L = 0;
+    std::vector<cd> coeff;
+    std::vector<int32_t> types, idx;
+    // canonical tables
+    std::vector<cx<double>> diag_coef;
+    std::vector<uint64_t> diag_zy;
+    std::vector<int> flip_begin{0}, flips, ent_begin{0};
+    std::vector<cx<double>> ent_coef;
+    std::vector<uint64_t> ent_zy;
+    int max_index = -1;
+    bool hermitian = true;
+    std::map<int, DevOp> dev;
+    ~rbmhip_op() {
+        for (auto& kv : dev) {
+            DevOp& d = kv.second;
+            d.diag_coef.release(); d.diag_zy.release(); d.flip_begin.release(); d.flips.release();
+            d.ent_begin.release(); d.ent_coef.release(); d.ent_zy.release();
+        }
+    }
+};
+
+static void bit_set(uint64_t* w, int i) { w[i >> 6] ^= (uint64_t)1 << (i & 63); }
+
+// Canonicalisation of include/operator/Operator.hpp:45-88: X and Y flip the
+// spin, Y multiplies by -i s, Z by s, with s the ORIGINAL configuration.  The
+// string's element is coef (-i)^{#Y} prod_{i in zy} s_i with zy the XOR of
+// the Y and Z positions (s_i^2 = 1), and its target is s ^ (XOR of the X and
+// Y positions).
+static void canonicalise(rbmhip_op* op) {
+    struct Ent { uint64_t mask[2]; uint64_t zy[2]; cd c; };
+    std::vector<Ent> ents;
+    const int64_t table = (int64_t)op->S * op->L;
+    for (int n = 0; n < op->S; n++) {
+        Ent e{{0, 0}, {0, 0}, op->coeff[n]};
+        int ny = 0;
+        int64_t t = (int64_t)n * op->L;
+        for (;; t++) {
+            RBM_REQUIRE(t < table, "operator string " + std::to_string(n) + " has no -1 terminator before the end of the table");
+            const int pidx = op->idx[t];
+            if (pidx == -1) break;
+            const int pt = op->types[t];
+            if (pt == 0) continue;
+            RBM_REQUIRE(pidx >= 0 && pidx < 128, "Pauli index out of range [0, 128)");
+            op->max_index = std::max(op->max_index, pidx);
+            if (pt == 1) { bit_set(e.mask, pidx); }
+            else if (pt == 2) { bit_set(e.mask, pidx); bit_set(e.zy, pidx); ny++; }
+            else if (pt == 3) { bit_set(e.zy, pidx); }
+        }
+        static const cd minus_i_pow[4] = {cd(1, 0), cd(0, -1), cd(-1, 0), cd(0, 1)};
+        e.c *= minus_i_pow[ny & 3];
+        ents.push_back(e);
+    }
+    // group by flip mask (first-appearance order), diagonal separately
+    std::vector<std::pair<uint64_t, uint64_t>> masks;
+    std::vector<std::vector<Ent>> groups;
+    for (const Ent& e : ents) {
+        if (e.mask[0] == 0 && e.mask[1] == 0) {
+            op->diag_coef.push_back(mk(e.c.real(), e.c.imag()));
+            op->diag_zy.push_back(e.zy[0]);
+            op->diag_zy.push_back(e.zy[1]);
+            continue;
+        }
+        size_t g = 0;
+        for (; g < masks.size(); g++)
+            if (masks[g].first == e.mask[0] && masks[g].second == e.mask[1]) break;
+        if (g == masks.size()) { masks.push_back({e.mask[0], e.mask[1]}); groups.emplace_back(); }
+        groups[g].push_back(e);
+    }
+    for (size_t g = 0; g < masks.size(); g++) {
+        for (int i = 0; i < 128; i++)
+            if (((i < 64 ? masks[g].first >> i : masks[g].second >> (i - 64)) & 1u)) op->flips.push_back(i);
+        op->flip_begin.push_back((int)op->flips.size());
+        for (const Ent& e : groups[g]) {
+            op->ent_coef.push_back(mk(e.c.real(), e.c.imag()));
+            op->ent_zy.push_back(e.zy[0]);
+            op->ent_zy.push_back(e.zy[1]);
+        }
+        op->ent_begin.push_back((int)op->ent_coef.size());
+    }
+    // Hermiticity: within a flip mask m the element at s is sum_z C_z chi_z(s)
+    // with chi_z(s) = prod_{i in z} s_i; <s|H|s^m> = conj(<s^m|H|s>) for all s
+    // iff C_z = (-1)^{|z & m|} conj(C_z) for every zy pattern z.
+    auto check = [&](uint64_t m0, uint64_t m1, const std::vector<Ent>& es) {
+        std::vector<std::pair<std::pair<uint64_t, uint64_t>, cd>> acc;
+        double scale = 0.0;
+        for (const Ent& e : es) {
+            scale = std::max(scale, std::abs(e.c));
+            bool found = false;
+            for (auto& a : acc)
+                if (a.first.first == e.zy[0] && a.first.second == e.zy[1]) { a.second += e.c; found = true; break; }
+            if (!found) acc.push_back({{e.zy[0], e.zy[1]}, e.c});
+        }
+        for (auto& a : acc) {
+            const int par = (__builtin_popcountll(a.first.first & m0) + __builtin_popcountll(a.first.second & m1)) & 1;
+            const cd target = (par ? -1.0 : 1.0) * std::conj(a.second);
+            if (std::abs(a.second - target) > 1e-12 * std::max(1.0, scale)) op->hermitian = false;
+        }
+    };
+    std::vector<Ent> diag;
+    for (const Ent& e : ents)
+        if (e.mask[0] == 0 && e.mask[1] == 0) diag.push_back(e);
+    check(0, 0, diag);
+    for (size_t g = 0; g < masks.size(); g++) check(masks[g].first, masks[g].second, groups[g]);
+}
+
+static OpDev op_dev(Ctx& c, rbmhip_op* op) {
+    DevOp& d = op->dev[c.device];
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+        b.ensure(bytes);
+        if (bytes) RBM_HIPCHK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    if (!d.ready) {
+        up(d.diag_coef, op->diag_coef.data(), op->diag_coef.size() * sizeof(cx<double>));
+        up(d.diag_zy, op->diag_zy.data(), op->diag_zy.size() * sizeof(uint64_t));
+        up(d.flip_begin, op->flip_begin.data(), op->flip_begin.size() * sizeof(int));
+        up(d.flips, op->flips.data(), op->flips.size() * sizeof(int));
+        up(d.ent_begin, op->ent_begin.data(), op->ent_begin.size() * sizeof(int));
+        up(d.ent_coef, op->ent_coef.data(), op->ent_coef.size() * sizeof(cx<double>));
+        up(d.ent_zy, op->ent_zy.data(), op->ent_zy.size() * sizeof(uint64_t));
+        d.ready = true;
+    }
+    OpDev o;
+    o.n_diag = (int)op->diag_coef.size();
+    o.diag_coef = reinterpret_cast<const cx<double>*>(d.diag_coef.p);
+    o.diag_zy = reinterpret_cast<const uint64_t*>(d.diag_zy.p);
+    o.n_terms = (int)op->flip_begin.size() - 1;
+    o.flip_begin = reinterpret_cast<const int*>(d.flip_begin.p);
+    o.flips = reinterpret_cast<const int*>(d.flips.p);
+    o.ent_begin = reinterpret_cast<const int*>(d.ent_begin.p);
+    o.ent_coef = reinterpret_cast<const cx<double>*>(d.ent_coef.p);
+    o.ent_zy = reinterpret_cast<const uint64_t*>(d.ent_zy.p);
+    return o;
+}
+
+static OpDev empty_op() {
+    OpDev o;
+    std::memset(&o, 0, sizeof(o));
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// ensembles
+// ---------------------------------------------------------------------------
+struct rbmhip_ens {
+    int kind = 0;  // 0 ExactSummation, 1 MonteCarloLoop
+    int N = 0;
+    bool has_sector = false;
+    int sector = 0;
+    std::vector<uint64_t> configs;  // sector list, 2 words each
+    int64_t num_configs = 0;
+    std::map<int, DevBuf> configs_dev;
+    uint32_t num_samples = 0, num_sweeps = 0, num_therm = 0, num_chains = 0;
+    uint64_t seed = 0;
+    uint32_t call = 0;
+    uint64_t acc = 0, rej = 0;
+    // records of the last Monte Carlo call
+    int rec_device = -1;
+    int64_t rec_n = 0;
+    DevBuf spins, lp, el;
+    ~rbmhip_ens() {
+        for (auto& kv : configs_dev) kv.second.release();
+        spins.release(); lp.release(); el.release();
+    }
+    int64_t num_steps() const { return kind == 0 ? num_configs : (int64_t)num_samples; }
+};
+
+static const uint64_t* ens_configs(Ctx& c, rbmhip_ens* e) {
+    if (e->kind != 0 || !e->has_sector) return nullptr;
+    DevBuf& b = e->configs_dev[c.device];
+    if (b.p == nullptr) {
+        b.ensure(e->configs.size() * sizeof(uint64_t));
+        RBM_HIPCHK(hipMemcpy(b.p, e->configs.data(), e->configs.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    }
+    return reinterpret_cast<const uint64_t*>(b.p);
+}
+
+// ---------------------------------------------------------------------------
+// sampling / evaluation driver
+// ---------------------------------------------------------------------------
+struct Samples {
+    int64_t n = 0;
+    uint64_t* spins = nullptr;
+    cx<double>* lp = nullptr;
+    cx<double>* el = nullptr;
+    void* tanh = nullptr;
+};
+
+template <typename T>
+static bool rbm_w_in_lds(Ctx& c, const rbmhip_psi* psi) {
+    return (size_t)psi->N * psi->M * sizeof(cx<T>) <= c.max_lds;
+}
+
+static void check_op_fits(const rbmhip_psi* psi, const rbmhip_op* op) {
+    if (op) RBM_REQUIRE(op->max_index < psi->N, "operator acts on a spin index >= N");
+}
+
+// Evaluate log psi (and E_loc / tanh) on n given configurations (configs or
+// index_base + idx).
+template <typename T>
+static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const uint64_t* configs,
+                       uint64_t index_base, const Records& rec) {
+    const OpDev od = op ? op_dev(c, op) : empty_op();
+    const int max_blocks = c.num_cus * 8;
+    if (psi->kind == 0) {
+        const RbmDev<T> md = rbm_dev<T>(c, psi);
+        const bool wlds = rbm_w_in_lds<T>(c, psi);
+        const size_t lds = (size_t)psi->N * psi->M * sizeof(cx<T>);
+        c.launch("eval", [&] { return launch_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, wlds, lds, max_blocks, c.stream); });
+    } else {
+        const DeepDev<T> md = deep_dev<T>(c, psi);
+        c.launch("deep_eval", [&] { return launch_deep_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, max_blocks, c.stream); });
+    }
+}
+
+// Draw the samples of an ensemble for psi and measure them.  For
+// ExactSummation the samples are the basis (or sector) states.
+template <typename T>
+static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, bool want_spins, bool want_tanh) {
+    RBM_REQUIRE(e->N == psi->N || e->kind == 1, "ensemble and wavefunction have different N");
+    check_op_fits(psi, op);
+    Samples s;
+    s.n = e->num_steps();
+    RBM_REQUIRE(s.n > 0, "the ensemble has no samples");
+    const size_t tsz = psi->kind == 0 ? (size_t)psi->M : (size_t)psi->num_params() - 2 * (size_t)psi->N;
+    if (want_tanh) s.tanh = c.buf("tanh", (size_t)s.n * tsz * sizeof(cx<T>));
+    if (e->kind == 1) {
+        RBM_REQUIRE(e->num_chains > 0 && e->num_samples % e->num_chains == 0,
+                    "num_samples must be a positive multiple of num_markov_chains (MonteCarloLoop.cu:78)");
+        e->spins.ensure((size_t)s.n * 2 * sizeof(uint64_t));
+        e->lp.ensure((size_t)s.n * sizeof(cx<double>));
+        e->el.ensure((size_t)s.n * sizeof(cx<double>));
+        e->rec_device = c.device;
+        e->rec_n = s.n;
+        s.spins = reinterpret_cast<uint64_t*>(e->spins.p);
+        s.lp = reinterpret_cast<cx<double>*>(e->lp.p);
+        s.el = reinterpret_cast<cx<double>*>(e->el.p);
+        Records rec{s.spins, s.lp, op ? s.el : nullptr, s.tanh};
+        McParams mp;
+        mp.chains = (int)e->num_chains;
+        mp.steps_per_chain = (int)(e->num_samples / e->num_chains);
+        mp.sweeps = (int)e->num_sweeps;
+        mp.therm = (int)e->num_therm;
+        mp.seed_lo = (uint32_t)e->seed;
+        mp.seed_hi = (uint32_t)(e->seed >> 32);
+        mp.call = e->call++;
+        uint32_t* counts = reinterpret_cast<uint32_t*>(c.buf("counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
+        const OpDev od = op ? op_dev(c, op) : empty_op();
+        if (psi->kind == 0) {
+            const RbmDev<T> md = rbm_dev<T>(c, psi);
+            const bool wlds = rbm_w_in_lds<T>(c, psi);
+            const size_t lds = (size_t)psi->N * psi->M * sizeof(cx<T>);
+            c.launch("mc", [&] { return launch_mc<T>(md, od, op != nullptr, mp, rec, counts, wlds, lds, c.stream); });
+        } else {
+            const DeepDev<T> md = deep_dev<T>(c, psi);
+            c.launch("deep_mc", [&] { return launch_deep_mc<T>(md, od, op != nullptr, mp, rec, counts, c.stream); });
+        }
+        std::vector<uint32_t> h((size_t)mp.chains * 2);
+        c.d2h(h.data(), counts, h.size() * sizeof(uint32_t));
+        c.sync();
+        e->acc = e->rej = 0;
+        for (int i = 0; i < mp.chains; i++) { e->acc += h[2 * i]; e->rej += h[2 * i + 1]; }
+        if (!op) s.el = nullptr;
+    } else {
+        s.lp = reinterpret_cast<cx<double>*>(c.buf("lp", (size_t)s.n * sizeof(cx<double>)));
+        if (op) s.el = reinterpret_cast<cx<double>*>(c.buf("el", (size_t)s.n * sizeof(cx<double>)));
+        if (want_spins) s.spins = reinterpret_cast<uint64_t*>(c.buf("spins", (size_t)s.n * 2 * sizeof(uint64_t)));
+        Records rec{s.spins, s.lp, s.el, s.tanh};
+        eval_model<T>(c, psi, op, s.n, ens_configs(c, e), 0, rec);
+    }
+    return s;
+}
+
+struct Scalars {
+    double v[SC_STRIDE];
+};
+
+static Scalars reduce_scalars(Ctx& c, int64_t n, const double* sc) {
+    int nb = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    if (nb < 1) nb = 1;
+    double* part = reinterpret_cast<double*>(c.buf("scpart", (size_t)nb * SC_STRIDE * sizeof(double)));
+    c.launch("scalar_reduce", [&] { return launch_scalar_partials(n, sc, nb, part, c.stream); });
+    std::vector<double> h((size_t)nb * SC_STRIDE);
+    c.d2h(h.data(), part, h.size() * sizeof(double));
+    c.sync();
+    Scalars r;
+    for (int q = 0; q < SC_STRIDE; q++) {
+        double s = 0.0;
+        for (int b = 0; b < nb; b++) s += h[(size_t)b * SC_STRIDE + q];
+        r.v[q] = s;
+    }
+    return r;
+}
+
+// R[q][r][col] = sum_s coef_q[s] row_r(s) conj(col(s)) for the RBM rows
+// {1, s_0..s_{N-1}} and columns {T_0..T_{M-1}, 1}.
+template <typename T>
+static std::vector<cd> ok_contract(Ctx& c, int N, int M, int64_t n, const void* tanh, const uint64_t* spins,
+                                   const cx<double>* coef, int nc) {
+    const int64_t plen = (int64_t)nc * (N + 1) * (M + 1);
+    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (plen * 16)));
+    nb = std::min<int64_t>(nb, std::max<int64_t>(1, (n + 63) / 64));
+    const int64_t chunk = (n + nb - 1) / nb;
+    cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * plen) * sizeof(cx<double>)));
+    cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("okout", (size_t)plen * sizeof(cx<double>)));
+    c.launch("ok_reduce", [&] { return launch_ok_reduce<T>(n, N, M, tanh, spins, coef, nc, chunk, (int)nb, part, c.stream); });
+    c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, plen, part, out, c.stream); });
+    std::vector<cd> h((size_t)plen);
+    c.d2h(h.data(), out, h.size() * sizeof(cx<double>));
+    c.sync();
+    return h;
+}
+
+// Map the contraction R of one coefficient vector to parameter order
+// [alpha (N), beta (N), b (M), W (N x M), a (N)]:  entries are
+// sum_s coef_s conj(O_k(s)) (Psi.hpp:165-202).
+static void rbm_params_from_R(const rbmhip_psi* psi, const cd* R, std::vector<cd>& out) {
+    const int N = psi->N, M = psi->M, C = M + 1;
+    out.assign((size_t)psi->num_params(), cd(0, 0));
+    for (int j = 0; j < M; j++) out[2 * N + j] = R[j];
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < M; j++) out[2 * N + M + (size_t)i * M + j] = R[(size_t)(i + 1) * C + j];
+    if (psi->has_a)
+        for (int i = 0; i < N; i++) out[2 * N + M + (size_t)N * M + i] = R[(size_t)(i + 1) * C + M];
+}
+
+template <typename T>
+static std::vector<cd> contract_params(Ctx& c, rbmhip_psi* psi, int64_t n, const void* tanh, const uint64_t* spins,
+                                       const cx<double>* coef, int nc) {
+    if (psi->kind == 0) {
+        std::vector<cd> R = ok_contract<T>(c, psi->N, psi->M, n, tanh, spins, coef, nc);
+        const size_t plen1 = (size_t)(psi->N + 1) * (psi->M + 1);
+        std::vector<cd> all;
+        for (int q = 0; q < nc; q++) {
+            std::vector<cd> one;
+            rbm_params_from_R(psi, R.data() + q * plen1, one);
+            all.insert(all.end(), one.begin(), one.end());
+        }
+        return all;
+    }
+    // PsiDeep: O_k rows are materialised per sample (deep kernels) and reduced
+    // column-wise.
+    const int64_t K = psi->num_params() - 2 * (int64_t)psi->N;
+    const int64_t plen = (int64_t)nc * K;
+    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (plen * 16)));
+    nb = std::min<int64_t>(nb, std::max<int64_t>(1, (n + 63) / 64));
+    const int64_t chunk = (n + nb - 1) / nb;
+    cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * plen) * sizeof(cx<double>)));
+    cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("okout", (size_t)plen * sizeof(cx<double>)));
+    c.launch("ok_reduce", [&] { return launch_rows_reduce<T>(n, K, tanh, coef, nc, chunk, (int)nb, part, c.stream); });
+    c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, plen, part, out, c.stream); });
+    std::vector<cd> h((size_t)plen);
+    c.d2h(h.data(), out, h.size() * sizeof(cx<double>));
+    c.sync();
+    std::vector<cd> all;
+    const int64_t P = psi->num_params();
+    for (int q = 0; q < nc; q++) {
+        std::vector<cd> one((size_t)P, cd(0, 0));
+        for (int64_t k = 0; k < K; k++) one[(size_t)(2 * psi->N + k)] = h[(size_t)(q * K + k)];
+        all.insert(all.end(), one.begin(), one.end());
+    }
+    return all;
+}
+
+template <class F>
+static void with_precision(int precision, F&& f) {
+    if (precision == RBMHIP_FP64) f(double(0)); else f(float(0));
+}
+
+static void check_psi(const rbmhip_psi* p) { RBM_REQUIRE(p != nullptr, "null wavefunction handle"); }
+static void check_ens(const rbmhip_ens* e) { RBM_REQUIRE(e != nullptr, "null ensemble handle"); }
+
+template <typename T>
+static Scalars energy_scalars(Ctx& c, rbmhip_psi* psi, const Samples& s, rbmhip_ens* ens, cx<double>* coef) {
+    double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
+    CoefArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = s.n; a.mode = 0; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs;
+    a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.el = s.el; a.coef = coef; a.sc = sc;
+    c.launch("coef", [&] { return launch_coef(a, c.stream); });
+    return reduce_scalars(c, s.n, sc);
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* rbmhip_last_error(void) { return g_err.c_str(); }
+int rbmhip_version(void) { return 10000; }
+
+int rbmhip_set_device(int device) {
+    return guard([&] {
+        RBM_REQUIRE(device >= 0, "device index must be >= 0");
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_device = device;
+    });
+}
+
+int rbmhip_device_count(int* count) {
+    return guard([&] {
+        RBM_REQUIRE(count, "null output");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int rbmhip_start_profiling(void) { return guard([&] { (void)ctx(); }); }
+int rbmhip_stop_profiling(void) { return guard([&] { Ctx& c = ctx(); c.sync(); }); }
+
+int rbmhip_activation(const double* z, int64_t n, int derivative, double* out) {
+    return guard([&] {
+        RBM_REQUIRE(n >= 0 && (n == 0 || (z && out)), "bad arguments");
+        if (n == 0) return;
+        Ctx& c = ctx();
+        cx<double>* dz = reinterpret_cast<cx<double>*>(c.buf("act_in", (size_t)n * sizeof(cx<double>)));
+        cx<double>* dout = reinterpret_cast<cx<double>*>(c.buf("act_out", (size_t)n * sizeof(cx<double>)));
+        c.h2d(dz, z, (size_t)n * sizeof(cx<double>));
+        c.launch("activation", [&] { return launch_activation(n, dz, dout, derivative, c.stream); });
+        c.d2h(out, dout, (size_t)n * sizeof(cx<double>));
+        c.sync();
+    });
+}
+
+// ---- wavefunctions --------------------------------------------------------
+static void fill_cd(std::vector<cd>& v, const double* src, size_t n) {
+    v.resize(n);
+    for (size_t i = 0; i < n; i++) v[i] = cd(src[2 * i], src[2 * i + 1]);
+}
+
+int rbmhip_rbm_create(int N, int M, const double* b, const double* W, const double* a, double prefactor, int precision,
+                      rbmhip_psi** out) {
+    return guard([&] {
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(N >= 1 && N <= 128, "N must be in [1, 128]");
+        RBM_REQUIRE(M >= 1 && M <= max_hidden_units(), "M must be in [1, 256]");
+        RBM_REQUIRE(b && W, "b and W are required");
+        RBM_REQUIRE(precision == RBMHIP_FP32 || precision == RBMHIP_FP64, "precision must be 32 or 64");
+        auto p = std::make_unique<rbmhip_psi>();
+        p->kind = 0; p->N = N; p->M = M; p->precision = precision; p->prefactor = prefactor;
+        fill_cd(p->b, b, (size_t)M);
+        fill_cd(p->W, W, (size_t)N * M);
+        if (a) { p->has_a = true; fill_cd(p->a, a, (size_t)N); }
+        *out = p.release();
+    });
+}
+
+int rbmhip_rbm_set_params(rbmhip_psi* psi, const double* b, const double* W, const double* a) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(psi->kind == 0, "not an RBM");
+        if (b) fill_cd(psi->b, b, (size_t)psi->M);
+        if (W) fill_cd(psi->W, W, (size_t)psi->N * psi->M);
+        if (a) { RBM_REQUIRE(psi->has_a, "this RBM has no visible bias"); fill_cd(psi->a, a, (size_t)psi->N); }
+        psi->version++;
+    });
+}
+
+int rbmhip_deep_create(int N, int num_layers, const int32_t* sizes, const int32_t* conn_counts, const uint32_t* conns,
+                       const double* weights, const double* biases, const double* final_weights, double prefactor,
+                       int translational_invariance, int ref_compat_final, int precision, rbmhip_psi** out) {
+    return guard([&] {
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(N >= 1 && N <= 128, "N must be in [1, 128]");
+        RBM_REQUIRE(num_layers >= 1 && num_layers <= DEEP_MAX_LAYERS, "1 to 4 layers are supported");
+        RBM_REQUIRE(sizes && conn_counts && conns && weights && biases && final_weights, "null layer tables");
+        RBM_REQUIRE(precision == RBMHIP_FP32 || precision == RBMHIP_FP64, "precision must be 32 or 64");
+        auto p = std::make_unique<rbmhip_psi>();
+        p->kind = 1; p->N = N; p->precision = precision; p->prefactor = prefactor;
+        p->translational_invariance = translational_invariance ? 1 : 0;
+        p->ref_compat_final = ref_compat_final ? 1 : 0;
+        int64_t nw = 0, nb = 0;
+        int prev = N, units = 0;
+        for (int l = 0; l < num_layers; l++) {
+            RBM_REQUIRE(sizes[l] >= 1 && sizes[l] <= DEEP_MAX_WIDTH, "layer width must be in [1, 256]");
+            RBM_REQUIRE(conn_counts[l] >= 1 && conn_counts[l] <= prev, "connectivity must be in [1, previous width]");
+            p->sizes.push_back(sizes[l]);
+            p->conn_counts.push_back(conn_counts[l]);
+            for (int64_t k = 0; k < (int64_t)conn_counts[l] * sizes[l]; k++) {
+                const uint32_t u = conns[nw + k];
+                RBM_REQUIRE((int)u < prev, "connection index out of range of the previous layer");
+            }
+            nw += (int64_t)conn_counts[l] * sizes[l];
+            nb += sizes[l];
+            prev = sizes[l];
+            units += sizes[l];
+        }
+        RBM_REQUIRE(units <= DEEP_MAX_UNITS, "too many units in total");
+        p->conns.assign(conns, conns + nw);
+        // Transpose of layer l+1's connections (the reference's rhs tables,
+        // source/quantum_state/PsiDeep.cu:86-117, same visiting order): for unit
+        // i of layer l, the slots c * size_{l+1} + j of layer l+1 that read it.
+        int64_t woff = 0;
+        for (int l = 0; l < num_layers; l++) {
+            const int64_t next_off = woff + (int64_t)conn_counts[l] * sizes[l];
+            p->rb_off.push_back((int)p->rhs_begin.size());
+            if (l + 1 < num_layers) {
+                const int size = sizes[l], nsize = sizes[l + 1], nC = conn_counts[l + 1];
+                std::vector<std::vector<int>> lists((size_t)size);
+                for (int j = 0; j < nsize; j++)
+                    for (int c = 0; c < nC; c++)
+                        lists[conns[next_off + (int64_t)c * nsize + j]].push_back(c * nsize + j);
+                for (int i = 0; i < size; i++) {
+                    p->rhs_begin.push_back((int)p->rhs_entries.size());
+                    p->rhs_entries.insert(p->rhs_entries.end(), lists[i].begin(), lists[i].end());
+                }
+                p->rhs_begin.push_back((int)p->rhs_entries.size());
+            }
+            woff = next_off;
+        }
+        fill_cd(p->weights, weights, (size_t)nw);
+        fill_cd(p->biases, biases, (size_t)nb);
+        fill_cd(p->final_weights, final_weights, (size_t)sizes[num_layers - 1]);
+        *out = p.release();
+    });
+}
+
+int rbmhip_deep_set_params(rbmhip_psi* psi, const double* weights, const double* biases, const double* final_weights) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(psi->kind == 1, "not a PsiDeep");
+        if (weights) fill_cd(psi->weights, weights, psi->weights.size());
+        if (biases) fill_cd(psi->biases, biases, psi->biases.size());
+        if (final_weights) fill_cd(psi->final_weights, final_weights, psi->final_weights.size());
+        psi->version++;
+    });
+}
+
+int rbmhip_deep_set_translational_invariance(rbmhip_psi* psi, int on) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(psi->kind == 1, "not a PsiDeep");
+        psi->translational_invariance = on ? 1 : 0;
+    });
+}
+
+int rbmhip_psi_set_prefactor(rbmhip_psi* psi, double prefactor) {
+    return guard([&] { check_psi(psi); psi->prefactor = prefactor; });
+}
+
+int rbmhip_psi_num_params(const rbmhip_psi* psi, int64_t* out) {
+    return guard([&] { check_psi(psi); RBM_REQUIRE(out, "null output"); *out = psi->num_params(); });
+}
+
+int rbmhip_psi_destroy(rbmhip_psi* psi) {
+    return guard([&] { delete psi; });
+}
+
+// ---- operators --------------------------------------------------------------
+int rbmhip_op_create(int S, int L, const double* coefficients, const int32_t* types, const int32_t* indices,
+                     rbmhip_op** out) {
+    return guard([&] {
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(S >= 0 && L >= 0, "negative table shape");
+        RBM_REQUIRE(S == 0 || (coefficients && types && indices && L > 0), "null operator tables");
+        auto op = std::make_unique<rbmhip_op>();
+        op->S = S; op->L = L;
+        fill_cd(op->coeff, coefficients, (size_t)S);
+        op->types.assign(types, types + (int64_t)S * L);
+        op->idx.assign(indices, indices + (int64_t)S * L);
+        canonicalise(op.get());
+        *out = op.release();
+    });
+}
+
+int rbmhip_op_info(const rbmhip_op* op, int* num_diagonal, int* num_flip_masks, int* hermitian, int* max_index) {
+    return guard([&] {
+        RBM_REQUIRE(op, "null operator");
+        if (num_diagonal) *num_diagonal = (int)op->diag_coef.size();
+        if (num_flip_masks) *num_flip_masks = (int)op->flip_begin.size() - 1;
+        if (hermitian) *hermitian = op->hermitian ? 1 : 0;
+        if (max_index) *max_index = op->max_index;
+    });
+}
+
+int rbmhip_op_destroy(rbmhip_op* op) {
+    return guard([&] { delete op; });
+}
+
+// ---- ensembles ----------------------------------------------------------------
+int rbmhip_exact_create(int N, rbmhip_ens** out) {
+    return guard([&] {
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(N >= 1 && N <= 40, "ExactSummation supports 1 <= N <= 40");
+        auto e = std::make_unique<rbmhip_ens>();
+        e->kind = 0; e->N = N; e->num_configs = (int64_t)1 << N;
+        *out = e.release();
+    });
+}
+
+int rbmhip_exact_set_total_z(rbmhip_ens* ens, int sector) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 0, "not an ExactSummation");
+        RBM_REQUIRE(ens->N <= 32, "sector enumeration supports N <= 32");
+        // ExactSummation.cu:26-45: keep basis indices whose total S_z
+        // (2 popcount - N) equals the sector, in increasing order.
+        ens->configs.clear();
+        const uint64_t dim = (uint64_t)1 << ens->N;
+        for (uint64_t idx = 0; idx < dim; idx++)
+            if (2 * __builtin_popcountll(idx) - ens->N == sector) { ens->configs.push_back(idx); ens->configs.push_back(0); }
+        ens->num_configs = (int64_t)(ens->configs.size() / 2);
+        ens->has_sector = true;
+        ens->sector = sector;
+        for (auto& kv : ens->configs_dev) kv.second.release();
+        ens->configs_dev.clear();
+    });
+}
+
+int rbmhip_mc_create(uint32_t num_samples, uint32_t num_sweeps, uint32_t num_thermalization_sweeps,
+                     uint32_t num_markov_chains, uint64_t seed, rbmhip_ens** out) {
+    return guard([&] {
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(num_markov_chains > 0, "num_markov_chains must be > 0");
+        RBM_REQUIRE(num_samples % num_markov_chains == 0,
+                    "num_samples must be a multiple of num_markov_chains (MonteCarloLoop.cu:78)");
+        auto e = std::make_unique<rbmhip_ens>();
+        e->kind = 1;
+        e->num_samples = num_samples; e->num_sweeps = num_sweeps; e->num_therm = num_thermalization_sweeps;
+        e->num_chains = num_markov_chains; e->seed = seed;
+        *out = e.release();
+    });
+}
+
+int rbmhip_mc_copy(const rbmhip_ens* other, rbmhip_ens** out) {
+    return guard([&] {
+        RBM_REQUIRE(other && other->kind == 1 && out, "bad arguments");
+        auto e = std::make_unique<rbmhip_ens>();
+        e->kind = 1;
+        e->num_samples = other->num_samples; e->num_sweeps = other->num_sweeps; e->num_therm = other->num_therm;
+        e->num_chains = other->num_chains; e->seed = other->seed;
+        e->has_sector = other->has_sector; e->sector = other->sector;
+        *out = e.release();
+    });
+}
+
+int rbmhip_mc_set_total_z(rbmhip_ens* ens, int sector) {
+    return guard([&] {
+        check_ens(ens);
+        (void)sector;
+        throw Error(RBMHIP_ERR_UNSUPPORTED, "MonteCarloLoop.set_total_z_symmetry (pair moves) is not implemented yet");
+    });
+}
+
+int rbmhip_mc_set_fast_sweep(rbmhip_ens* ens, uint32_t num_tries) {
+    return guard([&] {
+        check_ens(ens);
+        (void)num_tries;
+        throw Error(RBMHIP_ERR_UNSUPPORTED, "MonteCarloLoop.set_fast_sweep is not implemented yet");
+    });
+}
+
+int rbmhip_ens_num_steps(const rbmhip_ens* ens, uint64_t* out) {
+    return guard([&] { check_ens(ens); RBM_REQUIRE(out, "null output"); *out = (uint64_t)ens->num_steps(); });
+}
+
+int rbmhip_mc_acceptance(const rbmhip_ens* ens, uint64_t* acceptances, uint64_t* rejections) {
+    return guard([&] {
+        check_ens(ens);
+        if (acceptances) *acceptances = ens->acc;
+        if (rejections) *rejections = ens->rej;
+    });
+}
+
+int rbmhip_mc_last_samples(const rbmhip_ens* ens, uint64_t* spins, double* log_psi, double* e_loc) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 1 && ens->rec_n > 0, "no Monte Carlo samples recorded yet");
+        Ctx& c = ctx();
+        RBM_REQUIRE(c.device == ens->rec_device, "samples were recorded on another device");
+        if (spins) c.d2h(spins, ens->spins.p, (size_t)ens->rec_n * 2 * sizeof(uint64_t));
+        if (log_psi) c.d2h(log_psi, ens->lp.p, (size_t)ens->rec_n * sizeof(cx<double>));
+        if (e_loc) c.d2h(e_loc, ens->el.p, (size_t)ens->rec_n * sizeof(cx<double>));
+        c.sync();
+    });
+}
+
+int rbmhip_ens_destroy(rbmhip_ens* ens) {
+    return guard([&] { delete ens; });
+}
+
+// ---- batched evaluation ---------------------------------------------------------
+int rbmhip_log_psi(rbmhip_psi* psi, const uint64_t* spins, int64_t n, double* out) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(n >= 0 && (n == 0 || (spins && out)), "bad arguments");
+        if (n == 0) return;
+        Ctx& c = ctx();
+        uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("in_spins", (size_t)n * 2 * sizeof(uint64_t)));
+        cx<double>* lp = reinterpret_cast<cx<double>*>(c.buf("lp", (size_t)n * sizeof(cx<double>)));
+        c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Records rec{nullptr, lp, nullptr, nullptr};
+            eval_model<T>(c, psi, nullptr, n, ds, 0, rec);
+        });
+        c.d2h(out, lp, (size_t)n * sizeof(cx<double>));
+        c.sync();
+    });
+}
+
+int rbmhip_local_energy(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, int64_t n, double* out) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(op, "null operator");
+        RBM_REQUIRE(n >= 0 && (n == 0 || (spins && out)), "bad arguments");
+        check_op_fits(psi, op);
+        if (n == 0) return;
+        Ctx& c = ctx();
+        uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("in_spins", (size_t)n * 2 * sizeof(uint64_t)));
+        cx<double>* lp = reinterpret_cast<cx<double>*>(c.buf("lp", (size_t)n * sizeof(cx<double>)));
+        cx<double>* el = reinterpret_cast<cx<double>*>(c.buf("el", (size_t)n * sizeof(cx<double>)));
+        c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Records rec{nullptr, lp, el, nullptr};
+            eval_model<T>(c, psi, op, n, ds, 0, rec);
+        });
+        c.d2h(out, el, (size_t)n * sizeof(cx<double>));
+        c.sync();
+    });
+}
+
+int rbmhip_o_k_vector(rbmhip_psi* psi, const uint64_t* spins, double* out) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(spins && out, "bad arguments");
+        Ctx& c = ctx();
+        const int64_t P = psi->num_params();
+        uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("in_spins", 2 * sizeof(uint64_t)));
+        c.h2d(ds, spins, 2 * sizeof(uint64_t));
+        std::vector<cd> o((size_t)P, cd(0, 0));
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            const size_t K = psi->kind == 0 ? (size_t)psi->M : (size_t)(P - 2 * psi->N);
+            void* t = c.buf("tanh", K * sizeof(cx<T>));
+            cx<double>* lp = reinterpret_cast<cx<double>*>(c.buf("lp", sizeof(cx<double>)));
+            Records rec{nullptr, lp, nullptr, t};
+            eval_model<T>(c, psi, nullptr, 1, ds, 0, rec);
+            std::vector<cx<T>> h(K);
+            c.d2h(h.data(), t, K * sizeof(cx<T>));
+            c.sync();
+            if (psi->kind == 0) {
+                // O_k = tanh(theta_j) for b_j, tanh(theta_j) s_i for W_ij, s_i for a_i
+                const int N = psi->N, M = psi->M;
+                uint64_t s0 = spins[0], s1 = spins[1];
+                mask_words(s0, s1, N);
+                for (int j = 0; j < M; j++) o[2 * N + j] = cd(h[j].re, h[j].im);
+                for (int i = 0; i < N; i++) {
+                    const double si = spin_of<double>(s0, s1, i);
+                    for (int j = 0; j < M; j++) o[2 * N + M + (size_t)i * M + j] = si * cd(h[j].re, h[j].im);
+                }
+                if (psi->has_a)
+                    for (int i = 0; i < N; i++) o[2 * N + M + (size_t)N * M + i] = spin_of<double>(s0, s1, i);
+            } else {
+                for (size_t k = 0; k < K; k++) o[2 * psi->N + k] = cd(h[k].re, h[k].im);
+            }
+        });
+        std::memcpy(out, o.data(), sizeof(cd) * o.size());
+    });
+}
+
+int rbmhip_psi_vector(rbmhip_psi* psi, double* out) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(psi->N <= 30, "psi_vector supports N <= 30");
+        Ctx& c = ctx();
+        const int64_t n = (int64_t)1 << psi->N;
+        cx<double>* lp = reinterpret_cast<cx<double>*>(c.buf("lp", (size_t)n * sizeof(cx<double>)));
+        cx<double>* v = reinterpret_cast<cx<double>*>(c.buf("vec", (size_t)n * sizeof(cx<double>)));
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Records rec{nullptr, lp, nullptr, nullptr};
+            eval_model<T>(c, psi, nullptr, n, nullptr, 0, rec);
+        });
+        // PsiVector.cu:36-39: psi(s) = exp(log prefactor + log psi(s))
+        c.launch("exp_vector", [&] { return launch_exp_vector(n, lp, std::log(psi->prefactor), v, c.stream); });
+        c.d2h(out, v, (size_t)n * sizeof(cx<double>));
+        c.sync();
+    });
+}
+
+int rbmhip_psi_norm(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
+    return guard([&] {
+        check_psi(psi);
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 0, "psi_norm needs an ExactSummation");
+        RBM_REQUIRE(out, "null output");
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, nullptr, ens, false, false);
+            int nb = (int)std::min<int64_t>(1024, (s.n + 255) / 256);
+            double* part = reinterpret_cast<double*>(c.buf("lsepart", (size_t)nb * 2 * sizeof(double)));
+            c.launch("lse", [&] { return launch_lse_partials(s.n, s.lp, std::log(psi->prefactor), nb, part, c.stream); });
+            std::vector<double> h((size_t)nb * 2);
+            c.d2h(h.data(), part, h.size() * sizeof(double));
+            c.sync();
+            double m = -INFINITY;
+            for (int b = 0; b < nb; b++) m = std::max(m, h[2 * b]);
+            double sum = 0.0;
+            for (int b = 0; b < nb; b++)
+                if (h[2 * b] != -INFINITY) sum += h[2 * b + 1] * std::exp(h[2 * b] - m);
+            // sqrt(sum_s exp(x_s)) with x_s = 2 (log prefactor + Re log psi)
+            *out = std::exp(0.5 * m) * std::sqrt(sum);
+        });
+    });
+}
+
+// ---- network functions ------------------------------------------------------------
+int rbmhip_expectation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* out) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(op && out, "bad arguments");
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, op, ens, false, false);
+            Scalars r = energy_scalars<T>(c, psi, s, ens, nullptr);
+            const double inv = 1.0 / (double)ens->num_steps();
+            out[0] = r.v[1] * inv;
+            out[1] = r.v[2] * inv;
+        });
+    });
+}
+
+int rbmhip_expectation_list(rbmhip_psi* psi, rbmhip_op* const* ops, int n_ops, rbmhip_ens* ens, double* out) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(n_ops >= 0 && (n_ops == 0 || (ops && out)), "bad arguments");
+        for (int i = 0; i < n_ops; i++) { RBM_REQUIRE(ops[i], "null operator in list"); check_op_fits(psi, ops[i]); }
+        if (n_ops == 0) return;
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            // one pass of the ensemble, then every operator on the same samples
+            // (ExpectationValue.cu:448-518)
+            Samples s = sample<T>(c, psi, nullptr, ens, true, false);
+            const uint64_t* configs = ens->kind == 1 ? s.spins : ens_configs(c, ens);
+            cx<double>* el = reinterpret_cast<cx<double>*>(c.buf("el_list", (size_t)s.n * sizeof(cx<double>)));
+            cx<double>* lp2 = reinterpret_cast<cx<double>*>(c.buf("lp_list", (size_t)s.n * sizeof(cx<double>)));
+            const double inv = 1.0 / (double)ens->num_steps();
+            for (int i = 0; i < n_ops; i++) {
+                Records rec{nullptr, lp2, el, nullptr};
+                eval_model<T>(c, psi, ops[i], s.n, configs, 0, rec);
+                Samples si = s;
+                si.el = el;
+                Scalars r = energy_scalars<T>(c, psi, si, ens, nullptr);
+                out[2 * i] = r.v[1] * inv;
+                out[2 * i + 1] = r.v[2] * inv;
+            }
+        });
+    });
+}
+
+int rbmhip_fluctuation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* fluctuation, double* expectation) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(op && fluctuation && expectation, "bad arguments");
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, op, ens, false, false);
+            Scalars r = energy_scalars<T>(c, psi, s, ens, nullptr);
+            const double inv = 1.0 / (double)ens->num_steps();
+            const double Er = r.v[1] * inv, Ei = r.v[2] * inv, E2 = r.v[3] * inv;
+            expectation[0] = Er;
+            expectation[1] = Ei;
+            *fluctuation = std::sqrt(E2 - (Er * Er + Ei * Ei));
+        });
+    });
+}
+
+int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad, double* expectation) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(op && grad && expectation, "bad arguments");
+        if (!op->hermitian)
+            throw Error(RBMHIP_ERR_UNSUPPORTED,
+                        "gradient of a non-Hermitian operator (the reference's E_k term) is not implemented yet");
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, op, ens, true, true);
+            cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * 2 * sizeof(cx<double>)));
+            Scalars r = energy_scalars<T>(c, psi, s, ens, coef);
+            std::vector<cd> R = contract_params<T>(c, psi, s.n, s.tanh, s.spins, coef, 2);
+            const int64_t P = psi->num_params();
+            const double inv = 1.0 / (double)ens->num_steps();
+            const cd E(r.v[1] * inv, r.v[2] * inv);
+            // ExpectationValue.cu:243-253 with conj<E_k> = <E conj(O_k)> (Hermitian):
+            // g_k = 2 <E conj O_k> - 2 <E> conj<O_k>, where R0 = sum w conj(O_k)
+            // and R1 = sum w E conj(O_k).
+            for (int64_t k = 0; k < P; k++) {
+                const cd R0 = R[(size_t)k] * inv, R1 = R[(size_t)(P + k)] * inv;
+                const cd g = 2.0 * R1 - 2.0 * E * R0;
+                grad[2 * k] = g.real();
+                grad[2 * k + 1] = g.imag();
+            }
+            expectation[0] = E.real();
+            expectation[1] = E.imag();
+        });
+    });
+}
+
+int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(psi->kind == 0, "get_S_matrix is bound for Psi only (main.cpp:557-560)");
+        Ctx& c = ctx();
+        const int64_t P = psi->num_params();
+        RBM_REQUIRE(P <= 8192, "S matrix larger than 8192 x 8192");
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, nullptr, ens, true, true);
+            cx<double>* S = reinterpret_cast<cx<double>*>(c.buf("smat", (size_t)(P * P) * sizeof(cx<double>)));
+            const double inv = 1.0 / (double)ens->num_steps();
+            c.launch("s_matrix", [&] { return launch_s_matrix_ref<T>(s.n, psi->N, psi->M, psi->has_a, s.tanh, s.spins, inv, S, c.stream); });
+            c.d2h(out, S, (size_t)(P * P) * sizeof(cx<double>));
+            c.sync();
+        });
+    });
+}
+
+int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
+                       int want_grad, double nu, double* distance, double* grad) {
+    return guard([&] {
+        check_psi(psi); check_psi(psi_prime); check_ens(ens);
+        RBM_REQUIRE(op && distance && (!want_grad || grad), "bad arguments");
+        RBM_REQUIRE(psi->N == psi_prime->N, "psi and psi_prime have different N");
+        Ctx& c = ctx();
+        Samples s;
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            s = sample<T>(c, psi, op, ens, true, false);
+        });
+        const uint64_t* configs = ens->kind == 1 ? s.spins : (ens->has_sector ? ens_configs(c, ens) : nullptr);
+        // HilbertSpaceDistance.cu:40-44: log psi'(s) on the same samples (and tanh for O_k(psi'))
+        cx<double>* lp2 = reinterpret_cast<cx<double>*>(c.buf("lp2", (size_t)s.n * sizeof(cx<double>)));
+        void* tanh2 = nullptr;
+        uint64_t* spins2 = nullptr;
+        with_precision(psi_prime->precision, [&](auto tag) {
+            using T = decltype(tag);
+            if (want_grad) {
+                const size_t K = psi_prime->kind == 0 ? (size_t)psi_prime->M
+                                                      : (size_t)(psi_prime->num_params() - 2 * psi_prime->N);
+                tanh2 = c.buf("tanh2", (size_t)s.n * K * sizeof(cx<T>));
+                if (ens->kind == 0 && !s.spins) spins2 = reinterpret_cast<uint64_t*>(c.buf("spins", (size_t)s.n * 2 * sizeof(uint64_t)));
+            }
+            Records rec{spins2, lp2, nullptr, tanh2};
+            eval_model<T>(c, psi_prime, nullptr, s.n, configs, 0, rec);
+        });
+        const uint64_t* spins_for_grad = s.spins ? s.spins : spins2;
+        double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
+        cx<double>* coef = want_grad ? reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * 2 * sizeof(cx<double>))) : nullptr;
+        CoefArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.n = s.n; a.mode = 1; a.exact = ens->kind == 0; a.is_unitary = is_unitary ? 1 : 0;
+        a.num_configs = (double)ens->num_configs; a.log_prefactor = std::log(psi->prefactor);
+        a.lp = s.lp; a.el = s.el; a.lp2 = lp2; a.coef = coef; a.sc = sc;
+        c.launch("coef", [&] { return launch_coef(a, c.stream); });
+        Scalars r = reduce_scalars(c, s.n, sc);
+        const double inv = 1.0 / (double)ens->num_steps();
+        const cd omega(r.v[0] * inv, r.v[1] * inv);
+        const double ratio = r.v[2] * inv, nrm = r.v[3] * inv;
+        // HilbertSpaceDistance.cu:216-220, 248-262
+        const double u = std::norm(omega);
+        const double v = nrm * ratio;
+        const double d = std::sqrt(1.0 - u / v);
+        *distance = d;
+        if (want_grad) {
+            std::vector<cd> R;
+            with_precision(psi_prime->precision, [&](auto tag) {
+                using T = decltype(tag);
+                R = contract_params<T>(c, psi_prime, s.n, tanh2, spins_for_grad, coef, 2);
+            });
+            const int64_t P = psi_prime->num_params();
+            const double pref = std::pow(d, (double)(float)nu);
+            for (int64_t k = 0; k < P; k++) {
+                const cd oO = R[(size_t)k] * inv, rO = R[(size_t)(P + k)] * inv;
+                const cd ukp = std::conj(omega) * oO;
+                const cd vkp = nrm * rO;
+                const cd g = (-(ukp * v - u * vkp) / (v * v)) / pref;
+                grad[2 * k] = g.real();
+                grad[2 * k + 1] = g.imag();
+            }
+        }
+    });
+}
+
+// ---- timing -------------------------------------------------------------------------
+int rbmhip_timing_enable(int on) {
+    return guard([&] { Ctx& c = ctx(); c.sync(); c.timing = on != 0; });
+}
+
+int rbmhip_timing_reset(void) {
+    return guard([&] { Ctx& c = ctx(); c.sync(); c.totals.clear(); });
+}
+
+int rbmhip_timing_query(const char* name, double* total_ms, int64_t* launches) {
+    return guard([&] {
+        RBM_REQUIRE(name, "null name");
+        Ctx& c = ctx();
+        c.sync();
+        auto it = c.totals.find(name);
+        if (total_ms) *total_ms = it == c.totals.end() ? 0.0 : it->second.first;
+        if (launches) *launches = it == c.totals.end() ? 0 : it->second.second;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// Device-side primitives for the gfx950 VMC kernels: a minimal complex type,
+// the reference's rational log-cosh approximant and its derivative, the
+// Philox4x32-10 counter-based generator, spin-word helpers and 16-lane DPP
+// reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RBM_DI __device__ __forceinline__
+#define RBM_HDI __host__ __device__ __forceinline__
+
+namespace rbmhip {
+
+template <typename T>
+struct cx {
+    T re, im;
+};
+
+template <typename T> RBM_HDI cx<T> mk(T r, T i) { cx<T> z; z.re = r; z.im = i; return z; }
+template <typename T> RBM_HDI cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename T> RBM_HDI cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.re - b.re, a.im - b.im); }
+template <typename T> RBM_HDI cx<T> operator*(cx<T> a, cx<T> b) {
+    return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename T> RBM_HDI cx<T> operator*(T s, cx<T> a) { return mk(s * a.re, s * a.im); }
+template <typename T> RBM_HDI cx<T> operator+(cx<T> a, T s) { return mk(a.re + s, a.im); }
+template <typename T> RBM_HDI cx<T> operator-(cx<T> a, T s) { return mk(a.re - s, a.im); }
+template <typename T> RBM_HDI cx<T> conj(cx<T> a) { return mk(a.re, -a.im); }
+template <typename T> RBM_HDI cx<double> to_d(cx<T> a) { return mk<double>((double)a.re, (double)a.im); }
+
+RBM_DI float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+RBM_DI double rcp(double x) { return 1.0 / x; }
+
+// a / b computed as a * conj(b) / |b|^2
+template <typename T> RBM_DI cx<T> cdiv(cx<T> a, cx<T> b) {
+    const T r = rcp(b.re * b.re + b.im * b.im);
+    return mk((a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r);
+}
+
+RBM_DI cx<float> cexp(cx<float> z) {
+    float s, c;
+    sincosf(z.im, &s, &c);
+    const float e = expf(z.re);
+    return mk(e * c, e * s);
+}
+RBM_DI cx<double> cexp(cx<double> z) {
+    double s, c;
+    sincos(z.im, &s, &c);
+    const double e = exp(z.re);
+    return mk(e * c, e * s);
+}
+
+// Shared denominator of the approximant:
+// 9.19376335670885 + z (s 10.218.. + z (7.7714.. + z (s 3.7466.. + z)))
+// (include/quantum_state/psi_functions.hpp:56, 103).
+template <typename T> RBM_DI cx<T> lc_den(cx<T> z, T sg, T c1) {
+    cx<T> d = z + sg * T(3.746646023906276);
+    d = z * d + T(7.771429504240965);
+    d = z * d + sg * c1;
+    d = z * d + T(9.19376335670885);
+    return d;
+}
+
+// my_logcosh: include/quantum_state/psi_functions.hpp:49-57.  The branch
+// follows the reference exactly: sign = Re z > 0 ? 1 : -1.
+template <typename T> RBM_DI cx<T> logcosh(cx<T> z) {
+    const T sg = z.re > T(0) ? T(1) : T(-1);
+    const cx<T> d = lc_den(z, sg, T(10.2180213465));
+    const cx<T> n = mk(T(5.49914721954) - sg * T(2.16564366435) * z.re, -sg * T(2.16564366435) * z.im);
+    const cx<T> q = cdiv(n, d);
+    const T k = sg * T(0.9003320053750442);
+    return mk(k * z.re + q.re - T(0.598139), k * z.im + q.im);
+}
+
+// Real part of my_logcosh only (the Metropolis test needs Re log psi).
+template <typename T> RBM_DI T logcosh_re(cx<T> z) {
+    const T sg = z.re > T(0) ? T(1) : T(-1);
+    const cx<T> d = lc_den(z, sg, T(10.2180213465));
+    const T nr = T(5.49914721954) - sg * T(2.16564366435) * z.re;
+    const T ni = -sg * T(2.16564366435) * z.im;
+    const T qr = (nr * d.re + ni * d.im) * rcp(d.re * d.re + d.im * d.im);
+    return sg * T(0.9003320053750442) * z.re + qr - T(0.598139);
+}
+
+// my_tanh: include/quantum_state/psi_functions.hpp:97-114.
+template <typename T> RBM_DI cx<T> dlogcosh(cx<T> z) {
+    const T sg = z.re > T(0) ? T(1) : T(-1);
+    const cx<T> d = lc_den(z, sg, T(10.218021346543315));
+    cx<T> p = mk(T(6.746450656267947) + sg * T(0.9003320053750442) * z.re, sg * T(0.9003320053750442) * z.im);
+    p = z * p + sg * T(26.632014683761202);
+    p = z * p + T(70.82878897882324);
+    p = z * p + sg * T(146.36284300074402);
+    p = z * p + T(199.24474920889975);
+    p = z * p + sg * T(177.6769746361748);
+    p = z * p + T(83.68563506532087);
+    p = z * p;
+    return cdiv(p, d * d);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  One call yields 128 bits.
+// ---------------------------------------------------------------------------
+struct u32x4 { uint32_t x, y, z, w; };
+
+RBM_HDI u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1; c3 = (uint32_t)p0; c0 = n0; c2 = n2;
+    }
+    u32x4 r; r.x = c0; r.y = c1; r.z = c2; r.w = c3;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Spin words: spin i = +1 iff bit i is set (include/Spins.h:125-129).  Two
+// 64-bit words cover N <= 128.
+// ---------------------------------------------------------------------------
+RBM_HDI int bit_of(uint64_t s0, uint64_t s1, int i) {
+    return (int)(((i < 64) ? (s0 >> i) : (s1 >> (i - 64))) & 1u);
+}
+template <typename T> RBM_HDI T spin_of(uint64_t s0, uint64_t s1, int i) {
+    return bit_of(s0, s1, i) ? T(1) : T(-1);
+}
+RBM_HDI void flip_bit(uint64_t& s0, uint64_t& s1, int i) {
+    if (i < 64) s0 ^= (uint64_t)1 << i; else s1 ^= (uint64_t)1 << (i - 64);
+}
+RBM_HDI void mask_words(uint64_t& s0, uint64_t& s1, int N) {
+    if (N < 64) { s0 &= ((uint64_t)1 << N) - 1u; s1 = 0; }
+    else if (N == 64) { s1 = 0; }
+    else if (N < 128) { s1 &= ((uint64_t)1 << (N - 64)) - 1u; }
+}
+
+// (-1)^popcount(zy & ~s): the product of s_i over the bits of zy.
+RBM_HDI double zy_sign(uint64_t z0, uint64_t z1, uint64_t s0, uint64_t s1) {
+    const int par = (__builtin_popcountll(z0 & ~s0) + __builtin_popcountll(z1 & ~s1)) & 1;
+    return par ? -1.0 : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// 16-lane reductions with DPP row rotations.  Every lane of the row ends with
+// the same bits: at each level a lane adds its partner's partial, and the two
+// partners add the same two operands (the sum is commutative), so the whole
+// row agrees on the accept/reject decision.
+// ---------------------------------------------------------------------------
+template <int CTRL> RBM_DI float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> RBM_DI double dpp(double v) {
+    const int2 w = *reinterpret_cast<int2*>(&v);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, 0xF, 0xF, false);
+    r.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, 0xF, 0xF, false);
+    return *reinterpret_cast<double*>(&r);
+}
+
+// row_ror:8, 4, 2, 1 (dpp_ctrl 0x120 + n)
+template <typename T> RBM_DI T row_sum16(T v) {
+    v += dpp<0x128>(v);
+    v += dpp<0x124>(v);
+    v += dpp<0x122>(v);
+    v += dpp<0x121>(v);
+    return v;
+}
+template <typename T> RBM_DI cx<T> row_sum16(cx<T> v) {
+    v.re += dpp<0x128>(v.re); v.im += dpp<0x128>(v.im);
+    v.re += dpp<0x124>(v.re); v.im += dpp<0x124>(v.im);
+    v.re += dpp<0x122>(v.re); v.im += dpp<0x122>(v.im);
+    v.re += dpp<0x121>(v.re); v.im += dpp<0x121>(v.im);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Operator tables after host canonicalisation.  A Pauli string becomes
+// (flip mask, coefficient * (-i)^{#Y}, zy mask): its matrix element at s is
+// coef * prod_{i in zy} s_i and it maps s to s ^ mask
+// (include/operator/Operator.hpp:45-88).  Strings are grouped by flip mask;
+// mask 0 is the diagonal part.
+// ---------------------------------------------------------------------------
+struct OpDev {
+    int n_diag;
+    const cx<double>* diag_coef;   // n_diag
+    const uint64_t* diag_zy;       // 2 * n_diag
+    int n_terms;                   // off-diagonal flip masks
+    const int* flip_begin;         // n_terms + 1
+    const int* flips;              // flip positions of each term
+    const int* ent_begin;          // n_terms + 1
+    const cx<double>* ent_coef;    // entries
+    const uint64_t* ent_zy;        // 2 * entries
+};
+
+RBM_DI cx<double> diag_energy(const OpDev& op, uint64_t s0, uint64_t s1) {
+    cx<double> e = mk(0.0, 0.0);
+    for (int d = 0; d < op.n_diag; d++) {
+        const double sg = zy_sign(op.diag_zy[2 * d], op.diag_zy[2 * d + 1], s0, s1);
+        e = e + sg * op.diag_coef[d];
+    }
+    return e;
+}
+
+RBM_DI cx<double> term_coef(const OpDev& op, int t, uint64_t s0, uint64_t s1) {
+    cx<double> c = mk(0.0, 0.0);
+    for (int e = op.ent_begin[t]; e < op.ent_begin[t + 1]; e++) {
+        const double sg = zy_sign(op.ent_zy[2 * e], op.ent_zy[2 * e + 1], s0, s1);
+        c = c + sg * op.ent_coef[e];
+    }
+    return c;
+}
+
+}  // namespace rbmhip

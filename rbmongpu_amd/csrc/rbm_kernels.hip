@@ -1,0 +1,558 @@
+// gfx950 kernels for the RBM wavefunction (reference Psi):
+//   k_mc_rbm    Metropolis single-spin-flip chains with fused measurement
+//   k_eval_rbm  log psi / E_loc / tanh(theta) for given configurations
+//               (ExactSummation, psi_vector, psi_O_k_vector, batched calls)
+//   k_ok_reduce + k_partials_sum   FP64 O_k contractions
+//   k_coef, k_scalar_partials, k_lse_partials   per-sample FP64 reductions
+//
+// Execution model: a chain (or a basis state) is owned by a 16-lane DPP row;
+// a wavefront holds four chains and a 256-thread workgroup sixteen.  Hidden
+// unit j lives on lane j % 16 in slot j / 16, so theta_j stays in VGPRs and a
+// row of W (contiguous in LDS) is read conflict-free.  Sums over hidden units
+// are row rotations (DPP), never LDS round trips or __syncthreads.
+#include "rbm_device.hpp"
+#include "rbm_kernels.hpp"
+
+namespace rbmhip {
+
+// ---------------------------------------------------------------------------
+// local energy (include/operator/Operator.hpp:90-115, restated on canonical
+// flip-mask terms): E_loc(s) = diag(s) + sum_t c_t(s) exp(log psi(s^m_t) - log psi(s)).
+// Each lane adds its hidden units' lc(theta') - lc(theta); the row sums.
+// ---------------------------------------------------------------------------
+template <typename T, int JPL>
+RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const OpDev& op,
+                                   uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], const cx<T> (&lcv)[JPL],
+                                   int lane) {
+    const int M = md.M;
+    cx<double> E = diag_energy(op, s0, s1);
+    for (int t = 0; t < op.n_terms; t++) {
+        const int fb = op.flip_begin[t], fe = op.flip_begin[t + 1];
+        cx<T> part = mk(T(0), T(0));
+#pragma unroll
+        for (int u = 0; u < JPL; u++) {
+            const int j = lane + 16 * u;
+            if (j < M) {
+                cx<T> v = th[u];
+                for (int q = fb; q < fe; q++) {
+                    const int pos = op.flips[q];
+                    const T f = T(-2) * spin_of<T>(s0, s1, pos);
+                    v = v + f * Wsrc[pos * M + j];
+                }
+                part = part + (logcosh(v) - lcv[u]);
+            }
+        }
+        cx<T> d = row_sum16(part);
+        if (md.a) {
+            for (int q = fb; q < fe; q++) {
+                const int pos = op.flips[q];
+                d = d + (T(-2) * spin_of<T>(s0, s1, pos)) * md.a[pos];
+            }
+        }
+        const cx<double> c = term_coef(op, t, s0, s1);
+        E = E + c * to_d(cexp(d));
+    }
+    return E;
+}
+
+template <typename T>
+RBM_DI cx<T> visible_sum(const RbmDev<T>& md, uint64_t s0, uint64_t s1) {
+    cx<T> v = mk(T(0), T(0));
+    if (md.a)
+        for (int i = 0; i < md.N; i++) v = v + spin_of<T>(s0, s1, i) * md.a[i];
+    return v;
+}
+
+// theta_j = b_j + sum_i W_ij s_i  (include/quantum_state/Psi.hpp:43-53)
+template <typename T, int JPL>
+RBM_DI void init_angles(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, uint64_t s0, uint64_t s1,
+                        int lane, cx<T> (&th)[JPL]) {
+    const int N = md.N, M = md.M;
+#pragma unroll
+    for (int u = 0; u < JPL; u++) {
+        const int j = lane + 16 * u;
+        th[u] = mk(T(0), T(0));
+        if (j < M) {
+            cx<T> v = md.b[j];
+            for (int i = 0; i < N; i++) v = v + spin_of<T>(s0, s1, i) * Wsrc[i * M + j];
+            th[u] = v;
+        }
+    }
+}
+
+template <typename T, int JPL>
+RBM_DI void measure(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const OpDev& op, int has_op,
+                    uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], cx<T> vis, int lane, int64_t sample,
+                    const Records& rec) {
+    const int M = md.M;
+    cx<T> lcv[JPL];
+    cx<T> lsum = mk(T(0), T(0));
+#pragma unroll
+    for (int u = 0; u < JPL; u++) {
+        const int j = lane + 16 * u;
+        lcv[u] = j < M ? logcosh(th[u]) : mk(T(0), T(0));
+        lsum = lsum + lcv[u];
+    }
+    const cx<double> lp = row_sum16(to_d(lsum)) + to_d(vis);
+    cx<double> E = mk(0.0, 0.0);
+    if (has_op) E = local_energy_rbm<T, JPL>(md, Wsrc, op, s0, s1, th, lcv, lane);
+    if (lane == 0) {
+        if (rec.spins) { rec.spins[2 * sample] = s0; rec.spins[2 * sample + 1] = s1; }
+        if (rec.log_psi) rec.log_psi[sample] = lp;
+        if (rec.e_loc) rec.e_loc[sample] = E;
+    }
+    if (rec.tanh) {
+        cx<T>* Tr = reinterpret_cast<cx<T>*>(rec.tanh) + sample * (int64_t)M;
+#pragma unroll
+        for (int u = 0; u < JPL; u++) {
+            const int j = lane + 16 * u;
+            if (j < M) Tr[j] = dlogcosh(th[u]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Metropolis chains: include/spin_ensembles/MonteCarloLoop.hpp:55-256.
+// Per call: random start, therm*N proposals, then for each of the
+// steps_per_chain samples sweeps*N proposals followed by a measurement of
+// sample index step * chains + chain.  Proposal k of a chain draws
+// Philox(seed; chain, k >> 1, call, 0): words (x, y) for even k, (z, w) for
+// odd k; position = x % N, u = (y + 0.5) 2^-32; accept iff ratio > 1 or
+// u <= ratio with ratio = |psi(s')/psi(s)|^2.
+// ---------------------------------------------------------------------------
+template <typename T, int JPL, bool WLDS>
+__global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, OpDev op, int has_op, McParams mp,
+                                                Records rec, uint32_t* __restrict__ counts) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int N = md.N, M = md.M;
+    const cx<T>* Wsrc = md.W;
+    if (WLDS) {
+        cx<T>* Wl = reinterpret_cast<cx<T>*>(smem_raw);
+        for (int k = threadIdx.x; k < N * M; k += blockDim.x) Wl[k] = md.W[k];
+        __syncthreads();
+        Wsrc = Wl;
+    }
+    const int lane = threadIdx.x & 15;
+    const int chain = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);
+    if (chain >= mp.chains) return;
+
+    const u32x4 r0 = philox((uint32_t)chain, 0u, mp.call, 1u, mp.seed_lo, mp.seed_hi);
+    uint64_t s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
+    uint64_t s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
+    mask_words(s0, s1, N);
+
+    cx<T> th[JPL];
+    init_angles<T, JPL>(md, Wsrc, s0, s1, lane, th);
+    T lcr[JPL];
+#pragma unroll
+    for (int u = 0; u < JPL; u++) lcr[u] = (lane + 16 * u) < M ? logcosh_re(th[u]) : T(0);
+    cx<T> vis = visible_sum(md, s0, s1);
+
+    uint32_t nacc = 0, nrej = 0;
+    u32x4 rr = r0;
+    uint64_t k = 0;
+    const uint64_t n_therm = (uint64_t)mp.therm * N;
+    const uint64_t n_sweep = (uint64_t)mp.sweeps * N;
+
+    auto propose = [&](uint64_t kk) {
+        if ((kk & 1) == 0) rr = philox((uint32_t)chain, (uint32_t)(kk >> 1), mp.call, 0u, mp.seed_lo, mp.seed_hi);
+        const uint32_t x = (kk & 1) ? rr.z : rr.x;
+        const uint32_t y = (kk & 1) ? rr.w : rr.y;
+        const int p = (int)(x % (uint32_t)N);
+        const T f = T(-2) * spin_of<T>(s0, s1, p);
+        cx<T> thn[JPL];
+        T lrn[JPL];
+        T part = T(0);
+#pragma unroll
+        for (int u = 0; u < JPL; u++) {
+            const int j = lane + 16 * u;
+            thn[u] = th[u];
+            lrn[u] = lcr[u];
+            if (j < M) {
+                thn[u] = th[u] + f * Wsrc[p * M + j];
+                lrn[u] = logcosh_re(thn[u]);
+                part += lrn[u] - lcr[u];
+            }
+        }
+        T delta = row_sum16(part);
+        cx<T> ap = mk(T(0), T(0));
+        if (md.a) { ap = md.a[p]; delta += f * ap.re; }
+        const T ratio = (T)exp(T(2) * delta);
+        const T uu = (T)(((double)y + 0.5) * (1.0 / 4294967296.0));
+        const bool accept = (ratio > T(1)) || (uu <= ratio);
+        if (accept) {
+#pragma unroll
+            for (int u = 0; u < JPL; u++) { th[u] = thn[u]; lcr[u] = lrn[u]; }
+            flip_bit(s0, s1, p);
+            vis = vis + f * ap;
+            nacc++;
+        } else {
+            nrej++;
+        }
+    };
+
+    for (uint64_t i = 0; i < n_therm; i++) propose(k++);
+    for (int step = 0; step < mp.steps_per_chain; step++) {
+        for (uint64_t i = 0; i < n_sweep; i++) propose(k++);
+        const int64_t sample = (int64_t)step * mp.chains + chain;
+        measure<T, JPL>(md, Wsrc, op, has_op, s0, s1, th, vis, lane, sample, rec);
+    }
+    if (lane == 0 && counts) {
+        counts[2 * chain] = nacc;
+        counts[2 * chain + 1] = nrej;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation of given configurations (ExactSummation::kernel_foreach,
+// include/spin_ensembles/ExactSummation.hpp:36-84, and the batched entry
+// points).  Configuration idx is configs[idx] or, without a list,
+// index_base + idx masked to N bits (Spins(index, N), Spins.h:230-237).
+// Grid-strides over groups of 16 configurations per workgroup.
+// ---------------------------------------------------------------------------
+template <typename T, int JPL, bool WLDS>
+__global__ __launch_bounds__(256) void k_eval_rbm(RbmDev<T> md, OpDev op, int has_op, int64_t n,
+                                                  const uint64_t* __restrict__ configs, uint64_t index_base,
+                                                  Records rec) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int N = md.N, M = md.M;
+    const cx<T>* Wsrc = md.W;
+    if (WLDS) {
+        cx<T>* Wl = reinterpret_cast<cx<T>*>(smem_raw);
+        for (int k = threadIdx.x; k < N * M; k += blockDim.x) Wl[k] = md.W[k];
+        __syncthreads();
+        Wsrc = Wl;
+    }
+    const int lane = threadIdx.x & 15;
+    const int groups = blockDim.x >> 4;
+    for (int64_t idx = (int64_t)blockIdx.x * groups + (threadIdx.x >> 4); idx < n;
+         idx += (int64_t)gridDim.x * groups) {
+        uint64_t s0, s1;
+        if (configs) { s0 = configs[2 * idx]; s1 = configs[2 * idx + 1]; }
+        else { s0 = index_base + (uint64_t)idx; s1 = 0; }
+        mask_words(s0, s1, N);
+        cx<T> th[JPL];
+        init_angles<T, JPL>(md, Wsrc, s0, s1, lane, th);
+        const cx<T> vis = visible_sum(md, s0, s1);
+        measure<T, JPL>(md, Wsrc, op, has_op, s0, s1, th, vis, lane, idx, rec);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-sample coefficients and scalar records (FP64).
+//   weight w = exact ? num_configs * exp(2 (log prefactor + Re log psi)) : 1
+//     (ExactSummation.hpp:75, Psi.hpp:133-136)
+//   mode 0 (energy):   coef0 = w, coef1 = w E;  sc = {w, w E, w |E|^2}
+//   mode 1 (distance): HilbertSpaceDistance.cu:83-105 with log psi' in lp2:
+//     coef0 = w omega, coef1 = w ratio; sc = {w omega, w ratio, w norm}
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_coef(CoefArgs a) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)gridDim.x * blockDim.x) {
+        const cx<double> lp = a.lp[s];
+        const double w = a.exact ? a.num_configs * exp(2.0 * (a.log_prefactor + lp.re)) : 1.0;
+        const cx<double> E = a.el ? a.el[s] : mk(0.0, 0.0);
+        double* sc = a.sc + s * SC_STRIDE;
+        if (a.mode == 0) {
+            if (a.coef) {
+                a.coef[s] = mk(w, 0.0);
+                a.coef[a.n + s] = w * E;
+            }
+            sc[0] = w; sc[1] = w * E.re; sc[2] = w * E.im; sc[3] = w * (E.re * E.re + E.im * E.im);
+        } else {
+            const cx<double> lpp = a.lp2[s];
+            const cx<double> dl = conj(lpp - lp);
+            cx<double> omega;
+            double nrm;
+            if (a.is_unitary) {
+                omega = cexp(dl) * E;
+                nrm = E.re * E.re + E.im * E.im;
+            } else {
+                omega = cexp(E + dl);
+                nrm = exp(2.0 * E.re);
+            }
+            const double ratio = exp(2.0 * (lpp.re - lp.re));
+            if (a.coef) {
+                a.coef[s] = w * omega;
+                a.coef[a.n + s] = mk(w * ratio, 0.0);
+            }
+            sc[0] = w * omega.re; sc[1] = w * omega.im; sc[2] = w * ratio; sc[3] = w * nrm;
+        }
+    }
+}
+
+// Block partial sums of the SC_STRIDE scalar records, fixed order.
+__global__ __launch_bounds__(256) void k_scalar_partials(int64_t n, const double* __restrict__ sc,
+                                                         double* __restrict__ partials) {
+    __shared__ double red[SC_STRIDE][256];
+    double acc[SC_STRIDE];
+#pragma unroll
+    for (int q = 0; q < SC_STRIDE; q++) acc[q] = 0.0;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int q = 0; q < SC_STRIDE; q++) acc[q] += sc[s * SC_STRIDE + q];
+#pragma unroll
+    for (int q = 0; q < SC_STRIDE; q++) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < SC_STRIDE; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < SC_STRIDE) partials[blockIdx.x * SC_STRIDE + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Log-sum-exp partials of x_s = 2 (log prefactor + Re log psi_s): per block
+// (max, sum exp(x - max)).  Used for the norm (PsiNorm.cu:13-44) without
+// overflow.
+__global__ __launch_bounds__(256) void k_lse_partials(int64_t n, const cx<double>* __restrict__ lp,
+                                                      double log_prefactor, double* __restrict__ partials) {
+    __shared__ double rm[256], rs[256];
+    double m = -INFINITY, sum = 0.0;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+        const double x = 2.0 * (log_prefactor + lp[s].re);
+        if (x > m) { sum = sum * exp(m - x) + 1.0; m = x; }
+        else sum += exp(x - m);
+    }
+    rm[threadIdx.x] = m; rs[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double m1 = rm[threadIdx.x], m2 = rm[threadIdx.x + w];
+            const double s1 = rs[threadIdx.x], s2 = rs[threadIdx.x + w];
+            const double mm = m1 > m2 ? m1 : m2;
+            rm[threadIdx.x] = mm;
+            rs[threadIdx.x] = (m1 == -INFINITY ? 0.0 : s1 * exp(m1 - mm)) + (m2 == -INFINITY ? 0.0 : s2 * exp(m2 - mm));
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = rm[0]; partials[2 * blockIdx.x + 1] = rs[0]; }
+}
+
+// ---------------------------------------------------------------------------
+// O_k contractions.  With rows r = {1, s_0 .. s_{N-1}} and columns
+// c = {T_0 .. T_{M-1}, 1} (T_j = my_tanh(theta_j)):
+//   R[q][r][c] = sum_s coef_q[s] * row_r(s) * conj(col_c(s))
+// covers sum w conj(O_b), sum w conj(O_W), sum w O_a and sum w (c = M, r = 0).
+// Each block reduces a contiguous chunk of samples in FP64 and writes its
+// partials; k_partials_sum adds blocks in fixed order (deterministic).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_ok_reduce(int64_t n, int N, int M, const cx<T>* __restrict__ Trec,
+                                                   const uint64_t* __restrict__ spins,
+                                                   const cx<double>* __restrict__ coef, int nc, int64_t chunk,
+                                                   cx<double>* __restrict__ partials) {
+    constexpr int RB = OK_ROWS_PER_THREAD;
+    const int R = N + 1, C = M + 1;
+    const int nrb = (R + RB - 1) / RB;
+    const int nwork = nrb * C;
+    const int64_t s_begin = (int64_t)blockIdx.x * chunk;
+    const int64_t s_end = s_begin + chunk < n ? s_begin + chunk : n;
+    const int64_t plen = (int64_t)nc * R * C;
+    for (int w0 = 0; w0 < nwork; w0 += blockDim.x) {
+        const int w = w0 + threadIdx.x;
+        if (w >= nwork) break;
+        const int col = w % C, rb = w / C;
+        cx<double> acc[2][RB];
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int r = 0; r < RB; r++) acc[q][r] = mk(0.0, 0.0);
+        for (int64_t s = s_begin; s < s_end; s++) {
+            const cx<double> tv = col < M ? conj(to_d(Trec[s * M + col])) : mk(1.0, 0.0);
+            const uint64_t s0 = spins[2 * s], s1 = spins[2 * s + 1];
+            const cx<double> y0 = coef[s] * tv;
+            const cx<double> y1 = nc > 1 ? coef[n + s] * tv : mk(0.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                const int row = rb * RB + r;
+                const double sg = row == 0 ? 1.0 : spin_of<double>(s0, s1, row - 1);
+                acc[0][r] = acc[0][r] + sg * y0;
+                acc[1][r] = acc[1][r] + sg * y1;
+            }
+        }
+        for (int q = 0; q < nc; q++)
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                const int row = rb * RB + r;
+                if (row < R) partials[(int64_t)blockIdx.x * plen + ((int64_t)q * R + row) * C + col] = acc[q][r];
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_partials_sum(int nblocks, int64_t len, const cx<double>* __restrict__ partials,
+                                                      cx<double>* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+        cx<double> s = mk(0.0, 0.0);
+        for (int b = 0; b < nblocks; b++) s = s + partials[(int64_t)b * len + i];
+        out[i] = s;
+    }
+}
+
+// get_S_matrix (source/network_functions/S_matrix.cu:10-58, CPU semantics):
+// S[k][k'] = inv * sum_s O_k(s) conj(O_k'(s)), unweighted and uncentred;
+// rows/columns below 2N (alpha, beta) stay zero.
+template <typename T>
+__global__ __launch_bounds__(256) void k_s_matrix_ref(int64_t n, int N, int M, int has_a,
+                                                      const cx<T>* __restrict__ Trec,
+                                                      const uint64_t* __restrict__ spins, double inv,
+                                                      cx<double>* __restrict__ S) {
+    const int64_t P = 2 * (int64_t)N + M + (int64_t)N * M + (has_a ? N : 0);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < P * P; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = e / P - 2 * N, kp = e % P - 2 * N;
+        cx<double> acc = mk(0.0, 0.0);
+        if (k >= 0 && kp >= 0) {
+            for (int64_t s = 0; s < n; s++) {
+                const uint64_t s0 = spins[2 * s], s1 = spins[2 * s + 1];
+                const cx<T>* Tr = Trec + s * M;
+                auto ok = [&](int64_t kk) -> cx<double> {
+                    if (kk < M) return to_d(Tr[kk]);
+                    if (kk < M + (int64_t)N * M) {
+                        const int64_t i = (kk - M) / M, j = (kk - M) % M;
+                        return spin_of<double>(s0, s1, (int)i) * to_d(Tr[j]);
+                    }
+                    return mk(spin_of<double>(s0, s1, (int)(kk - M - (int64_t)N * M)), 0.0);
+                };
+                acc = acc + ok(k) * conj(ok(kp));
+            }
+            acc = inv * acc;
+        }
+        S[e] = acc;
+    }
+}
+
+__global__ void k_activation(int64_t n, const cx<double>* __restrict__ z, cx<double>* __restrict__ out, int deriv) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = deriv ? dlogcosh(z[i]) : logcosh(z[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers.  JPL (hidden units per lane) is a compile-time
+// parameter so theta stays in registers; dispatch picks the smallest JPL with
+// 16 * JPL >= M.
+// ---------------------------------------------------------------------------
+#define RBM_JPL_LIST(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12) X(13) X(16)
+
+static int pick_jpl(int M) {
+    static const int list[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 16};
+    for (int j : list) if (16 * j >= M) return j;
+    return -1;
+}
+
+int max_hidden_units() { return 16 * 16; }
+
+template <typename T>
+hipError_t launch_mc(const RbmDev<T>& md, const OpDev& op, int has_op, const McParams& mp, const Records& rec,
+                     uint32_t* counts, bool wlds, size_t lds_bytes, hipStream_t stream) {
+    const int jpl = pick_jpl(md.M);
+    const dim3 block(256), grid((mp.chains + 15) / 16);
+    const size_t shm = wlds ? lds_bytes : 0;
+#define RBM_MC_CASE(J)                                                                                   \
+    case J:                                                                                              \
+        if (wlds) {                                                                                      \
+            if (shm > 65536)                                                                             \
+                (void)hipFuncSetAttribute((const void*)&k_mc_rbm<T, J, true>,                            \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);          \
+            hipLaunchKernelGGL((k_mc_rbm<T, J, true>), grid, block, shm, stream, md, op, has_op, mp, rec, counts); \
+        }                                                                                                \
+        else hipLaunchKernelGGL((k_mc_rbm<T, J, false>), grid, block, 0, stream, md, op, has_op, mp, rec, counts);       \
+        break;
+    switch (jpl) {
+        RBM_JPL_LIST(RBM_MC_CASE)
+        default: return hipErrorInvalidValue;
+    }
+#undef RBM_MC_CASE
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_eval(const RbmDev<T>& md, const OpDev& op, int has_op, int64_t n, const uint64_t* configs,
+                       uint64_t index_base, const Records& rec, bool wlds, size_t lds_bytes, int max_blocks,
+                       hipStream_t stream) {
+    const int jpl = pick_jpl(md.M);
+    int64_t blocks = (n + 15) / 16;
+    if (blocks > max_blocks) blocks = max_blocks;
+    if (blocks < 1) blocks = 1;
+    const dim3 block(256), grid((unsigned)blocks);
+    const size_t shm = wlds ? lds_bytes : 0;
+#define RBM_EVAL_CASE(J)                                                                                              \
+    case J:                                                                                                           \
+        if (wlds) {                                                                                                   \
+            if (shm > 65536)                                                                                          \
+                (void)hipFuncSetAttribute((const void*)&k_eval_rbm<T, J, true>,                                       \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                       \
+            hipLaunchKernelGGL((k_eval_rbm<T, J, true>), grid, block, shm, stream, md, op, has_op, n, configs, index_base, rec); \
+        }                                                                                                             \
+        else hipLaunchKernelGGL((k_eval_rbm<T, J, false>), grid, block, 0, stream, md, op, has_op, n, configs, index_base, rec);       \
+        break;
+    switch (jpl) {
+        RBM_JPL_LIST(RBM_EVAL_CASE)
+        default: return hipErrorInvalidValue;
+    }
+#undef RBM_EVAL_CASE
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ok_reduce(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
+                            int nc, int64_t chunk, int nblocks, cx<double>* partials, hipStream_t stream) {
+    hipLaunchKernelGGL((k_ok_reduce<T>), dim3(nblocks), dim3(256), 0, stream, n, N, M,
+                       reinterpret_cast<const cx<T>*>(Trec), spins, coef, nc, chunk, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_partials_sum(int nblocks, int64_t len, const cx<double>* partials, cx<double>* out, hipStream_t stream) {
+    int64_t g = (len + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_partials_sum, dim3((unsigned)g), dim3(256), 0, stream, nblocks, len, partials, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_coef(const CoefArgs& a, hipStream_t stream) {
+    int64_t g = (a.n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_coef, dim3((unsigned)g), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scalar_partials(int64_t n, const double* sc, int nblocks, double* partials, hipStream_t stream) {
+    hipLaunchKernelGGL(k_scalar_partials, dim3(nblocks), dim3(256), 0, stream, n, sc, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_lse_partials(int64_t n, const cx<double>* lp, double log_prefactor, int nblocks, double* partials,
+                               hipStream_t stream) {
+    hipLaunchKernelGGL(k_lse_partials, dim3(nblocks), dim3(256), 0, stream, n, lp, log_prefactor, partials);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_s_matrix_ref(int64_t n, int N, int M, int has_a, const void* Trec, const uint64_t* spins, double inv,
+                               cx<double>* S, hipStream_t stream) {
+    const int64_t P = 2 * (int64_t)N + M + (int64_t)N * M + (has_a ? N : 0);
+    int64_t g = (P * P + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL((k_s_matrix_ref<T>), dim3((unsigned)g), dim3(256), 0, stream, n, N, M, has_a,
+                       reinterpret_cast<const cx<T>*>(Trec), spins, inv, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, int deriv, hipStream_t stream) {
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_activation, dim3((unsigned)g), dim3(256), 0, stream, n, z, out, deriv);
+    return hipGetLastError();
+}
+
+template hipError_t launch_mc<float>(const RbmDev<float>&, const OpDev&, int, const McParams&, const Records&, uint32_t*, bool, size_t, hipStream_t);
+template hipError_t launch_mc<double>(const RbmDev<double>&, const OpDev&, int, const McParams&, const Records&, uint32_t*, bool, size_t, hipStream_t);
+template hipError_t launch_eval<float>(const RbmDev<float>&, const OpDev&, int, int64_t, const uint64_t*, uint64_t, const Records&, bool, size_t, int, hipStream_t);
+template hipError_t launch_eval<double>(const RbmDev<double>&, const OpDev&, int, int64_t, const uint64_t*, uint64_t, const Records&, bool, size_t, int, hipStream_t);
+template hipError_t launch_ok_reduce<float>(int64_t, int, int, const void*, const uint64_t*, const cx<double>*, int, int64_t, int, cx<double>*, hipStream_t);
+template hipError_t launch_ok_reduce<double>(int64_t, int, int, const void*, const uint64_t*, const cx<double>*, int, int64_t, int, cx<double>*, hipStream_t);
+template hipError_t launch_s_matrix_ref<float>(int64_t, int, int, int, const void*, const uint64_t*, double, cx<double>*, hipStream_t);
+template hipError_t launch_s_matrix_ref<double>(int64_t, int, int, int, const void*, const uint64_t*, double, cx<double>*, hipStream_t);
+
+}  // namespace rbmhip

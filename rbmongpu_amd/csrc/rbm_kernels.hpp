@@ -1,0 +1,76 @@
+// Kernel-side data views and host launchers shared by rbm_kernels.hip and
+// the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rbm_device.hpp"
+
+namespace rbmhip {
+
+// Device view of an RBM: W row-major N x M, b (M), optional visible bias a (N).
+template <typename T>
+struct RbmDev {
+    int N, M;
+    const cx<T>* W;
+    const cx<T>* b;
+    const cx<T>* a;
+};
+
+// Per-sample outputs; any pointer may be null.
+struct Records {
+    uint64_t* spins;      // 2 words per sample
+    cx<double>* log_psi;  // per sample
+    cx<double>* e_loc;    // per sample
+    void* tanh;           // cx<T> per sample x M
+};
+
+struct McParams {
+    int chains;
+    int steps_per_chain;
+    int sweeps;
+    int therm;
+    uint32_t seed_lo, seed_hi, call;
+};
+
+constexpr int SC_STRIDE = 4;
+constexpr int OK_ROWS_PER_THREAD = 8;
+
+struct CoefArgs {
+    int64_t n;
+    int mode;            // 0 energy, 1 Hilbert-space distance
+    int exact;           // ExactSummation weights
+    int is_unitary;
+    double num_configs;
+    double log_prefactor;
+    const cx<double>* lp;
+    const cx<double>* el;
+    const cx<double>* lp2;
+    cx<double>* coef;    // 2 x n, may be null
+    double* sc;          // SC_STRIDE x n
+};
+
+int max_hidden_units();
+
+template <typename T>
+hipError_t launch_mc(const RbmDev<T>& md, const OpDev& op, int has_op, const McParams& mp, const Records& rec,
+                     uint32_t* counts, bool wlds, size_t lds_bytes, hipStream_t stream);
+template <typename T>
+hipError_t launch_eval(const RbmDev<T>& md, const OpDev& op, int has_op, int64_t n, const uint64_t* configs,
+                       uint64_t index_base, const Records& rec, bool wlds, size_t lds_bytes, int max_blocks,
+                       hipStream_t stream);
+template <typename T>
+hipError_t launch_ok_reduce(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
+                            int nc, int64_t chunk, int nblocks, cx<double>* partials, hipStream_t stream);
+hipError_t launch_partials_sum(int nblocks, int64_t len, const cx<double>* partials, cx<double>* out, hipStream_t stream);
+hipError_t launch_coef(const CoefArgs& a, hipStream_t stream);
+hipError_t launch_scalar_partials(int64_t n, const double* sc, int nblocks, double* partials, hipStream_t stream);
+hipError_t launch_lse_partials(int64_t n, const cx<double>* lp, double log_prefactor, int nblocks, double* partials,
+                               hipStream_t stream);
+template <typename T>
+hipError_t launch_s_matrix_ref(int64_t n, int N, int M, int has_a, const void* Trec, const uint64_t* spins, double inv,
+                               cx<double>* S, hipStream_t stream);
+hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, int deriv, hipStream_t stream);
+
+}  // namespace rbmhip
