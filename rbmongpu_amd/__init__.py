@@ -19,11 +19,12 @@ try:
         Psi, PsiDeep, Operator, Spins, MonteCarloLoop, ExactSummation, ExpectationValue,
         HilbertSpaceDistance, psi_O_k_vector, get_S_matrix, activation_function, activation_derivative,
         setDevice, start_profiling, stop_profiling, device_count, timing_enable, timing_reset, timing_query,
+        log_psi_batch, local_energy_batch,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(
         "rbmongpu_amd: the compiled HIP module _pyRBMonGPU is missing; build it with "
-        "`python -m rbmongpu_amd.build` (needs hipcc for gfx950)") from exc
+        "`python rbmongpu_amd/build.py` (needs hipcc for gfx950)") from exc
 
 from .json_numpy import NumpyEncoder, NumpyDecoder  # noqa: E402
 from .new_neural_network import new_neural_network, new_deep_neural_network, complex_noise, real_noise  # noqa: E402,F401

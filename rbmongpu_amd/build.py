@@ -3,7 +3,8 @@
   librbmhip.so       hipcc --offload-arch=gfx950: kernels + C ABI (include/rbmhip.h)
   _pyRBMonGPU*.so    g++: pybind11 module over the C ABI, rpath $ORIGIN
 
-Run `python -m rbmongpu_amd.build` (or __graft_entry__.build()).  Objects go to
+Run `python rbmongpu_amd/build.py` (or __graft_entry__.build(); the script is loaded by path so
+the package itself, which needs the built module, is not imported).  Objects go to
 rbmongpu_amd/_build/; the two shared libraries land in rbmongpu_amd/.
 """
 from __future__ import annotations

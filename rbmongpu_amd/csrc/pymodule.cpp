@@ -137,7 +137,6 @@ struct PsiDeep : PsiBase {
     }
     std::unique_ptr<PsiDeep> copy() const {
         auto c = std::make_unique<PsiDeep>();
-        *static_cast<PsiBase*>(c.get()) = PsiBase();
         c->N = N; c->gpu = gpu; c->free_quantum_axis = free_quantum_axis; c->precision = precision;
         c->prefactor_ = prefactor_; c->alpha = alpha; c->beta = beta;
         c->sizes = sizes; c->conn_counts = conn_counts; c->conns = conns; c->weights = weights; c->biases = biases;
@@ -365,6 +364,36 @@ static carr o_k_vector(const P& psi, const Spins& spins) {
     const uint64_t s[2] = {spins.c, 0};
     check(rbmhip_o_k_vector(psi.h, s, reinterpret_cast<double*>(full.data())));
     return from_vec(std::vector<cd>(full.begin() + 2 * psi.N, full.end()));
+}
+
+using warr = py::array_t<uint64_t, py::array::c_style | py::array::forcecast>;
+
+static void check_words(const warr& w) {
+    if (w.ndim() != 2 || w.shape(1) != 2) throw py::value_error("spins must be an (n, 2) uint64 array of spin words");
+}
+
+template <class P>
+static carr log_psi_batch(const P& psi, const warr& spins) {
+    check_words(spins);
+    const int64_t n = spins.shape(0);
+    carr out((py::ssize_t)n);
+    double* op = reinterpret_cast<double*>(out.mutable_data());
+    const uint64_t* sp = spins.data();
+    py::gil_scoped_release nogil;
+    check(rbmhip_log_psi(psi.h, sp, n, op));
+    return out;
+}
+
+template <class P>
+static carr local_energy_batch(const P& psi, const Operator& o, const warr& spins) {
+    check_words(spins);
+    const int64_t n = spins.shape(0);
+    carr out((py::ssize_t)n);
+    double* op = reinterpret_cast<double*>(out.mutable_data());
+    const uint64_t* sp = spins.data();
+    py::gil_scoped_release nogil;
+    check(rbmhip_local_energy(psi.h, o.h, sp, n, op));
+    return out;
 }
 
 PYBIND11_MODULE(_pyRBMonGPU, m) {
@@ -654,6 +683,10 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
     RBM_HS_BIND(PsiDeep, ExactSummation)
 #undef RBM_HS_BIND
 
+    m.def("log_psi_batch", &log_psi_batch<Psi>, "log psi of (n, 2) uint64 spin words");
+    m.def("log_psi_batch", &log_psi_batch<PsiDeep>);
+    m.def("local_energy_batch", &local_energy_batch<Psi>, "E_loc of (n, 2) uint64 spin words");
+    m.def("local_energy_batch", &local_energy_batch<PsiDeep>);
     m.def("psi_O_k_vector", [](const Psi& p, const Spins& s) { return o_k_vector(p, s); });
     m.def("psi_O_k_vector", [](const PsiDeep& p, const Spins& s) { return o_k_vector(p, s); });
     m.def("get_S_matrix", [](const Psi& psi, ExactSummation& ens) {

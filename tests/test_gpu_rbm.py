@@ -1,0 +1,310 @@
+"""GPU parity of the RBM path against the FP64 oracle.
+
+Tolerances: complex128 kernels (precision=64) agree with the oracle to 1e-10
+relative; complex64 kernels (precision=32, FP64 reductions) to 1e-5 relative,
+measured as max|gpu - ref| / max|ref| over each output.  Basis enumeration,
+spin indexing and, in FP64, whole Markov chains are bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import O, basis_words, dense_hamiltonian, random_operator, random_rbm, rel_err
+
+import rbmongpu_amd as R
+from rbmongpu_amd import hamiltonians as HAM
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = {64: 1e-10, 32: 1e-5}
+PRECISIONS = [64, 32]
+
+
+def gpu_psi(b, W, a=None, prefactor=1.0, precision=64):
+    N = W.shape[0]
+    return R.Psi(np.zeros(N), np.zeros(N), b, W, prefactor, False, True, a=a, precision=precision)
+
+
+def gpu_op(tab):
+    return R.Operator(tab[0], tab[1], tab[2], True)
+
+
+def words_to_int(w):
+    return [int(x[0]) | (int(x[1]) << 64) for x in w]
+
+
+# ------------------------------------------------------------------ golden
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("pre", ["", "vis_"])
+def test_c1_golden(precision, pre):
+    print(f"[c1 golden] precision={precision} visible={bool(pre)}", flush=True)
+    d = np.load(os.path.join(GOLDEN, "c1_tfim_rbm.npz"))
+    a = d["vis_a"] if pre else None
+    tol = TOL[precision]
+    psi = gpu_psi(d[pre + "b"], d[pre + "W"], a, precision=precision)
+    op = gpu_op(HAM.tfim_1d(10))
+    ex = R.ExactSummation(10, True)
+    assert rel_err(psi.vector, d[pre + "psi"]) < tol
+    psi.normalize(ex)
+    assert abs(psi.prefactor - d[pre + "prefactor"]) < tol * d[pre + "prefactor"]
+    psi.prefactor = float(d[pre + "prefactor"])
+    ev = R.ExpectationValue(True)
+    g, E = ev.gradient(psi, op, ex)
+    assert abs(E - d[pre + "energy"]) < tol * abs(d[pre + "energy"])
+    assert rel_err(g, d[pre + "gradient"]) < tol
+    f, E2 = ev.fluctuation(psi, op, ex)
+    assert abs(f - d[pre + "fluctuation"]) < tol * abs(d[pre + "fluctuation"]) * 10
+    psi2 = gpu_psi(d[pre + "b"], d[pre + "W2"], a, precision=precision)
+    hs = R.HilbertSpaceDistance(10, psi2.num_params, True)
+    hg, hd = hs.gradient(psi, psi2, op, True, ex, 1.0)
+    assert abs(hd - d[pre + "hs_distance"]) < tol * d[pre + "hs_distance"]
+    assert rel_err(hg, d[pre + "hs_gradient"]) < tol
+    assert abs(hs(psi, psi2, op, True, ex) - hd) < 1e-12
+
+
+# ------------------------------------------------------------------ shapes
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("N,M", [(1, 1), (5, 16), (7, 17), (20, 80), (64, 40), (65, 33), (100, 200), (128, 256), (3, 250)])
+def test_log_psi_shapes(precision, N, M):
+    print(f"[log_psi shapes] precision={precision} N={N} M={M}", flush=True)
+    b, W, a = random_rbm(N, M, seed=N * 1000 + M, scale=0.05, visible=True)
+    rng = np.random.default_rng(N + M)
+    configs = [int.from_bytes(rng.bytes(16), "little") & ((1 << N) - 1) for _ in range(300)]
+    words = O.spins_words(configs)
+    ref = O.log_psi(O.RBM(b, W, a), words)
+    psi = gpu_psi(b, W, a, precision=precision)
+    got = R.log_psi_batch(psi, words)
+    assert rel_err(got, ref) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("kind", ["tfim", "heis", "random", "repeated", "continuation", "visible"])
+def test_local_energy_operators(precision, kind):
+    print(f"[local energy] precision={precision} kind={kind}", flush=True)
+    N, M = 8, 21
+    b, W, a = random_rbm(N, M, seed=3, scale=0.1, visible=kind == "visible")
+    if kind == "tfim":
+        tab = HAM.tfim_1d(N, 1.0, 0.9)
+    elif kind in ("heis", "visible"):
+        tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)])
+    elif kind == "random":
+        tab = random_operator(N, 20, 4, seed=8)
+    elif kind == "repeated":
+        # repeated indices follow the reference's literal rule (factors use the original spins)
+        tab = HAM.table([(1.0, [(1, 0), (1, 0)]), (0.5, [(2, 1), (2, 1)]), (0.3j, [(1, 2), (3, 2)]),
+                         (0.7, [(3, 3), (2, 3), (1, 4)])])
+    else:
+        # a row without -1 continues into the next row (Operator.hpp:148-152)
+        tab = (np.array([1.0, 0.5 + 0.1j, 0.2]), np.array([[1, 3], [1, 2], [3, 0]], np.int32),
+               np.array([[0, 1], [2, 5], [4, -1]], np.int32))
+    psi = gpu_psi(b, W, a, precision=precision)
+    ref = O.local_energy(O.RBM(b, W, a), O.Operator(*tab), basis_words(N))
+    got = R.local_energy_batch(psi, gpu_op(tab), basis_words(N))
+    assert rel_err(got, ref) < TOL[precision]
+
+
+def test_empty_operator():
+    N, M = 4, 5
+    b, W, _ = random_rbm(N, M, seed=1)
+    psi = gpu_psi(b, W)
+    op = R.Operator(np.zeros(0, complex), np.zeros((0, 1), np.int32), np.zeros((0, 1), np.int32), True)
+    assert np.all(R.local_energy_batch(psi, op, basis_words(N)) == 0)
+    assert R.ExpectationValue(True)(psi, op, R.ExactSummation(N, True)) == 0
+
+
+# ------------------------------------------------------------------ exact summation
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("sector", [None, 0, 2])
+def test_exact_functions(precision, sector):
+    print(f"[exact functions] precision={precision} sector={sector}", flush=True)
+    N, M = 8, 13
+    b, W, a = random_rbm(N, M, seed=17, scale=0.2, visible=True)
+    tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)])
+    tab2 = HAM.tfim_1d(N, 0.5, 1.5)
+    ref_psi = O.RBM(b, W, a, prefactor=0.3)
+    oex = O.Exact(N, sector)
+    psi = gpu_psi(b, W, a, prefactor=0.3, precision=precision)
+    ex = R.ExactSummation(N, True)
+    if sector is not None:
+        ex.set_total_z_symmetry(sector)
+    assert ex.num_steps == oex.num_configs
+    tol = TOL[precision]
+    ev = R.ExpectationValue(True)
+    E = ev(psi, gpu_op(tab), ex)
+    assert abs(E - O.expectation(ref_psi, O.Operator(*tab), oex)) < tol * abs(E)
+    El = ev(psi, [gpu_op(tab), gpu_op(tab2)], ex)
+    assert abs(El[1] - O.expectation(ref_psi, O.Operator(*tab2), oex)) < tol * abs(El[1])
+    assert abs(El[0] - E) < 1e-12 * abs(E)
+    f, E2 = ev.fluctuation(psi, gpu_op(tab), ex)
+    fr, _ = O.fluctuation(ref_psi, O.Operator(*tab), oex)
+    if np.isfinite(fr):
+        assert abs(f - fr) < tol * 100 * abs(fr)
+    g, E3 = ev.gradient(psi, gpu_op(tab), ex)
+    gr, _ = O.gradient(ref_psi, O.Operator(*tab), oex, mode=0)
+    assert rel_err(g, gr) < tol
+    assert abs(psi.norm(ex) - O.psi_norm(ref_psi, oex)) < tol * O.psi_norm(ref_psi, oex)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_s_matrix_and_o_k_vector(precision):
+    N, M = 5, 6
+    b, W, _ = random_rbm(N, M, seed=23, scale=0.2)
+    psi = gpu_psi(b, W, precision=precision)
+    ref = O.RBM(b, W)
+    S = R.get_S_matrix(psi, R.ExactSummation(N, True))
+    assert rel_err(S, O.s_matrix(ref, O.Exact(N))) < TOL[precision]
+    for c in (0, 11, 31):
+        ok = R.psi_O_k_vector(psi, R.Spins(c, N))
+        okr = O.o_k(ref, O.spins_words([c])[0])
+        assert len(ok) == M + N * M
+        assert rel_err(ok, okr[2 * N:]) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_hs_distance_nonunitary_and_mc(precision):
+    N, M = 7, 9
+    b, W, _ = random_rbm(N, M, seed=5, scale=0.2)
+    b2, W2, _ = random_rbm(N, M, seed=6, scale=0.2)
+    tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)])
+    op_t = (1e-2j * tab[0], tab[1], tab[2])
+    psi, psi2 = gpu_psi(b, W, precision=precision), gpu_psi(b2, W2, precision=precision)
+    hs = R.HilbertSpaceDistance(N, psi2.num_params, True)
+    g, d = hs.gradient(psi, psi2, gpu_op(op_t), False, R.ExactSummation(N, True), 0.5)
+    gr, dr = O.hs_distance(O.RBM(b, W), O.RBM(b2, W2), O.Operator(*op_t), False, O.Exact(N), want_grad=True, nu=0.5)
+    assert abs(d - dr) < TOL[precision] * dr
+    assert rel_err(g, gr) < TOL[precision] * 10
+    # Monte Carlo: same samples as the oracle's chains
+    mc = R.MonteCarloLoop(32 * 40, 1, 5, 32, True, seed=11)
+    gm, dm = hs.gradient(psi, psi2, gpu_op(op_t), False, mc, 0.5)
+    omc = O.MonteCarlo(N, 32 * 40, 1, 5, 32, seed=11, call_index=0)
+    grm, drm = O.hs_distance(O.RBM(b, W), O.RBM(b2, W2), O.Operator(*op_t), False, omc, want_grad=True, nu=0.5)
+    if precision == 64:
+        assert abs(dm - drm) < 1e-10 * drm
+        assert rel_err(gm, grm) < 1e-9
+
+
+# ------------------------------------------------------------------ Monte Carlo
+@pytest.mark.parametrize("visible", [False, True])
+def test_mc_chains_bit_identical_fp64(visible):
+    """FP64 kernels replay the oracle's chains: identical spins at every
+    sample, identical acceptance counts, E_loc to 1e-10."""
+    print(f"[mc identity] visible={visible}", flush=True)
+    N, M = 9, 14
+    b, W, a = random_rbm(N, M, seed=31, scale=0.3, visible=visible)
+    tab = HAM.tfim_1d(N, 1.0, 0.8)
+    psi = gpu_psi(b, W, a, precision=64)
+    chains, per = 64, 30
+    mc = R.MonteCarloLoop(chains * per, 2, 5, chains, True, seed=99)
+    ev = R.ExpectationValue(True)
+    g, E = ev.gradient(psi, gpu_op(tab), mc)
+    spins, lp, el = mc.last_samples()
+    omc = O.MonteCarlo(N, chains * per, 2, 5, chains, seed=99, call_index=0)
+    rs, rlp, rel, _ = O.records(O.RBM(b, W, a), O.Operator(*tab), omc)
+    assert np.array_equal(spins, rs)
+    assert rel_err(lp, rlp) < 1e-12 and rel_err(el, rel) < 1e-10
+    acc = mc.acceptance_rate
+    assert abs(acc - omc.acceptances / (omc.acceptances + omc.rejections)) < 1e-7
+    gr, Er = O.gradient(O.RBM(b, W, a), O.Operator(*tab), O.MonteCarlo(N, chains * per, 2, 5, chains, seed=99), mode=1)
+    assert abs(E - Er) < 1e-10 * abs(Er) and rel_err(g, gr) < 1e-10
+    # the next call continues the stream: different samples
+    ev.gradient(psi, gpu_op(tab), mc)
+    spins2, _, _ = mc.last_samples()
+    assert not np.array_equal(spins, spins2)
+    rs2, _, _, _ = O.records(O.RBM(b, W, a), O.Operator(*tab), O.MonteCarlo(N, chains * per, 2, 5, chains, seed=99, call_index=1))
+    assert np.array_equal(spins2, rs2)
+
+
+def test_mc_fp32_follows_fp64_chains():
+    N, M = 10, 20
+    b, W, _ = random_rbm(N, M, seed=41, scale=0.05, fp32_exact=True)
+    tab = HAM.tfim_1d(N)
+    chains, per = 256, 20
+    out = {}
+    for precision in (32, 64):
+        mc = R.MonteCarloLoop(chains * per, 1, 10, chains, True, seed=5)
+        R.ExpectationValue(True)(gpu_psi(b, W, precision=precision), gpu_op(tab), mc)
+        out[precision] = mc.last_samples()
+    same = np.mean(np.all(out[32][0] == out[64][0], axis=1))
+    assert same > 0.98
+    ok = np.all(out[32][0] == out[64][0], axis=1)
+    assert rel_err(out[32][2][ok], out[64][2][ok]) < 1e-5
+
+
+def _chain_mean_se(x, chains):
+    per_chain = x.reshape(-1, chains).mean(axis=0)
+    return per_chain.mean(), per_chain.std(ddof=1) / np.sqrt(chains)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_mc_energy_within_three_standard_errors(precision):
+    """c2-shaped check at reduced size: sampled <H> vs ExactSummation."""
+    N, M = 12, 24
+    b, W, _ = random_rbm(N, M, seed=51, scale=0.1)
+    tab = HAM.tfim_1d(N, 1.0, 1.0)
+    psi = gpu_psi(b, W, precision=precision)
+    ex = R.ExactSummation(N, True)
+    psi.normalize(ex)
+    ev = R.ExpectationValue(True)
+    E_exact = ev(psi, gpu_op(tab), ex).real
+    chains, per = 1024, 100
+    mc = R.MonteCarloLoop(chains * per, 1, 50, chains, True, seed=3)
+    E_mc = ev(psi, gpu_op(tab), mc)
+    _, _, el = mc.last_samples()
+    m, se = _chain_mean_se(el.real, chains)
+    assert abs(m - E_mc.real) < 1e-9 * abs(m)
+    assert abs(E_mc.real - E_exact) < 3 * se, (E_mc, E_exact, se)
+    assert 0.05 < mc.acceptance_rate < 1.0
+
+
+def test_mc_gradient_matches_exact_statistically():
+    N, M = 10, 12
+    b, W, _ = random_rbm(N, M, seed=61, scale=0.15)
+    tab = HAM.tfim_1d(N, 1.0, 1.0)
+    psi = gpu_psi(b, W, precision=32)
+    ex = R.ExactSummation(N, True)
+    psi.normalize(ex)
+    ev = R.ExpectationValue(True)
+    g_ex, _ = ev.gradient(psi, gpu_op(tab), ex)
+    reps = [ev.gradient(psi, gpu_op(tab), R.MonteCarloLoop(512 * 50, 1, 30, 512, True, seed=100 + r))[0] for r in range(6)]
+    reps = np.array(reps)
+    mean, se = reps.mean(axis=0), reps.std(axis=0, ddof=1) / np.sqrt(len(reps))
+    big = np.abs(g_ex) > 0.05 * np.abs(g_ex).max()
+    z = np.abs(mean - g_ex)[big] / np.maximum(np.abs(se[big]), 1e-12)
+    assert np.median(z) < 1.5 and np.mean(z < 4) > 0.95
+
+
+# ------------------------------------------------------------------ full size (c2)
+def test_c2_full_size_properties():
+    """N=20, M=80 TFIM with N(0, 0.01^2) parameters: ExactSummation over 2^20
+    states in complex64 agrees with complex128 to 1e-5; the 4096-chain
+    sampler agrees with the exact energy within 3 standard errors; the
+    distance to the W + N(0, 1e-3^2) copy agrees between precisions."""
+    print("[c2 full size]", flush=True)
+    rng = np.random.default_rng(1236)
+    N, M = 20, 80
+    cn = lambda *s: 0.01 * (rng.normal(size=s) + 1j * rng.normal(size=s))  # noqa: E731
+    b, W = cn(M), cn(N, M)
+    W2 = W + 1e-3 * (rng.normal(size=(N, M)) + 1j * rng.normal(size=(N, M)))
+    op = gpu_op(HAM.tfim_1d(N))
+    ident = R.Operator(np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32), True)
+    ex = R.ExactSummation(N, True)
+    ev = R.ExpectationValue(True)
+    res = {}
+    for precision in (64, 32):
+        psi = gpu_psi(b, W, precision=precision)
+        psi.normalize(ex)
+        g, E = ev.gradient(psi, op, ex)
+        hs = R.HilbertSpaceDistance(N, psi.num_params, True)
+        d = hs(psi, gpu_psi(b, W2, precision=precision), ident, True, ex)
+        res[precision] = (psi.prefactor, E, g, d)
+    assert abs(res[32][0] - res[64][0]) < 1e-5 * res[64][0]
+    assert abs(res[32][1] - res[64][1]) < 1e-5 * abs(res[64][1])
+    assert rel_err(res[32][2], res[64][2]) < 1e-5
+    assert abs(res[32][3] - res[64][3]) < 1e-3 * res[64][3]
+    psi = gpu_psi(b, W, precision=32)
+    psi.prefactor = res[64][0]
+    mc = R.MonteCarloLoop(4096 * 100, 1, 50, 4096, True, seed=2)
+    E_mc = ev(psi, op, mc)
+    _, _, el = mc.last_samples()
+    m, se = _chain_mean_se(el.real, 4096)
+    assert abs(E_mc.real - res[64][1].real) < 3 * se, (E_mc, res[64][1], se)

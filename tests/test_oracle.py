@@ -1,0 +1,325 @@
+"""Pins of the FP64 CPU oracle (oracle/rbm_oracle.c) against routes that do
+not restate its formulas: Random123 known-answer vectors, calculus identities
+of the log-cosh approximant, closed-form log psi with numpy angles (the
+reference's own test_Psi / test_PsiDeep checks), dense Hamiltonians from
+Kronecker products of Pauli matrices, central finite differences and the
+stationary distribution of the Metropolis chain."""
+import numpy as np
+import pytest
+
+from helpers import (O, basis_spins, basis_words, dense_hamiltonian, deep_layers, random_operator, random_rbm,
+                     rel_err)
+from rbmongpu_amd import hamiltonians as HAM
+
+
+# ---------------------------------------------------------------- Philox
+@pytest.mark.parametrize("ctr,key,expected", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(ctr, key, expected):
+    """Random123 kat_vectors for philox4x32-10."""
+    assert O.philox(ctr, key).tolist() == expected
+
+
+# ---------------------------------------------------------------- activation
+def test_logcosh_approximates_log_cosh_on_real_axis():
+    x = np.linspace(-1, 1, 2001)
+    err = np.abs(O.logcosh(x).real - np.log(np.cosh(x)))
+    assert err.max() < 3e-2            # a rational fit of log cosh near 0, not log cosh itself
+    assert np.abs(O.logcosh(x).imag).max() == 0.0
+    f0, f1 = O.logcosh(0.0)[0].real, O.logcosh(0.01)[0].real
+    assert abs((f1 - f0) / 0.01 ** 2 - 0.5) < 0.05   # log cosh x = x^2 / 2 + O(x^4)
+    assert abs(O.logcosh(0.0)[0].real - (-1.306188e-7)) < 1e-12  # value at 0 quoted in the survey
+
+
+def test_logcosh_is_even():
+    rng = np.random.default_rng(3)
+    z = rng.normal(size=200) + 1j * rng.normal(size=200)
+    z = z[np.abs(z.real) > 1e-3]
+    assert np.allclose(O.logcosh(z), O.logcosh(-z), rtol=0, atol=1e-14)
+
+
+def test_tanh_is_derivative_of_logcosh():
+    """my_tanh = d my_logcosh / dz (holomorphic on each half plane): a dropped
+    or altered term in either function breaks this."""
+    rng = np.random.default_rng(4)
+    z = rng.normal(size=300) + 1j * rng.normal(size=300)
+    z = z[np.abs(z.real) > 1e-2]
+    h = 1e-6
+    for d in (h, 1j * h):
+        fd = (O.logcosh(z + d) - O.logcosh(z - d)) / (2 * d)
+        assert np.allclose(fd, O.tanh(z), rtol=1e-7, atol=1e-8)
+
+
+def test_logcosh_branch_jump_is_in_imaginary_part():
+    y = 0.05
+    hi, lo = O.logcosh(1e-14 + 1j * y)[0], O.logcosh(-1e-14 + 1j * y)[0]
+    assert abs(hi.real - lo.real) < 1e-12
+    assert 1e-6 < abs(hi.imag - lo.imag) < 1e-4
+
+
+# ---------------------------------------------------------------- RBM
+@pytest.mark.parametrize("N,M,visible", [(3, 9, False), (6, 5, True), (10, 20, False)])
+def test_rbm_log_psi_closed_form(N, M, visible):
+    """psi(s) = exp(sum_j activation(W[:, j] . s + b_j)) (+ a . s), test_Psi.py:21-30."""
+    b, W, a = random_rbm(N, M, seed=N, visible=visible)
+    psi = O.RBM(b, W, a)
+    S = basis_spins(N)
+    theta = S @ W + b
+    ref = O.logcosh(theta.ravel()).reshape(theta.shape).sum(axis=1)
+    if visible:
+        ref = ref + S @ a
+    got = O.log_psi(psi, basis_words(N))
+    assert rel_err(got, ref) < 1e-13
+    assert rel_err(O.psi_vector(psi), np.exp(ref)) < 1e-13
+
+
+def _fd_logpsi(model_from_params, params, words, k, h=1e-6):
+    p1, p2 = params.copy(), params.copy()
+    p1[k] += h
+    p2[k] -= h
+    return (O.log_psi(model_from_params(p1), words) - O.log_psi(model_from_params(p2), words)) / (2 * h)
+
+
+def test_rbm_o_k_finite_differences():
+    N, M = 4, 6
+    b, W, a = random_rbm(N, M, seed=7, visible=True)
+    params = np.concatenate([b, W.ravel(), a])
+
+    def model(p):
+        return O.RBM(p[:M], p[M:M + N * M].reshape(N, M), p[M + N * M:])
+
+    words = O.spins_words([5, 9])
+    for w in words:
+        ok = O.o_k(model(params), w)
+        assert np.all(ok[:2 * N] == 0)
+        for k in range(len(params)):
+            fd = _fd_logpsi(model, params, w[None, :], k)[0]
+            assert abs(fd - ok[2 * N + k]) < 1e-7
+
+
+@pytest.mark.parametrize("ham", ["tfim", "heis", "random"])
+def test_local_energy_dense(ham):
+    N, M = 6, 7
+    b, W, _ = random_rbm(N, M, seed=11)
+    psi = O.RBM(b, W)
+    if ham == "tfim":
+        tab = HAM.tfim_1d(N, 1.0, 0.7)
+    elif ham == "heis":
+        tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)])
+    else:
+        tab = random_operator(N, 12, 3, seed=5)
+    op = O.Operator(*tab)
+    H = dense_hamiltonian(N, *tab)
+    v = O.psi_vector(psi)
+    ref = (H @ v) / v
+    got = O.local_energy(psi, op, basis_words(N))
+    assert rel_err(got, ref) < 1e-12
+
+
+def test_exact_expectation_fluctuation_norm_dense():
+    N, M = 6, 4
+    b, W, _ = random_rbm(N, M, seed=12)
+    tab = HAM.tfim_1d(N, 1.0, 1.3)
+    H = dense_hamiltonian(N, *tab)
+    psi = O.RBM(b, W)
+    v = O.psi_vector(psi)
+    ex = O.Exact(N)
+    assert abs(O.psi_norm(psi, ex) - np.linalg.norm(v)) < 1e-12 * np.linalg.norm(v)
+    # unnormalised: <E> = sum_s |psi|^2 E_loc = psi^H H psi
+    assert abs(O.expectation(psi, O.Operator(*tab), ex) - np.vdot(v, H @ v)) < 1e-10 * abs(np.vdot(v, H @ v))
+    psi.prefactor = 1.0 / np.linalg.norm(v)
+    vn = v / np.linalg.norm(v)
+    f, E = O.fluctuation(psi, O.Operator(*tab), ex)
+    E_ref = np.vdot(vn, H @ vn).real
+    assert abs(E - E_ref) < 1e-12
+    assert abs(f - np.sqrt(np.vdot(H @ vn, H @ vn).real - E_ref ** 2)) < 1e-9
+
+
+def test_exact_sector_enumeration():
+    N = 6
+    ex = O.Exact(N, sector=0)
+    assert ex.num_configs == 20
+    b, W, _ = random_rbm(N, 3, seed=2)
+    psi = O.RBM(b, W)
+    v = O.psi_vector(psi)
+    sel = [i for i in range(2 ** N) if 2 * bin(i).count("1") - N == 0]
+    assert abs(O.psi_norm(psi, ex) - np.linalg.norm(v[sel])) < 1e-12
+
+
+def test_gradient_is_derivative_of_energy():
+    """For Hermitian H and normalised psi the reference estimator equals
+    d<H>/dRe p + i d<H>/dIm p of the dense normalised energy."""
+    N, M = 4, 3
+    b, W, _ = random_rbm(N, M, seed=21, scale=0.3)
+    tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)], 1.0)
+    tab = (np.concatenate([tab[0], [0.4]]), np.concatenate([tab[1], [[1, 0, 0]]]),
+           np.concatenate([tab[2], [[2, -1, -1]]]))
+    H = dense_hamiltonian(N, *tab)
+    params = np.concatenate([b, W.ravel()])
+
+    def model(p):
+        return O.RBM(p[:M], p[M:].reshape(N, M))
+
+    def energy(p):
+        v = O.psi_vector(model(p))
+        return (np.vdot(v, H @ v) / np.vdot(v, v)).real
+
+    psi = model(params)
+    psi.prefactor = 1.0 / np.linalg.norm(O.psi_vector(psi))
+    g0, E0 = O.gradient(psi, O.Operator(*tab), O.Exact(N), mode=0)
+    g1, _ = O.gradient(psi, O.Operator(*tab), O.Exact(N), mode=1)
+    assert abs(E0 - energy(params)) < 1e-12
+    assert np.allclose(g0, g1, rtol=0, atol=1e-12)
+    h = 1e-6
+    for k in range(len(params)):
+        d = np.zeros_like(params)
+        d[k] = h
+        fd = (energy(params + d) - energy(params - d)) / (2 * h)
+        fd += 1j * (energy(params + 1j * d) - energy(params - 1j * d)) / (2 * h)
+        assert abs(g0[2 * N + k] - fd) < 1e-7
+    assert np.all(g0[:2 * N] == 0)
+
+
+def test_s_matrix_outer_products():
+    N, M = 3, 4
+    b, W, _ = random_rbm(N, M, seed=31)
+    psi = O.RBM(b, W)
+    S = O.s_matrix(psi, O.Exact(N))
+    ref = np.zeros_like(S)
+    for w in basis_words(N):
+        ok = O.o_k(psi, w)
+        ref += np.outer(ok, ok.conj())
+    assert rel_err(S, ref / 2 ** N) < 1e-13
+
+
+def test_hs_distance_identity_closed_form_and_gradient():
+    """Identity operator, unitary: d = sqrt(1 - |<psi|psi'>|^2 / (<psi|psi><psi'|psi'>))
+    (test_HilbertSpaceDistance.py:182-242 uses the same setting)."""
+    N, M = 4, 5
+    b, W, _ = random_rbm(N, M, seed=41, scale=0.3)
+    b2, W2, _ = random_rbm(N, M, seed=42, scale=0.3)
+    ident = O.Operator(np.array([1.0 + 0j]), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    psi = O.RBM(b, W)
+    params = np.concatenate([b2, W2.ravel()])
+
+    def model(p):
+        return O.RBM(p[:M], p[M:].reshape(N, M))
+
+    def dist(p):
+        v, v2 = O.psi_vector(psi), O.psi_vector(model(p))
+        return np.sqrt(1 - abs(np.vdot(v2, v)) ** 2 / (np.vdot(v, v).real * np.vdot(v2, v2).real))
+
+    g, d = O.hs_distance(psi, model(params), ident, True, O.Exact(N), want_grad=True, nu=1.0)
+    assert abs(d - dist(params)) < 1e-12
+    h = 1e-6
+    for k in range(len(params)):
+        dd = np.zeros_like(params)
+        dd[k] = h
+        fd = (dist(params + dd) - dist(params - dd)) / (2 * h) + 1j * (dist(params + 1j * dd) - dist(params - 1j * dd)) / (2 * h)
+        assert abs(g[2 * N + k] - fd) < 1e-6
+
+
+def test_hs_distance_small_time_step():
+    """Non-unitary operator i t H against sqrt(1 - |<psi|exp(iHt)|psi>|^2),
+    the reference's test_distance1 (test_HilbertSpaceDistance.py:10-29)."""
+    N, M = 5, 4
+    b, W, _ = random_rbm(N, M, seed=51, scale=0.3)
+    psi = O.RBM(b, W)
+    v = O.psi_vector(psi)
+    psi.prefactor = 1 / np.linalg.norm(v)
+    tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)])
+    t = 1e-3
+    H = dense_hamiltonian(N, *tab)
+    e, U = np.linalg.eigh(H)
+    Ut = U @ np.diag(np.exp(1j * e * t)) @ U.conj().T
+    vn = v / np.linalg.norm(v)
+    ref = np.sqrt(1 - abs(np.vdot(vn, Ut @ vn)) ** 2)
+    op = O.Operator(1j * t * tab[0], tab[1], tab[2])
+    d = O.hs_distance(psi, psi, op, False, O.Exact(N))
+    assert d == pytest.approx(ref, rel=1e-3, abs=1e-8)
+
+
+def test_metropolis_stationary_distribution():
+    """Visit frequencies of independent chains against |psi|^2: per-bin
+    z-scores with the standard error taken across chains (chains are
+    independent, samples within a chain are not)."""
+    N, M = 4, 3
+    b, W, _ = random_rbm(N, M, seed=61, scale=0.6)
+    psi = O.RBM(b, W)
+    p = np.abs(O.psi_vector(psi)) ** 2
+    p /= p.sum()
+    chains, per = 256, 300
+    mc = O.MonteCarlo(N, chains * per, 2, 10, chains, seed=7)
+    spins, _, _, _ = O.records(psi, None, mc)
+    idx = spins[:, 0].astype(np.int64).reshape(per, chains)
+    freq = np.stack([np.bincount(idx[:, c], minlength=2 ** N) / per for c in range(chains)])
+    mean, se = freq.mean(axis=0), freq.std(axis=0, ddof=1) / np.sqrt(chains)
+    z = np.abs(mean - p) / np.maximum(se, 1e-12)
+    assert z.max() < 4.5, z
+    assert mc.acceptances + mc.rejections == chains * (10 + per * 2) * N
+
+
+# ---------------------------------------------------------------- PsiDeep
+@pytest.mark.parametrize("ti", [False, True])
+def test_deep_log_psi_closed_form(ti):
+    """test_PsiDeep.py:20-49: layer j reads activations[(j * delta + i) % n],
+    translational invariance averages np.roll shifts."""
+    N, M, C = 6, [12, 4], [3, 6]
+    biases, conns, weights, final = deep_layers(N, M, C, seed=3)
+    psi = O.Deep(N, biases, conns, weights, final, translational_invariance=ti)
+    S = basis_spins(N)
+    ref = np.zeros(2 ** N, dtype=complex)
+    for s_idx in range(2 ** N):
+        acc = 0
+        for shift in range(N if ti else 1):
+            x = np.roll(S[s_idx], shift)
+            for w, b, cn in zip(weights, biases, conns):
+                a = np.array([sum(w[i, j] * x[cn[i, j]] for i in range(w.shape[0])) + b[j] for j in range(len(b))])
+                x = O.logcosh(a)
+            acc += np.sum(x * final)
+        ref[s_idx] = acc / (N if ti else 1)
+    assert rel_err(O.log_psi(psi, basis_words(N)), ref) < 1e-13
+
+
+def test_deep_o_k_finite_differences_and_final_weight_convention():
+    N, M, C = 5, [10, 5], [2, 4]
+    biases, conns, weights, final = deep_layers(N, M, C, seed=8)
+
+    def split(p):
+        out_b, out_w, off = [], [], 0
+        for m, c in zip(M, C):
+            out_b.append(p[off:off + m]); off += m
+            out_w.append(p[off:off + c * m].reshape(c, m)); off += c * m
+        return out_b, out_w, p[off:]
+
+    params = np.concatenate([np.concatenate([b, w.ravel()]) for b, w in zip(biases, weights)] + [final])
+
+    def model(p, compat=False):
+        bb, ww, ff = split(p)
+        return O.Deep(N, bb, conns, ww, ff, ref_compat_final=compat)
+
+    w = O.spins_words([13])[0]
+    ok = O.o_k(model(params), w)
+    for k in range(len(params)):
+        fd = _fd_logpsi(model, params, w[None, :], k)[0]
+        assert abs(fd - ok[2 * N + k]) < 1e-6
+    ok_ref = O.o_k(model(params, compat=True), w)
+    lp = O.log_psi(model(params), w[None, :])[0]
+    nf = len(final)
+    assert np.allclose(ok_ref[:-nf], ok[:-nf], rtol=0, atol=0)
+    assert np.allclose(ok_ref[-nf:], lp * ok[-nf:], rtol=1e-14, atol=0)
+
+
+def test_deep_local_energy_dense():
+    N, M, C = 5, [10, 5], [5, 10]
+    biases, conns, weights, final = deep_layers(N, M, C, seed=9)
+    psi = O.Deep(N, biases, conns, weights, final)
+    tab = random_operator(N, 10, 3, seed=9)
+    H = dense_hamiltonian(N, *tab)
+    v = O.psi_vector(psi)
+    got = O.local_energy(psi, O.Operator(*tab), basis_words(N))
+    assert rel_err(got, (H @ v) / v) < 1e-12
