@@ -154,6 +154,8 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
                        int want_grad, double nu, double* distance, double* grad);
 
 /* ---- kernel timing (HIP events on the library stream) -------------------- */
+/* wait for all work on the library stream of the current device */
+int rbmhip_synchronize(void);
 int rbmhip_timing_enable(int on);
 int rbmhip_timing_reset(void);
 /* total milliseconds and launch count of a kernel family ("mc", "eval", "ok_reduce", ...) */

@@ -19,7 +19,7 @@ try:
         Psi, PsiDeep, Operator, Spins, MonteCarloLoop, ExactSummation, ExpectationValue,
         HilbertSpaceDistance, psi_O_k_vector, get_S_matrix, activation_function, activation_derivative,
         setDevice, start_profiling, stop_profiling, device_count, timing_enable, timing_reset, timing_query,
-        log_psi_batch, local_energy_batch,
+        log_psi_batch, local_energy_batch, synchronize,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(

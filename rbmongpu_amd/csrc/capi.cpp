@@ -1291,6 +1291,10 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
 }
 
 // ---- timing -------------------------------------------------------------------------
+int rbmhip_synchronize(void) {
+    return guard([&] { ctx().sync(); });
+}
+
 int rbmhip_timing_enable(int on) {
     return guard([&] { Ctx& c = ctx(); c.sync(); c.timing = on != 0; });
 }

@@ -713,6 +713,7 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
     m.def("start_profiling", [] { check(rbmhip_start_profiling()); });
     m.def("stop_profiling", [] { check(rbmhip_stop_profiling()); });
     m.def("device_count", [] { int n = 0; check(rbmhip_device_count(&n)); return n; });
+    m.def("synchronize", [] { check(rbmhip_synchronize()); });
     m.def("timing_enable", [](bool on) { check(rbmhip_timing_enable(on)); });
     m.def("timing_reset", [] { check(rbmhip_timing_reset()); });
     m.def("timing_query", [](const std::string& name) {
