@@ -124,7 +124,7 @@ def main():
     wall = time.perf_counter() - t0
     R.timing_enable(False)
     ms_per_step = 1e3 * wall / args.steps
-    kernel_ms = {k: R.timing_query(k) for k in ("mc", "coef", "scalar_reduce", "ok_reduce", "partials_sum")}
+    kernel_ms = {k: R.timing_query(k) for k in ("mc", "eval", "coef", "scalar_reduce", "ok_reduce", "partials_sum")}
     mc_ms, mc_launches = kernel_ms["mc"]
     mc_avg_ms = mc_ms / max(mc_launches, 1)
 
@@ -135,9 +135,12 @@ def main():
     ess_per_s = samples / (2.0 * tau) / (ms_per_step / 1e3)
 
     terms = N  # N single-flip X terms; the N ZZ strings are diagonal
-    launch_flops = F.mc_launch_flops(N, M, terms, CHAINS, spc, THERM, SWEEPS)
+    launch_flops = F.mc_launch_flops(N, M, CHAINS, spc, THERM, SWEEPS)
     achieved = launch_flops / (mc_avg_ms / 1e3) / 1e12
-    traffic = F.mc_launch_bytes(N, M, CHAINS, spc, tanh_bytes=8 if args.precision == 32 else 16)
+    traffic = F.mc_launch_bytes(CHAINS, spc)
+    ev_ms, ev_launches = kernel_ms["eval"]
+    ev_avg_ms = ev_ms / max(ev_launches, 1)
+    ev_flops = F.eval_flops(N, M, terms, CHAINS * spc)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -174,6 +177,9 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "k_mc_rbm", "flop_per_launch": launch_flops, "avg_launch_ms": mc_avg_ms,
                      "algorithmic_bytes_per_launch": traffic},
+        "eval_kernel": {"kernel": "k_eval_rbm (log psi, E_loc, my_tanh of the recorded samples)",
+                        "avg_launch_ms": ev_avg_ms, "flop_per_launch": ev_flops,
+                        "achieved_tflops": ev_flops / (ev_avg_ms / 1e3) / 1e12 if ev_avg_ms > 0 else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -225,9 +225,15 @@ static void upload_cx(Ctx& c, DevBuf& buf, const std::vector<cd>& v) {
 template <typename T>
 static RbmDev<T> rbm_dev(Ctx& c, rbmhip_psi* psi) {
     DevModel& d = psi->dev[c.device];
+    const int ld = rbm_padded_width(psi->M);
     if (d.version != psi->version || d.precision != psi->precision) {
-        upload_cx<T>(c, d.W, psi->W);
-        upload_cx<T>(c, d.b, psi->b);
+        // W and b padded with zero columns to ld hidden units
+        std::vector<cd> Wp((size_t)psi->N * ld, cd(0, 0)), bp((size_t)ld, cd(0, 0));
+        for (int i = 0; i < psi->N; i++)
+            for (int j = 0; j < psi->M; j++) Wp[(size_t)i * ld + j] = psi->W[(size_t)i * psi->M + j];
+        for (int j = 0; j < psi->M; j++) bp[j] = psi->b[j];
+        upload_cx<T>(c, d.W, Wp);
+        upload_cx<T>(c, d.b, bp);
         if (psi->has_a) upload_cx<T>(c, d.a, psi->a);
         d.version = psi->version;
         d.precision = psi->precision;
@@ -235,6 +241,7 @@ static RbmDev<T> rbm_dev(Ctx& c, rbmhip_psi* psi) {
     RbmDev<T> md;
     md.N = psi->N;
     md.M = psi->M;
+    md.ld = ld;
     md.W = reinterpret_cast<const cx<T>*>(d.W.p);
     md.b = reinterpret_cast<const cx<T>*>(d.b.p);
     md.a = psi->has_a ? reinterpret_cast<const cx<T>*>(d.a.p) : nullptr;
@@ -486,8 +493,8 @@ struct Samples {
 };
 
 template <typename T>
-static bool rbm_w_in_lds(Ctx& c, const rbmhip_psi* psi) {
-    return (size_t)psi->N * psi->M * sizeof(cx<T>) <= c.max_lds;
+static size_t rbm_w_bytes(const rbmhip_psi* psi) {
+    return (size_t)psi->N * rbm_padded_width(psi->M) * sizeof(cx<T>);
 }
 
 static void check_op_fits(const rbmhip_psi* psi, const rbmhip_op* op) {
@@ -503,8 +510,8 @@ static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const 
     const int max_blocks = c.num_cus * 8;
     if (psi->kind == 0) {
         const RbmDev<T> md = rbm_dev<T>(c, psi);
-        const bool wlds = rbm_w_in_lds<T>(c, psi);
-        const size_t lds = (size_t)psi->N * psi->M * sizeof(cx<T>);
+        const size_t lds = rbm_w_bytes<T>(psi);
+        const bool wlds = lds <= c.max_lds;
         c.launch("eval", [&] { return launch_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, wlds, lds, max_blocks, c.stream); });
     } else {
         const DeepDev<T> md = deep_dev<T>(c, psi);
@@ -534,7 +541,6 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
         s.spins = reinterpret_cast<uint64_t*>(e->spins.p);
         s.lp = reinterpret_cast<cx<double>*>(e->lp.p);
         s.el = reinterpret_cast<cx<double>*>(e->el.p);
-        Records rec{s.spins, s.lp, op ? s.el : nullptr, s.tanh};
         McParams mp;
         mp.chains = (int)e->num_chains;
         mp.steps_per_chain = (int)(e->num_samples / e->num_chains);
@@ -546,11 +552,16 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
         uint32_t* counts = reinterpret_cast<uint32_t*>(c.buf("counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
         const OpDev od = op ? op_dev(c, op) : empty_op();
         if (psi->kind == 0) {
+            // sampling records configurations; measurement runs at full
+            // occupancy in the evaluation kernel on the recorded samples
             const RbmDev<T> md = rbm_dev<T>(c, psi);
-            const bool wlds = rbm_w_in_lds<T>(c, psi);
-            const size_t lds = (size_t)psi->N * psi->M * sizeof(cx<T>);
-            c.launch("mc", [&] { return launch_mc<T>(md, od, op != nullptr, mp, rec, counts, wlds, lds, c.stream); });
+            const size_t lds = rbm_w_bytes<T>(psi);
+            const bool wlds = lds + 4096 <= c.max_lds;
+            c.launch("mc", [&] { return launch_mc<T>(md, mp, s.spins, counts, wlds, lds, c.stream); });
+            Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
+            eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
         } else {
+            Records rec{s.spins, s.lp, op ? s.el : nullptr, s.tanh};
             const DeepDev<T> md = deep_dev<T>(c, psi);
             c.launch("deep_mc", [&] { return launch_deep_mc<T>(md, od, op != nullptr, mp, rec, counts, c.stream); });
         }

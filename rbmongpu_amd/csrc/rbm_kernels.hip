@@ -24,24 +24,23 @@ template <typename T, int JPL>
 RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const OpDev& op,
                                    uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], const cx<T> (&lcv)[JPL],
                                    int lane) {
-    const int M = md.M;
+    const int ld = md.ld;
     cx<double> E = diag_energy(op, s0, s1);
     for (int t = 0; t < op.n_terms; t++) {
         const int fb = op.flip_begin[t], fe = op.flip_begin[t + 1];
+        cx<T> v[JPL];
+#pragma unroll
+        for (int u = 0; u < JPL; u++) v[u] = th[u];
+        for (int q = fb; q < fe; q++) {
+            const int pos = op.flips[q];
+            const T f = T(-2) * spin_of<T>(s0, s1, pos);
+#pragma unroll
+            for (int u = 0; u < JPL; u++) v[u] = v[u] + f * Wsrc[pos * ld + lane + 16 * u];
+        }
+        // padded units have theta = theta' = 0 and contribute nothing
         cx<T> part = mk(T(0), T(0));
 #pragma unroll
-        for (int u = 0; u < JPL; u++) {
-            const int j = lane + 16 * u;
-            if (j < M) {
-                cx<T> v = th[u];
-                for (int q = fb; q < fe; q++) {
-                    const int pos = op.flips[q];
-                    const T f = T(-2) * spin_of<T>(s0, s1, pos);
-                    v = v + f * Wsrc[pos * M + j];
-                }
-                part = part + (logcosh(v) - lcv[u]);
-            }
-        }
+        for (int u = 0; u < JPL; u++) part = part + (logcosh(v[u]) - lcv[u]);
         cx<T> d = row_sum16(part);
         if (md.a) {
             for (int q = fb; q < fe; q++) {
@@ -67,35 +66,33 @@ RBM_DI cx<T> visible_sum(const RbmDev<T>& md, uint64_t s0, uint64_t s1) {
 template <typename T, int JPL>
 RBM_DI void init_angles(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, uint64_t s0, uint64_t s1,
                         int lane, cx<T> (&th)[JPL]) {
-    const int N = md.N, M = md.M;
+    const int N = md.N, ld = md.ld;
 #pragma unroll
-    for (int u = 0; u < JPL; u++) {
-        const int j = lane + 16 * u;
-        th[u] = mk(T(0), T(0));
-        if (j < M) {
-            cx<T> v = md.b[j];
-            for (int i = 0; i < N; i++) v = v + spin_of<T>(s0, s1, i) * Wsrc[i * M + j];
-            th[u] = v;
-        }
+    for (int u = 0; u < JPL; u++) th[u] = md.b[lane + 16 * u];
+    for (int i = 0; i < N; i++) {
+        const T si = spin_of<T>(s0, s1, i);
+#pragma unroll
+        for (int u = 0; u < JPL; u++) th[u] = th[u] + si * Wsrc[i * ld + lane + 16 * u];
     }
 }
 
 template <typename T, int JPL>
 RBM_DI void measure(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const OpDev& op, int has_op,
                     uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], cx<T> vis, int lane, int64_t sample,
-                    const Records& rec) {
+                    const Records& rec, bool valid = true) {
     const int M = md.M;
     cx<T> lcv[JPL];
     cx<T> lsum = mk(T(0), T(0));
 #pragma unroll
     for (int u = 0; u < JPL; u++) {
-        const int j = lane + 16 * u;
-        lcv[u] = j < M ? logcosh(th[u]) : mk(T(0), T(0));
-        lsum = lsum + lcv[u];
+        lcv[u] = logcosh(th[u]);
+        const bool real_unit = lane + 16 * u < M;
+        lsum = lsum + (real_unit ? lcv[u] : mk(T(0), T(0)));
     }
     const cx<double> lp = row_sum16(to_d(lsum)) + to_d(vis);
     cx<double> E = mk(0.0, 0.0);
     if (has_op) E = local_energy_rbm<T, JPL>(md, Wsrc, op, s0, s1, th, lcv, lane);
+    if (!valid) return;
     if (lane == 0) {
         if (rec.spins) { rec.spins[2 * sample] = s0; rec.spins[2 * sample + 1] = s1; }
         if (rec.log_psi) rec.log_psi[sample] = lp;
@@ -114,27 +111,43 @@ RBM_DI void measure(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const O
 // ---------------------------------------------------------------------------
 // Metropolis chains: include/spin_ensembles/MonteCarloLoop.hpp:55-256.
 // Per call: random start, therm*N proposals, then for each of the
-// steps_per_chain samples sweeps*N proposals followed by a measurement of
-// sample index step * chains + chain.  Proposal k of a chain draws
+// steps_per_chain samples sweeps*N proposals followed by recording the
+// configuration as sample index step * chains + chain.  Proposal k of a chain draws
 // Philox(seed; chain, k >> 1, call, 0): words (x, y) for even k, (z, w) for
 // odd k; position = x % N, u = (y + 0.5) 2^-32; accept iff ratio > 1 or
 // u <= ratio with ratio = |psi(s')/psi(s)|^2.
+//
+// The draws do not depend on the chain state, so they are produced ahead of
+// use: every 32 proposals the 16 lanes of a row compute the next 16 Philox
+// blocks in parallel (lane l takes pair k/2 + l) and leave (position, y) in
+// the row's slice of LDS.  The W row of the next proposal is loaded while the
+// current one is evaluated.  Rows past the last chain replay the last chain
+// and write nothing, so every wavefront reaches every barrier.
 // ---------------------------------------------------------------------------
+constexpr int MC_DRAW_BLOCK = 32;
+constexpr size_t MC_DRAW_LDS = 16 * MC_DRAW_BLOCK * sizeof(uint2);
+
+RBM_DI float exp_fast(float x) { return __expf(x); }
+RBM_DI double exp_fast(double x) { return exp(x); }
+
 template <typename T, int JPL, bool WLDS>
-__global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, OpDev op, int has_op, McParams mp,
-                                                Records rec, uint32_t* __restrict__ counts) {
+__global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint64_t* __restrict__ spins_out,
+                                                uint32_t* __restrict__ counts) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int N = md.N, M = md.M;
+    const int N = md.N, ld = md.ld;
+    const size_t wbytes = WLDS ? (size_t)N * ld * sizeof(cx<T>) : 0;
     const cx<T>* Wsrc = md.W;
     if (WLDS) {
         cx<T>* Wl = reinterpret_cast<cx<T>*>(smem_raw);
-        for (int k = threadIdx.x; k < N * M; k += blockDim.x) Wl[k] = md.W[k];
-        __syncthreads();
+        for (int k = threadIdx.x; k < N * ld; k += blockDim.x) Wl[k] = md.W[k];
         Wsrc = Wl;
     }
-    const int lane = threadIdx.x & 15;
-    const int chain = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);
-    if (chain >= mp.chains) return;
+    const int lane = threadIdx.x & 15, row = threadIdx.x >> 4;
+    uint2* draws = reinterpret_cast<uint2*>(smem_raw + wbytes) + row * MC_DRAW_BLOCK;
+    const int chain_raw = blockIdx.x * 16 + row;
+    const bool valid = chain_raw < mp.chains;
+    const int chain = valid ? chain_raw : mp.chains - 1;
+    __syncthreads();
 
     const u32x4 r0 = philox((uint32_t)chain, 0u, mp.call, 1u, mp.seed_lo, mp.seed_hi);
     uint64_t s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
@@ -145,61 +158,96 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, OpDev op, int has_
     init_angles<T, JPL>(md, Wsrc, s0, s1, lane, th);
     T lcr[JPL];
 #pragma unroll
-    for (int u = 0; u < JPL; u++) lcr[u] = (lane + 16 * u) < M ? logcosh_re(th[u]) : T(0);
+    for (int u = 0; u < JPL; u++) lcr[u] = logcosh_re(th[u]);
     cx<T> vis = visible_sum(md, s0, s1);
 
-    uint32_t nacc = 0, nrej = 0;
-    u32x4 rr = r0;
-    uint64_t k = 0;
     const uint64_t n_therm = (uint64_t)mp.therm * N;
     const uint64_t n_sweep = (uint64_t)mp.sweeps * N;
+    const uint64_t total = n_therm + (uint64_t)mp.steps_per_chain * n_sweep;
+    uint64_t next_measure = n_therm + n_sweep;
+    int sample = 0;
+    uint32_t nacc = 0;
 
-    auto propose = [&](uint64_t kk) {
-        if ((kk & 1) == 0) rr = philox((uint32_t)chain, (uint32_t)(kk >> 1), mp.call, 0u, mp.seed_lo, mp.seed_hi);
-        const uint32_t x = (kk & 1) ? rr.z : rr.x;
-        const uint32_t y = (kk & 1) ? rr.w : rr.y;
-        const int p = (int)(x % (uint32_t)N);
-        const T f = T(-2) * spin_of<T>(s0, s1, p);
-        cx<T> thn[JPL];
-        T lrn[JPL];
-        T part = T(0);
-#pragma unroll
-        for (int u = 0; u < JPL; u++) {
-            const int j = lane + 16 * u;
-            thn[u] = th[u];
-            lrn[u] = lcr[u];
-            if (j < M) {
-                thn[u] = th[u] + f * Wsrc[p * M + j];
-                lrn[u] = logcosh_re(thn[u]);
-                part += lrn[u] - lcr[u];
-            }
+    // A sample is recorded as its configuration; log psi, E_loc and O_k are
+    // evaluated afterwards by k_eval_rbm at full occupancy.
+    auto measure_now = [&]() {
+        if (valid && lane == 0) {
+            const int64_t idx = (int64_t)sample * mp.chains + chain;
+            spins_out[2 * idx] = s0;
+            spins_out[2 * idx + 1] = s1;
         }
-        T delta = row_sum16(part);
-        cx<T> ap = mk(T(0), T(0));
-        if (md.a) { ap = md.a[p]; delta += f * ap.re; }
-        const T ratio = (T)exp(T(2) * delta);
-        const T uu = (T)(((double)y + 0.5) * (1.0 / 4294967296.0));
-        const bool accept = (ratio > T(1)) || (uu <= ratio);
-        if (accept) {
-#pragma unroll
-            for (int u = 0; u < JPL; u++) { th[u] = thn[u]; lcr[u] = lrn[u]; }
-            flip_bit(s0, s1, p);
-            vis = vis + f * ap;
-            nacc++;
-        } else {
-            nrej++;
-        }
+        sample++;
+        next_measure += n_sweep;
     };
-
-    for (uint64_t i = 0; i < n_therm; i++) propose(k++);
-    for (int step = 0; step < mp.steps_per_chain; step++) {
-        for (uint64_t i = 0; i < n_sweep; i++) propose(k++);
-        const int64_t sample = (int64_t)step * mp.chains + chain;
-        measure<T, JPL>(md, Wsrc, op, has_op, s0, s1, th, vis, lane, sample, rec);
+    if (n_sweep == 0) {
+        // num_sweeps = 0: every sample is taken right after thermalisation
+        next_measure = ~(uint64_t)0;
     }
-    if (lane == 0 && counts) {
+    if (n_sweep == 0 && n_therm == 0)
+        while (sample < mp.steps_per_chain) measure_now();
+
+    for (uint64_t kb = 0; kb < total; kb += MC_DRAW_BLOCK) {
+        __syncthreads();
+        {
+            const u32x4 r = philox((uint32_t)chain, (uint32_t)((kb >> 1) + lane), mp.call, 0u, mp.seed_lo, mp.seed_hi);
+            draws[2 * lane] = make_uint2(r.x % (uint32_t)N, r.y);
+            draws[2 * lane + 1] = make_uint2(r.z % (uint32_t)N, r.w);
+        }
+        __syncthreads();
+        const int nk = (int)(total - kb < (uint64_t)MC_DRAW_BLOCK ? total - kb : (uint64_t)MC_DRAW_BLOCK);
+        int i = 0;
+        while (i < nk) {
+            // proposals up to the next measurement point or the end of the block
+            const int stop = next_measure > kb && next_measure - kb < (uint64_t)nk ? (int)(next_measure - kb) : nk;
+            uint2 d = draws[i];
+            cx<T> wr[JPL];
+#pragma unroll
+            for (int u = 0; u < JPL; u++) wr[u] = Wsrc[(int)d.x * ld + lane + 16 * u];
+            for (; i < stop; i++) {
+                const uint2 dn = draws[i + 1 < nk ? i + 1 : i];
+                cx<T> wn[JPL];
+#pragma unroll
+                for (int u = 0; u < JPL; u++) wn[u] = Wsrc[(int)dn.x * ld + lane + 16 * u];
+                const int p = (int)d.x;
+                const T f = T(-2) * spin_of<T>(s0, s1, p);
+                cx<T> thn[JPL];
+                T lrn[JPL];
+                T part = T(0);
+                // padded units: theta = theta' = 0, difference 0
+#pragma unroll
+                for (int u = 0; u < JPL; u++) {
+                    thn[u] = th[u] + f * wr[u];
+                    lrn[u] = logcosh_re(thn[u]);
+                    part += lrn[u] - lcr[u];
+                }
+                T delta = row_sum16(part);
+                cx<T> ap = mk(T(0), T(0));
+                if (md.a) { ap = md.a[p]; delta += f * ap.re; }
+                const T ratio = exp_fast(T(2) * delta);
+                const T uu = (T)(((double)d.y + 0.5) * (1.0 / 4294967296.0));
+                const bool accept = (ratio > T(1)) || (uu <= ratio);
+#pragma unroll
+                for (int u = 0; u < JPL; u++) {
+                    th[u] = accept ? thn[u] : th[u];
+                    lcr[u] = accept ? lrn[u] : lcr[u];
+                }
+                const uint64_t bit = (uint64_t)accept;
+                s0 ^= p < 64 ? bit << (p & 63) : 0u;
+                s1 ^= p >= 64 ? bit << (p & 63) : 0u;
+                vis = accept ? vis + f * ap : vis;
+                nacc += accept ? 1u : 0u;
+                d = dn;
+#pragma unroll
+                for (int u = 0; u < JPL; u++) wr[u] = wn[u];
+            }
+            if (kb + (uint64_t)i == next_measure) measure_now();
+            if (n_sweep == 0 && kb + (uint64_t)i == n_therm)
+                while (sample < mp.steps_per_chain) measure_now();
+        }
+    }
+    if (lane == 0 && counts && valid) {
         counts[2 * chain] = nacc;
-        counts[2 * chain + 1] = nrej;
+        counts[2 * chain + 1] = (uint32_t)(total - nacc);
     }
 }
 
@@ -215,11 +263,11 @@ __global__ __launch_bounds__(256) void k_eval_rbm(RbmDev<T> md, OpDev op, int ha
                                                   const uint64_t* __restrict__ configs, uint64_t index_base,
                                                   Records rec) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int N = md.N, M = md.M;
+    const int N = md.N, ld = md.ld;
     const cx<T>* Wsrc = md.W;
     if (WLDS) {
         cx<T>* Wl = reinterpret_cast<cx<T>*>(smem_raw);
-        for (int k = threadIdx.x; k < N * M; k += blockDim.x) Wl[k] = md.W[k];
+        for (int k = threadIdx.x; k < N * ld; k += blockDim.x) Wl[k] = md.W[k];
         __syncthreads();
         Wsrc = Wl;
     }
@@ -440,22 +488,23 @@ static int pick_jpl(int M) {
 }
 
 int max_hidden_units() { return 16 * 16; }
+int rbm_padded_width(int M) { return 16 * pick_jpl(M); }
 
 template <typename T>
-hipError_t launch_mc(const RbmDev<T>& md, const OpDev& op, int has_op, const McParams& mp, const Records& rec,
-                     uint32_t* counts, bool wlds, size_t lds_bytes, hipStream_t stream) {
+hipError_t launch_mc(const RbmDev<T>& md, const McParams& mp, uint64_t* spins, uint32_t* counts, bool wlds,
+                     size_t lds_bytes, hipStream_t stream) {
     const int jpl = pick_jpl(md.M);
     const dim3 block(256), grid((mp.chains + 15) / 16);
-    const size_t shm = wlds ? lds_bytes : 0;
+    const size_t shm = (wlds ? lds_bytes : 0) + MC_DRAW_LDS;
 #define RBM_MC_CASE(J)                                                                                   \
     case J:                                                                                              \
         if (wlds) {                                                                                      \
             if (shm > 65536)                                                                             \
                 (void)hipFuncSetAttribute((const void*)&k_mc_rbm<T, J, true>,                            \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);          \
-            hipLaunchKernelGGL((k_mc_rbm<T, J, true>), grid, block, shm, stream, md, op, has_op, mp, rec, counts); \
+            hipLaunchKernelGGL((k_mc_rbm<T, J, true>), grid, block, shm, stream, md, mp, spins, counts); \
         }                                                                                                \
-        else hipLaunchKernelGGL((k_mc_rbm<T, J, false>), grid, block, 0, stream, md, op, has_op, mp, rec, counts);       \
+        else hipLaunchKernelGGL((k_mc_rbm<T, J, false>), grid, block, shm, stream, md, mp, spins, counts);     \
         break;
     switch (jpl) {
         RBM_JPL_LIST(RBM_MC_CASE)
@@ -546,8 +595,8 @@ hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, in
     return hipGetLastError();
 }
 
-template hipError_t launch_mc<float>(const RbmDev<float>&, const OpDev&, int, const McParams&, const Records&, uint32_t*, bool, size_t, hipStream_t);
-template hipError_t launch_mc<double>(const RbmDev<double>&, const OpDev&, int, const McParams&, const Records&, uint32_t*, bool, size_t, hipStream_t);
+template hipError_t launch_mc<float>(const RbmDev<float>&, const McParams&, uint64_t*, uint32_t*, bool, size_t, hipStream_t);
+template hipError_t launch_mc<double>(const RbmDev<double>&, const McParams&, uint64_t*, uint32_t*, bool, size_t, hipStream_t);
 template hipError_t launch_eval<float>(const RbmDev<float>&, const OpDev&, int, int64_t, const uint64_t*, uint64_t, const Records&, bool, size_t, int, hipStream_t);
 template hipError_t launch_eval<double>(const RbmDev<double>&, const OpDev&, int, int64_t, const uint64_t*, uint64_t, const Records&, bool, size_t, int, hipStream_t);
 template hipError_t launch_ok_reduce<float>(int64_t, int, int, const void*, const uint64_t*, const cx<double>*, int, int64_t, int, cx<double>*, hipStream_t);

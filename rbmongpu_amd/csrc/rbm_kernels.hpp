@@ -9,10 +9,11 @@
 
 namespace rbmhip {
 
-// Device view of an RBM: W row-major N x M, b (M), optional visible bias a (N).
+// Device view of an RBM: W row-major N x ld with ld = rbm_padded_width(M)
+// (columns M..ld-1 are zero), b (ld, zero padded), optional visible bias a (N).
 template <typename T>
 struct RbmDev {
-    int N, M;
+    int N, M, ld;
     const cx<T>* W;
     const cx<T>* b;
     const cx<T>* a;
@@ -52,10 +53,12 @@ struct CoefArgs {
 };
 
 int max_hidden_units();
+// hidden units rounded up to 16 lanes x the compiled units-per-lane count
+int rbm_padded_width(int M);
 
 template <typename T>
-hipError_t launch_mc(const RbmDev<T>& md, const OpDev& op, int has_op, const McParams& mp, const Records& rec,
-                     uint32_t* counts, bool wlds, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_mc(const RbmDev<T>& md, const McParams& mp, uint64_t* spins, uint32_t* counts, bool wlds,
+                     size_t lds_bytes, hipStream_t stream);
 template <typename T>
 hipError_t launch_eval(const RbmDev<T>& md, const OpDev& op, int has_op, int64_t n, const uint64_t* configs,
                        uint64_t index_base, const Records& rec, bool wlds, size_t lds_bytes, int max_blocks,

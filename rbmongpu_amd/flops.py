@@ -23,28 +23,28 @@ TERM_PER_FLIP = 4
 TERM = LOGCOSH + 4
 
 
-def sample_flops(N, M, terms, flips_per_term=1, sweeps=1, with_tanh=True):
-    """FLOP of one measured sample: sweeps*N proposals, log psi, E_loc over
-    the off-diagonal flip-mask terms and (for O_k) my_tanh of every angle."""
-    sweep = sweeps * N * PROPOSAL * M
-    measure = (LOGCOSH + 2) * M
-    e_loc = terms * (TERM + TERM_PER_FLIP * flips_per_term) * M
-    tanh = TANH * M if with_tanh else 0
-    return sweep + measure + e_loc + tanh
+def mc_launch_flops(N, M, chains, samples_per_chain, therm, sweeps=1):
+    """FLOP of one Metropolis launch: (therm + samples * sweeps) * N proposals
+    per chain, PROPOSAL FLOP per hidden unit."""
+    return chains * (therm + samples_per_chain * sweeps) * N * PROPOSAL * M
 
 
-def mc_launch_flops(N, M, terms, chains, samples_per_chain, therm, sweeps=1, flips_per_term=1, with_tanh=True):
-    """FLOP of one Metropolis launch (thermalisation + measured samples)."""
-    per_chain = therm * N * PROPOSAL * M + samples_per_chain * sample_flops(
-        N, M, terms, flips_per_term, sweeps, with_tanh)
-    return chains * per_chain
+def eval_flops(N, M, terms, n, flips_per_term=1, with_tanh=True):
+    """FLOP of evaluating n configurations: angles from scratch (4 N M),
+    log psi, E_loc over the off-diagonal flip-mask terms and my_tanh."""
+    per = 4 * N * M + (LOGCOSH + 2) * M + terms * (TERM + TERM_PER_FLIP * flips_per_term) * M
+    if with_tanh:
+        per += TANH * M
+    return n * per
 
 
-def mc_launch_bytes(N, M, chains, samples_per_chain, tanh_bytes=8, with_tanh=True):
-    """Algorithmic HBM bytes of one launch: W and b read once per workgroup
-    (16 chains), per-sample records written (spins 16 B, log psi 16 B,
-    E_loc 16 B, my_tanh M x tanh_bytes)."""
-    groups = (chains + 15) // 16
-    reads = groups * (N * M + M) * tanh_bytes
-    per_sample = 48 + (M * tanh_bytes if with_tanh else 0)
-    return reads + chains * samples_per_chain * per_sample
+def mc_launch_bytes(chains, samples_per_chain):
+    """Algorithmic HBM bytes of one Metropolis launch: one 16-byte spin
+    record per sample (W is read into LDS once per workgroup)."""
+    return chains * samples_per_chain * 16
+
+
+def eval_bytes(M, n, tanh_bytes=8, with_tanh=True):
+    """Algorithmic HBM bytes of evaluating n recorded samples: read 16-byte
+    configurations, write log psi and E_loc (16 B each) and M my_tanh values."""
+    return n * (16 + 32 + (M * tanh_bytes if with_tanh else 0))
