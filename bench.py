@@ -58,6 +58,30 @@ def tau_int(series, c=5.0):
     return float(tau)
 
 
+def kernel_source_sha():
+    """Hash of the sampler's source: a stored traffic measurement is used only
+    for the kernel it was taken on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("rbm_kernels.hip", "rbm_device.hpp", "rbm_kernels.hpp"):
+        h.update(open(os.path.join(ROOT, "rbmongpu_amd", "csrc", f), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def copy_calibration(nbytes=1 << 30):
+    """A 1 GiB device-to-device hipMemcpy (the runtime's copy kernel reads and
+    writes nbytes): the reference reading for the FETCH_SIZE / WRITE_SIZE passes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def cpu_baseline(budget_s):
     """The oracle's Metropolis + E_loc + O_k gradient on one core, chains of
     the same workload (200 thermalisation sweeps, then measured sweeps)."""
@@ -95,6 +119,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--samples-per-chain", type=int, default=SAMPLES_PER_CHAIN)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (reference kernel for the FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py measures one GPU: the sampler's chains are independent, and this build runs on a "
@@ -142,6 +168,16 @@ def main():
     ev_avg_ms = ev_ms / max(ev_launches, 1)
     ev_flops = F.eval_flops(N, M, terms, CHAINS * spc)
 
+    if args.copy_calibration:
+        copy_calibration()
+
+    traffic_meas = None
+    tpath = os.path.join(ROOT, "profiles", "mc_rbm_c2_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("kernel_source_sha") == kernel_source_sha():
+            traffic_meas = t["bytes_per_launch"]
+
     cpu = None
     if not args.no_cpu_baseline:
         cps, cchains, cspc, cwall = cpu_baseline(args.cpu_budget)
@@ -174,7 +210,7 @@ def main():
         "energy": E.real,
         "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kernel_ms.items()},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_meas,
                      "kernel": "k_mc_rbm", "flop_per_launch": launch_flops, "avg_launch_ms": mc_avg_ms,
                      "algorithmic_bytes_per_launch": traffic},
         "eval_kernel": {"kernel": "k_eval_rbm (log psi, E_loc, my_tanh of the recorded samples)",
