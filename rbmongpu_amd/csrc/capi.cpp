@@ -602,19 +602,35 @@ static Scalars reduce_scalars(Ctx& c, int64_t n, const double* sc) {
     return r;
 }
 
-// R[q][r][col] = sum_s coef_q[s] row_r(s) conj(col(s)) for the RBM rows
-// {1, s_0..s_{N-1}} and columns {T_0..T_{M-1}, 1}.
+// R[r][col] = sum_s coef[s] row_r(s) conj(col(s)) for the RBM rows
+// {1, s_0..s_{N-1}} and columns {T_0..T_{M-1}, 1}: FP32 MFMA on complex64
+// records (chunk partials added in FP64), FP64 VALU on complex128 records.
 template <typename T>
 static std::vector<cd> ok_contract(Ctx& c, int N, int M, int64_t n, const void* tanh, const uint64_t* spins,
-                                   const cx<double>* coef, int nc) {
-    const int64_t plen = (int64_t)nc * (N + 1) * (M + 1);
-    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (plen * 16)));
-    nb = std::min<int64_t>(nb, std::max<int64_t>(1, (n + 63) / 64));
-    const int64_t chunk = (n + nb - 1) / nb;
-    cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * plen) * sizeof(cx<double>)));
+                                   const cx<double>* coef) {
+    const int64_t plen = (int64_t)(N + 1) * (M + 1);
     cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("okout", (size_t)plen * sizeof(cx<double>)));
-    c.launch("ok_reduce", [&] { return launch_ok_reduce<T>(n, N, M, tanh, spins, coef, nc, chunk, (int)nb, part, c.stream); });
-    c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, plen, part, out, c.stream); });
+    if (sizeof(T) == sizeof(float)) {
+        int rows_p = 0, cols_p = 0;
+        ok_mfma_shape(N, M, &rows_p, &cols_p);
+        const int64_t per_chunk = (int64_t)rows_p * cols_p;
+        int64_t nch = std::min<int64_t>(2048, std::max<int64_t>(1, (256ll << 20) / (per_chunk * 4)));
+        nch = std::min<int64_t>(nch, std::max<int64_t>(1, (n + 63) / 64));
+        int64_t chunk = (n + nch - 1) / nch;
+        chunk = (chunk + 3) / 4 * 4;
+        nch = (n + chunk - 1) / chunk;
+        float* part = reinterpret_cast<float*>(c.buf("mfpart", (size_t)(nch * per_chunk) * sizeof(float)));
+        double* psum = reinterpret_cast<double*>(c.buf("mfsum", (size_t)per_chunk * MFMA_SUM_SPLIT * sizeof(double)));
+        c.launch("ok_reduce", [&] { return launch_ok_mfma(n, N, M, tanh, spins, coef, chunk, (int)nch, rows_p, cols_p, part, c.stream); });
+        c.launch("partials_sum", [&] { return launch_mfma_finish((int)nch, N, M, rows_p, cols_p, part, psum, out, c.stream); });
+    } else {
+        int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (plen * 16)));
+        nb = std::min<int64_t>(nb, std::max<int64_t>(1, (n + 63) / 64));
+        const int64_t chunk = (n + nb - 1) / nb;
+        cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * plen) * sizeof(cx<double>)));
+        c.launch("ok_reduce", [&] { return launch_ok_reduce<T>(n, N, M, tanh, spins, coef, 1, chunk, (int)nb, part, c.stream); });
+        c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, plen, part, out, c.stream); });
+    }
     std::vector<cd> h((size_t)plen);
     c.d2h(h.data(), out, h.size() * sizeof(cx<double>));
     c.sync();
@@ -634,42 +650,32 @@ static void rbm_params_from_R(const rbmhip_psi* psi, const cd* R, std::vector<cd
         for (int i = 0; i < N; i++) out[2 * N + M + (size_t)N * M + i] = R[(size_t)(i + 1) * C + M];
 }
 
+// sum_s coef(s) conj(O_k(s)) in parameter order (entries below 2N are zero).
 template <typename T>
 static std::vector<cd> contract_params(Ctx& c, rbmhip_psi* psi, int64_t n, const void* tanh, const uint64_t* spins,
-                                       const cx<double>* coef, int nc) {
+                                       const cx<double>* coef) {
+    std::vector<cd> one;
     if (psi->kind == 0) {
-        std::vector<cd> R = ok_contract<T>(c, psi->N, psi->M, n, tanh, spins, coef, nc);
-        const size_t plen1 = (size_t)(psi->N + 1) * (psi->M + 1);
-        std::vector<cd> all;
-        for (int q = 0; q < nc; q++) {
-            std::vector<cd> one;
-            rbm_params_from_R(psi, R.data() + q * plen1, one);
-            all.insert(all.end(), one.begin(), one.end());
-        }
-        return all;
+        std::vector<cd> R = ok_contract<T>(c, psi->N, psi->M, n, tanh, spins, coef);
+        rbm_params_from_R(psi, R.data(), one);
+        return one;
     }
     // PsiDeep: O_k rows are materialised per sample (deep kernels) and reduced
     // column-wise.
     const int64_t K = psi->num_params() - 2 * (int64_t)psi->N;
-    const int64_t plen = (int64_t)nc * K;
-    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (plen * 16)));
+    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (K * 16)));
     nb = std::min<int64_t>(nb, std::max<int64_t>(1, (n + 63) / 64));
     const int64_t chunk = (n + nb - 1) / nb;
-    cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * plen) * sizeof(cx<double>)));
-    cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("okout", (size_t)plen * sizeof(cx<double>)));
-    c.launch("ok_reduce", [&] { return launch_rows_reduce<T>(n, K, tanh, coef, nc, chunk, (int)nb, part, c.stream); });
-    c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, plen, part, out, c.stream); });
-    std::vector<cd> h((size_t)plen);
+    cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * K) * sizeof(cx<double>)));
+    cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("okout", (size_t)K * sizeof(cx<double>)));
+    c.launch("ok_reduce", [&] { return launch_rows_reduce<T>(n, K, tanh, coef, 1, chunk, (int)nb, part, c.stream); });
+    c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, K, part, out, c.stream); });
+    std::vector<cd> h((size_t)K);
     c.d2h(h.data(), out, h.size() * sizeof(cx<double>));
     c.sync();
-    std::vector<cd> all;
-    const int64_t P = psi->num_params();
-    for (int q = 0; q < nc; q++) {
-        std::vector<cd> one((size_t)P, cd(0, 0));
-        for (int64_t k = 0; k < K; k++) one[(size_t)(2 * psi->N + k)] = h[(size_t)(q * K + k)];
-        all.insert(all.end(), one.begin(), one.end());
-    }
-    return all;
+    one.assign((size_t)psi->num_params(), cd(0, 0));
+    for (int64_t k = 0; k < K; k++) one[(size_t)(2 * psi->N + k)] = h[(size_t)k];
+    return one;
 }
 
 template <class F>
@@ -1194,18 +1200,22 @@ int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* gra
         with_precision(psi->precision, [&](auto tag) {
             using T = decltype(tag);
             Samples s = sample<T>(c, psi, op, ens, true, true);
-            cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * 2 * sizeof(cx<double>)));
-            Scalars r = energy_scalars<T>(c, psi, s, ens, coef);
-            std::vector<cd> R = contract_params<T>(c, psi, s.n, s.tanh, s.spins, coef, 2);
+            Scalars r = energy_scalars<T>(c, psi, s, ens, nullptr);
             const int64_t P = psi->num_params();
-            const double inv = 1.0 / (double)ens->num_steps();
-            const cd E(r.v[1] * inv, r.v[2] * inv);
+            const double n = (double)ens->num_steps();
+            const cd E(r.v[1] / n, r.v[2] / n);
             // ExpectationValue.cu:243-253 with conj<E_k> = <E conj(O_k)> (Hermitian):
-            // g_k = 2 <E conj O_k> - 2 <E> conj<O_k>, where R0 = sum w conj(O_k)
-            // and R1 = sum w E conj(O_k).
+            // g_k = 2 <E conj O_k> - 2 <E> conj<O_k> = (2/n) sum_s w (E - <E>) conj(O_k)
+            cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * sizeof(cx<double>)));
+            CoefArgs a;
+            std::memset(&a, 0, sizeof(a));
+            a.n = s.n; a.mode = 2; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs;
+            a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.el = s.el; a.coef = coef;
+            a.e_mean = mk(E.real(), E.imag());
+            c.launch("coef", [&] { return launch_coef(a, c.stream); });
+            std::vector<cd> R = contract_params<T>(c, psi, s.n, s.tanh, s.spins, coef);
             for (int64_t k = 0; k < P; k++) {
-                const cd R0 = R[(size_t)k] * inv, R1 = R[(size_t)(P + k)] * inv;
-                const cd g = 2.0 * R1 - 2.0 * E * R0;
+                const cd g = (2.0 / n) * R[(size_t)k];
                 grad[2 * k] = g.real();
                 grad[2 * k + 1] = g.imag();
             }
@@ -1265,35 +1275,39 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
         });
         const uint64_t* spins_for_grad = s.spins ? s.spins : spins2;
         double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
-        cx<double>* coef = want_grad ? reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * 2 * sizeof(cx<double>))) : nullptr;
         CoefArgs a;
         std::memset(&a, 0, sizeof(a));
         a.n = s.n; a.mode = 1; a.exact = ens->kind == 0; a.is_unitary = is_unitary ? 1 : 0;
         a.num_configs = (double)ens->num_configs; a.log_prefactor = std::log(psi->prefactor);
-        a.lp = s.lp; a.el = s.el; a.lp2 = lp2; a.coef = coef; a.sc = sc;
+        a.lp = s.lp; a.el = s.el; a.lp2 = lp2; a.sc = sc;
         c.launch("coef", [&] { return launch_coef(a, c.stream); });
         Scalars r = reduce_scalars(c, s.n, sc);
-        const double inv = 1.0 / (double)ens->num_steps();
-        const cd omega(r.v[0] * inv, r.v[1] * inv);
-        const double ratio = r.v[2] * inv, nrm = r.v[3] * inv;
+        const double n = (double)ens->num_steps();
+        const cd omega(r.v[0] / n, r.v[1] / n);
+        const double ratio = r.v[2] / n, nrm = r.v[3] / n;
         // HilbertSpaceDistance.cu:216-220, 248-262
         const double u = std::norm(omega);
         const double v = nrm * ratio;
         const double d = std::sqrt(1.0 - u / v);
         *distance = d;
         if (want_grad) {
+            // g_k = -(conj(w) <w O*> v - u n <r O*>) / v^2 / d^nu
+            //     = -conj(w) n_avg / (n v^2 d^nu) * sum_s w (r_avg omega - omega_avg r) conj(O_k)
+            cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * sizeof(cx<double>)));
+            a.mode = 3; a.coef = coef; a.sc = nullptr;
+            a.omega_mean = mk(omega.real(), omega.imag());
+            a.ratio_mean = ratio;
+            c.launch("coef", [&] { return launch_coef(a, c.stream); });
             std::vector<cd> R;
             with_precision(psi_prime->precision, [&](auto tag) {
                 using T = decltype(tag);
-                R = contract_params<T>(c, psi_prime, s.n, tanh2, spins_for_grad, coef, 2);
+                R = contract_params<T>(c, psi_prime, s.n, tanh2, spins_for_grad, coef);
             });
             const int64_t P = psi_prime->num_params();
             const double pref = std::pow(d, (double)(float)nu);
+            const cd fac = -std::conj(omega) * nrm / (n * v * v * pref);
             for (int64_t k = 0; k < P; k++) {
-                const cd oO = R[(size_t)k] * inv, rO = R[(size_t)(P + k)] * inv;
-                const cd ukp = std::conj(omega) * oO;
-                const cd vkp = nrm * rO;
-                const cd g = (-(ukp * v - u * vkp) / (v * v)) / pref;
+                const cd g = fac * R[(size_t)k];
                 grad[2 * k] = g.real();
                 grad[2 * k + 1] = g.imag();
             }

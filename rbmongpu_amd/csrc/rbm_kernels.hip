@@ -299,12 +299,13 @@ __global__ __launch_bounds__(256) void k_coef(CoefArgs a) {
         const cx<double> lp = a.lp[s];
         const double w = a.exact ? a.num_configs * exp(2.0 * (a.log_prefactor + lp.re)) : 1.0;
         const cx<double> E = a.el ? a.el[s] : mk(0.0, 0.0);
-        double* sc = a.sc + s * SC_STRIDE;
-        if (a.mode == 0) {
-            if (a.coef) {
-                a.coef[s] = mk(w, 0.0);
-                a.coef[a.n + s] = w * E;
+        if (a.mode == 0 || a.mode == 2) {
+            if (a.mode == 2) {
+                // w (E - <E>): sum_s coef conj(O_k) = n (<E conj O_k> - <E> conj<O_k>)
+                a.coef[s] = w * (E - a.e_mean);
+                continue;
             }
+            double* sc = a.sc + s * SC_STRIDE;
             sc[0] = w; sc[1] = w * E.re; sc[2] = w * E.im; sc[3] = w * (E.re * E.re + E.im * E.im);
         } else {
             const cx<double> lpp = a.lp2[s];
@@ -319,10 +320,13 @@ __global__ __launch_bounds__(256) void k_coef(CoefArgs a) {
                 nrm = exp(2.0 * E.re);
             }
             const double ratio = exp(2.0 * (lpp.re - lp.re));
-            if (a.coef) {
-                a.coef[s] = w * omega;
-                a.coef[a.n + s] = mk(w * ratio, 0.0);
+            if (a.mode == 3) {
+                // w (r_mean omega - omega_mean r): the numerator of the distance
+                // gradient in one coefficient (HilbertSpaceDistance.cu:257-262)
+                a.coef[s] = w * (a.ratio_mean * omega - ratio * a.omega_mean);
+                continue;
             }
+            double* sc = a.sc + s * SC_STRIDE;
             sc[0] = w * omega.re; sc[1] = w * omega.im; sc[2] = w * ratio; sc[3] = w * nrm;
         }
     }
@@ -426,6 +430,169 @@ __global__ __launch_bounds__(256) void k_ok_reduce(int64_t n, int N, int M, cons
                 if (row < R) partials[(int64_t)blockIdx.x * plen + ((int64_t)q * R + row) * C + col] = acc[q][r];
             }
     }
+}
+
+// ---------------------------------------------------------------------------
+// O_k contraction on FP32 MFMA (v_mfma_f32_16x16x4f32).  Real GEMM over the
+// samples:  P[m][n] = sum_s A_m(s) B_n(s),
+//   m = 2r + p: A = (p ? Im : Re)(coef(s)) * row_r(s), rows {1, s_0..s_{N-1}}
+//   n = 2j + p: B = (p ? Im : Re)(col_j(s)), columns {T_0..T_{M-1}, 1}
+// so that R[r][j] = sum_s coef row_r conj(col_j) = (P_rr + P_ii) + i (P_ir - P_ri).
+// A K-step takes 4 samples: lane l feeds sample l/16 and tile row/column l%16
+// (A: lane l holds A[l%16][l/16], B: B[l/16][l%16], D: D[4(l/16)+v][l%16]).
+// One wavefront owns up to 4 x 8 output tiles of one sample chunk, accumulates
+// the chunk in FP32 and writes its partials; chunks are added in FP64.  The
+// coefficients are centred (zero mean), so the FP32 chunk sums carry no large
+// cancellation.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// RT x CT output tiles per wavefront, fixed at compile time so the K loop has
+// no branches and the next K-step's loads overlap the current MFMAs.  Rows
+// past 2(N+1) and columns past 2(M+1) are padding and contribute zero.
+template <int RT, int CT>
+__global__ __launch_bounds__(64) void k_ok_mfma(int64_t n, int N, int M, const float* __restrict__ Tf,
+                                                const uint64_t* __restrict__ spins,
+                                                const cx<double>* __restrict__ coef, int64_t chunk, int rows_p,
+                                                int cols_p, float* __restrict__ partials) {
+    const int lane = threadIdx.x;
+    const int kk = lane >> 4, ii = lane & 15;
+    const int rt0 = blockIdx.y * RT, ct0 = blockIdx.z * CT;
+    floatx4 acc[RT][CT];
+#pragma unroll
+    for (int a = 0; a < RT; a++)
+#pragma unroll
+        for (int b = 0; b < CT; b++) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    int rrow[RT], rpart[RT], coff[CT];
+    float cfix[CT], cload[CT];
+#pragma unroll
+    for (int a = 0; a < RT; a++) { const int m = 16 * (rt0 + a) + ii; rrow[a] = m >> 1; rpart[a] = m & 1; }
+#pragma unroll
+    for (int b = 0; b < CT; b++) {
+        const int q = 16 * (ct0 + b) + ii, j = q >> 1;
+        coff[b] = j < M ? q : 0;                         // clamped: always a valid load
+        cload[b] = j < M ? 1.f : 0.f;                    // take the loaded value
+        cfix[b] = (j == M && (q & 1) == 0) ? 1.f : 0.f;  // the constant column
+    }
+    const int64_t s_begin = (int64_t)blockIdx.x * chunk;
+    const int64_t s_end = s_begin + chunk < n ? s_begin + chunk : n;
+    const int twoM = 2 * M;
+#pragma unroll 2
+    for (int64_t s0 = s_begin; s0 < s_end; s0 += 4) {
+        const int64_t s = s0 + kk;
+        const bool ok = s < s_end;
+        const int64_t sc = ok ? s : s_begin;
+        const cx<double> c = coef[sc];
+        const uint64_t w0 = spins[2 * sc], w1 = spins[2 * sc + 1];
+        const float cre = ok ? (float)c.re : 0.f, cim = ok ? (float)c.im : 0.f;
+        const float* Tr = Tf + sc * twoM;
+        float bv[CT];
+#pragma unroll
+        for (int b = 0; b < CT; b++) bv[b] = Tr[coff[b]] * cload[b] + cfix[b];
+        float av[RT];
+#pragma unroll
+        for (int a = 0; a < RT; a++) {
+            const int r = rrow[a];
+            const float sg = r == 0 ? 1.f : (r <= N ? spin_of<float>(w0, w1, (r - 1) & 127) : 0.f);
+            av[a] = (rpart[a] ? cim : cre) * sg;
+        }
+#pragma unroll
+        for (int a = 0; a < RT; a++)
+#pragma unroll
+            for (int b = 0; b < CT; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    float* out = partials + (int64_t)blockIdx.x * rows_p * cols_p;
+#pragma unroll
+    for (int a = 0; a < RT; a++)
+#pragma unroll
+        for (int b = 0; b < CT; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                out[(int64_t)(16 * (rt0 + a) + 4 * kk + v) * cols_p + 16 * (ct0 + b) + ii] = acc[a][b][v];
+}
+
+// Sum of FP32 chunk partials in FP64: block (x, y) adds chunks y, y + S, ... of
+// 64 consecutive elements with 4 threads per element; out[y][len].
+__global__ __launch_bounds__(256) void k_fsum(int nchunks, int S, int64_t len, const float* __restrict__ partials,
+                                              double* __restrict__ out) {
+    __shared__ double red[4][64];
+    const int e_local = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * 64 + e_local;
+    double acc = 0.0;
+    if (e < len)
+        for (int b = blockIdx.y + S * sub; b < nchunks; b += 4 * S) acc += (double)partials[(int64_t)b * len + e];
+    red[sub][e_local] = acc;
+    __syncthreads();
+    if (sub == 0 && e < len)
+        out[(int64_t)blockIdx.y * len + e] = red[0][e_local] + red[1][e_local] + red[2][e_local] + red[3][e_local];
+}
+
+__global__ __launch_bounds__(256) void k_mfma_combine(int N, int M, int cols_p, int S, int64_t len,
+                                                      const double* __restrict__ P, cx<double>* __restrict__ R) {
+    const int C = M + 1;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < (N + 1) * C; e += gridDim.x * blockDim.x) {
+        const int r = e / C, j = e % C;
+        double rr = 0, ri = 0, ir = 0, ii = 0;
+        for (int y = 0; y < S; y++) {
+            const double* Py = P + (int64_t)y * len;
+            rr += Py[(int64_t)(2 * r) * cols_p + 2 * j];
+            ri += Py[(int64_t)(2 * r) * cols_p + 2 * j + 1];
+            ir += Py[(int64_t)(2 * r + 1) * cols_p + 2 * j];
+            ii += Py[(int64_t)(2 * r + 1) * cols_p + 2 * j + 1];
+        }
+        R[e] = mk(rr + ii, ir - ri);
+    }
+}
+
+static void mfma_tiling(int N, int M, int& RT, int& CT, int& rows_p, int& cols_p) {
+    const int rt = (2 * (N + 1) + 15) / 16, ct = (2 * (M + 1) + 15) / 16;
+    RT = rt < 4 ? rt : 4;
+    static const int cts[] = {4, 8, 11, 13};
+    const int groups = (ct + 12) / 13;
+    const int need = (ct + groups - 1) / groups;
+    CT = 13;
+    for (int c : cts) if (c >= need) { CT = c; break; }
+    rows_p = 16 * RT * ((rt + RT - 1) / RT);
+    cols_p = 16 * CT * ((ct + CT - 1) / CT);
+}
+
+void ok_mfma_shape(int N, int M, int* rows_p, int* cols_p) {
+    int RT, CT;
+    mfma_tiling(N, M, RT, CT, *rows_p, *cols_p);
+}
+
+hipError_t launch_ok_mfma(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
+                          int64_t chunk, int nchunks, int rows_p, int cols_p, float* partials, hipStream_t stream) {
+    int RT, CT, rp, cp;
+    mfma_tiling(N, M, RT, CT, rp, cp);
+    if (rp != rows_p || cp != cols_p) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)nchunks, (unsigned)(rows_p / (16 * RT)), (unsigned)(cols_p / (16 * CT)));
+    const float* Tf = reinterpret_cast<const float*>(Trec);
+#define RBM_MF_CASE(R_, C_)                                                                                       \
+    if (RT == R_ && CT == C_) {                                                                                   \
+        hipLaunchKernelGGL((k_ok_mfma<R_, C_>), grid, dim3(64), 0, stream, n, N, M, Tf, spins, coef, chunk, rows_p, \
+                           cols_p, partials);                                                                     \
+        return hipGetLastError();                                                                                 \
+    }
+#define RBM_MF_ROW(R_) RBM_MF_CASE(R_, 4) RBM_MF_CASE(R_, 8) RBM_MF_CASE(R_, 11) RBM_MF_CASE(R_, 13)
+    RBM_MF_ROW(1) RBM_MF_ROW(2) RBM_MF_ROW(3) RBM_MF_ROW(4)
+#undef RBM_MF_ROW
+#undef RBM_MF_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_mfma_finish(int nchunks, int N, int M, int rows_p, int cols_p, const float* partials, double* Psum,
+                              cx<double>* R, hipStream_t stream) {
+    const int64_t len = (int64_t)rows_p * cols_p;
+    const int S = MFMA_SUM_SPLIT;
+    hipLaunchKernelGGL(k_fsum, dim3((unsigned)((len + 63) / 64), (unsigned)S), dim3(256), 0, stream, nchunks, S, len,
+                       partials, Psum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int outs = (N + 1) * (M + 1);
+    hipLaunchKernelGGL(k_mfma_combine, dim3((unsigned)((outs + 255) / 256)), dim3(256), 0, stream, N, M, cols_p, S,
+                       len, Psum, R);
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void k_partials_sum(int nblocks, int64_t len, const cx<double>* __restrict__ partials,

@@ -40,7 +40,7 @@ constexpr int OK_ROWS_PER_THREAD = 8;
 
 struct CoefArgs {
     int64_t n;
-    int mode;            // 0 energy, 1 Hilbert-space distance
+    int mode;            // 0 energy, 1 Hilbert-space distance, 2 centred energy, 3 centred distance
     int exact;           // ExactSummation weights
     int is_unitary;
     double num_configs;
@@ -48,8 +48,11 @@ struct CoefArgs {
     const cx<double>* lp;
     const cx<double>* el;
     const cx<double>* lp2;
-    cx<double>* coef;    // 2 x n, may be null
-    double* sc;          // SC_STRIDE x n
+    cx<double>* coef;    // n (modes 2, 3), may be null
+    double* sc;          // SC_STRIDE x n (modes 0, 1)
+    cx<double> e_mean;   // mode 2: sum w E / n
+    cx<double> omega_mean;  // mode 3
+    double ratio_mean;      // mode 3
 };
 
 int max_hidden_units();
@@ -66,6 +69,17 @@ hipError_t launch_eval(const RbmDev<T>& md, const OpDev& op, int has_op, int64_t
 template <typename T>
 hipError_t launch_ok_reduce(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
                             int nc, int64_t chunk, int nblocks, cx<double>* partials, hipStream_t stream);
+// FP32 MFMA contraction (complex64 records): P[m][n] = sum_s A_m(s) B_n(s)
+// with m = 2r + part (rows {1, s_i}, Re/Im of the coefficient) and
+// n = 2j + part (columns {T_j, 1}, Re/Im); per-chunk FP32 partials.
+// padded row / column counts of the MFMA contraction for an N x M RBM
+void ok_mfma_shape(int N, int M, int* rows_p, int* cols_p);
+constexpr int MFMA_SUM_SPLIT = 16;   // FP64 partial-sum groups (Psum holds MFMA_SUM_SPLIT x rows_p x cols_p)
+hipError_t launch_ok_mfma(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
+                          int64_t chunk, int nchunks, int rows_p, int cols_p, float* partials, hipStream_t stream);
+// FP64 sum of FP32 partials over chunks, then R[r][j] (complex) from P.
+hipError_t launch_mfma_finish(int nchunks, int N, int M, int rows_p, int cols_p, const float* partials, double* Psum,
+                              cx<double>* R, hipStream_t stream);
 hipError_t launch_partials_sum(int nblocks, int64_t len, const cx<double>* partials, cx<double>* out, hipStream_t stream);
 hipError_t launch_coef(const CoefArgs& a, hipStream_t stream);
 hipError_t launch_scalar_partials(int64_t n, const double* sc, int nblocks, double* partials, hipStream_t stream);
