@@ -38,11 +38,22 @@ template <typename T> RBM_DI cx<T> cdiv(cx<T> a, cx<T> b) {
     return mk((a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r);
 }
 
+RBM_DI float fm(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+RBM_DI double fm(double a, double b, double c) { return __builtin_fma(a, b, c); }
+RBM_DI float fab(float x) { return __builtin_fabsf(x); }
+RBM_DI double fab(double x) { return __builtin_fabs(x); }
+
+// exp(z) in FP32 from the hardware transcendentals: the phase is reduced to
+// [-pi, pi] with a two-constant Cody-Waite step, then v_sin/v_cos take it in
+// revolutions.  Measured on gfx950 against FP64: |error| <= 3.8e-7 for sin
+// and cos over |Im z| <= 80, relative error of exp <= 6e-7 for |Re z| <= 8.
 RBM_DI cx<float> cexp(cx<float> z) {
-    float s, c;
-    sincosf(z.im, &s, &c);
-    const float e = expf(z.re);
-    return mk(e * c, e * s);
+    const float k = __builtin_rintf(z.im * 0.15915494309189535f);
+    float r = __builtin_fmaf(-k, 6.28318548202514648f, z.im);
+    r = __builtin_fmaf(-k, -1.7484555e-07f, r);
+    const float rev = r * 0.15915494309189535f;
+    const float e = __expf(z.re);
+    return mk(e * __builtin_amdgcn_cosf(rev), e * __builtin_amdgcn_sinf(rev));
 }
 RBM_DI cx<double> cexp(cx<double> z) {
     double s, c;
@@ -62,25 +73,54 @@ template <typename T> RBM_DI cx<T> lc_den(cx<T> z, T sg, T c1) {
     return d;
 }
 
-// my_logcosh: include/quantum_state/psi_functions.hpp:49-57.  The branch
-// follows the reference exactly: sign = Re z > 0 ? 1 : -1.
+// my_logcosh (include/quantum_state/psi_functions.hpp:49-57):
+//   lc(z) = sg k z + (n0 - sg n1 z) / D_sg(z) - c,   sg = Re z > 0 ? 1 : -1,
+//   D_sg(z) = z^4 + sg a3 z^3 + a2 z^2 + sg a1 z + a0.
+// Every odd power of z carries one factor sg, so with w = sg z the function is
+// L(w) = k w + (n0 - n1 w) / D_+(w) - c.  L has real coefficients, hence
+// L(conj w) = conj L(w); with v = |Re z| + i Im z (w = v for sg = 1 and
+// w = conj v for sg = -1):
+//   Re lc(z) = Re L(v),   Im lc(z) = sg Im L(v).
+// This evaluates the reference's function (its branch at Re z = 0 included)
+// without sign-dependent coefficients.
+constexpr double LC_A3 = 3.746646023906276, LC_A2 = 7.771429504240965, LC_A1 = 10.2180213465,
+                 LC_A0 = 9.19376335670885, LC_N0 = 5.49914721954, LC_N1 = 2.16564366435,
+                 LC_K = 0.9003320053750442, LC_C = 0.598139;
+
+// D_+(w) = (((w + a3) w + a2) w + a1) w + a0 at w = wr + i wi
+template <typename T> RBM_DI void lc_den_w(T wr, T wi, T& dr, T& di) {
+    const T ur = wr + T(LC_A3);
+    const T vr = fm(wr, ur, fm(-wi, wi, T(LC_A2)));
+    const T vi = wi * (wr + ur);
+    const T xr = fm(wr, vr, fm(-wi, vi, T(LC_A1)));
+    const T xi = fm(wr, vi, wi * vr);
+    dr = fm(wr, xr, fm(-wi, xi, T(LC_A0)));
+    di = fm(wr, xi, wi * xr);
+}
+
 template <typename T> RBM_DI cx<T> logcosh(cx<T> z) {
-    const T sg = z.re > T(0) ? T(1) : T(-1);
-    const cx<T> d = lc_den(z, sg, T(10.2180213465));
-    const cx<T> n = mk(T(5.49914721954) - sg * T(2.16564366435) * z.re, -sg * T(2.16564366435) * z.im);
-    const cx<T> q = cdiv(n, d);
-    const T k = sg * T(0.9003320053750442);
-    return mk(k * z.re + q.re - T(0.598139), k * z.im + q.im);
+    const T wr = fab(z.re), wi = z.im;
+    T dr, di;
+    lc_den_w(wr, wi, dr, di);
+    const T nr = fm(T(-LC_N1), wr, T(LC_N0));
+    const T ni = T(-LC_N1) * wi;
+    const T r = rcp(fm(dr, dr, di * di));
+    const T qr = fm(nr, dr, ni * di);
+    const T qi = fm(ni, dr, -(nr * di));
+    const T lr = fm(qr, r, fm(T(LC_K), wr, T(-LC_C)));
+    const T li = fm(qi, r, T(LC_K) * wi);
+    return mk(lr, z.re > T(0) ? li : -li);
 }
 
 // Real part of my_logcosh only (the Metropolis test needs Re log psi).
 template <typename T> RBM_DI T logcosh_re(cx<T> z) {
-    const T sg = z.re > T(0) ? T(1) : T(-1);
-    const cx<T> d = lc_den(z, sg, T(10.2180213465));
-    const T nr = T(5.49914721954) - sg * T(2.16564366435) * z.re;
-    const T ni = -sg * T(2.16564366435) * z.im;
-    const T qr = (nr * d.re + ni * d.im) * rcp(d.re * d.re + d.im * d.im);
-    return sg * T(0.9003320053750442) * z.re + qr - T(0.598139);
+    const T wr = fab(z.re), wi = z.im;
+    T dr, di;
+    lc_den_w(wr, wi, dr, di);
+    const T nr = fm(T(-LC_N1), wr, T(LC_N0));
+    const T num = fm(nr, dr, T(-LC_N1) * (wi * di));
+    const T r = rcp(fm(dr, dr, di * di));
+    return fm(num, r, fm(T(LC_K), wr, T(-LC_C)));
 }
 
 // my_tanh: include/quantum_state/psi_functions.hpp:97-114.

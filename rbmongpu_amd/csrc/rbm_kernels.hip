@@ -18,30 +18,36 @@ namespace rbmhip {
 // ---------------------------------------------------------------------------
 // local energy (include/operator/Operator.hpp:90-115, restated on canonical
 // flip-mask terms): E_loc(s) = diag(s) + sum_t c_t(s) exp(log psi(s^m_t) - log psi(s)).
-// Each lane adds its hidden units' lc(theta') - lc(theta); the row sums.
+// Each lane sums lc(theta'_j) over its hidden units and subtracts lsum, the
+// same sum at theta (padded units, theta = theta' = 0, cancel); the row adds
+// the lanes.  Every off-diagonal term flips at least one spin, so theta' is
+// initialised from the first flip without a copy of theta.
 // ---------------------------------------------------------------------------
 template <typename T, int JPL>
 RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const OpDev& op,
-                                   uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], const cx<T> (&lcv)[JPL],
-                                   int lane) {
+                                   uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], cx<T> lsum, int lane) {
     const int ld = md.ld;
+    const cx<T>* __restrict__ Wl = Wsrc + lane;
     cx<double> E = diag_energy(op, s0, s1);
     for (int t = 0; t < op.n_terms; t++) {
         const int fb = op.flip_begin[t], fe = op.flip_begin[t + 1];
         cx<T> v[JPL];
+        {
+            const int pos = op.flips[fb];
+            const T f = T(-2) * spin_of<T>(s0, s1, pos);
 #pragma unroll
-        for (int u = 0; u < JPL; u++) v[u] = th[u];
-        for (int q = fb; q < fe; q++) {
+            for (int u = 0; u < JPL; u++) v[u] = th[u] + f * Wl[pos * ld + 16 * u];
+        }
+        for (int q = fb + 1; q < fe; q++) {
             const int pos = op.flips[q];
             const T f = T(-2) * spin_of<T>(s0, s1, pos);
 #pragma unroll
-            for (int u = 0; u < JPL; u++) v[u] = v[u] + f * Wsrc[pos * ld + lane + 16 * u];
+            for (int u = 0; u < JPL; u++) v[u] = v[u] + f * Wl[pos * ld + 16 * u];
         }
-        // padded units have theta = theta' = 0 and contribute nothing
-        cx<T> part = mk(T(0), T(0));
+        cx<T> part = logcosh(v[0]);
 #pragma unroll
-        for (int u = 0; u < JPL; u++) part = part + (logcosh(v[u]) - lcv[u]);
-        cx<T> d = row_sum16(part);
+        for (int u = 1; u < JPL; u++) part = part + logcosh(v[u]);
+        cx<T> d = row_sum16(part - lsum);
         if (md.a) {
             for (int q = fb; q < fe; q++) {
                 const int pos = op.flips[q];
@@ -81,17 +87,18 @@ RBM_DI void measure(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const O
                     uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], cx<T> vis, int lane, int64_t sample,
                     const Records& rec, bool valid = true) {
     const int M = md.M;
-    cx<T> lcv[JPL];
-    cx<T> lsum = mk(T(0), T(0));
+    cx<T> lsum = mk(T(0), T(0));   // hidden units of the model
+    cx<T> lall = logcosh(th[0]);   // all slots, padded ones included
+    lsum = lane < M ? lall : lsum;
 #pragma unroll
-    for (int u = 0; u < JPL; u++) {
-        lcv[u] = logcosh(th[u]);
-        const bool real_unit = lane + 16 * u < M;
-        lsum = lsum + (real_unit ? lcv[u] : mk(T(0), T(0)));
+    for (int u = 1; u < JPL; u++) {
+        const cx<T> lc = logcosh(th[u]);
+        lall = lall + lc;
+        lsum = lsum + (lane + 16 * u < M ? lc : mk(T(0), T(0)));
     }
     const cx<double> lp = row_sum16(to_d(lsum)) + to_d(vis);
     cx<double> E = mk(0.0, 0.0);
-    if (has_op) E = local_energy_rbm<T, JPL>(md, Wsrc, op, s0, s1, th, lcv, lane);
+    if (has_op) E = local_energy_rbm<T, JPL>(md, Wsrc, op, s0, s1, th, lall, lane);
     if (!valid) return;
     if (lane == 0) {
         if (rec.spins) { rec.spins[2 * sample] = s0; rec.spins[2 * sample + 1] = s1; }
@@ -127,10 +134,26 @@ RBM_DI void measure(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const O
 constexpr int MC_DRAW_BLOCK = 32;
 constexpr size_t MC_DRAW_LDS = 16 * MC_DRAW_BLOCK * sizeof(uint2);
 
-RBM_DI float exp_fast(float x) { return __expf(x); }
-RBM_DI double exp_fast(double x) { return exp(x); }
+// exp(2 delta), the ratio |psi(s')/psi(s)|^2
+RBM_DI float ratio_of(float delta) { return __builtin_amdgcn_exp2f(delta * 2.8853900817779268f); }
+RBM_DI double ratio_of(double delta) { return exp(2.0 * delta); }
 
-template <typename T, int JPL, bool WLDS>
+// A draw in LDS is (position << 24 | position * ld, U) where U holds the
+// uniform u = (y + 0.5) 2^-32: as FP32 bits for FP32 chains, as y itself for
+// FP64 chains (which form u in FP64 like the FP64 restatement).
+RBM_DI uint32_t pack_u(uint32_t y, float) { return __float_as_uint((float)(((double)y + 0.5) * (1.0 / 4294967296.0))); }
+RBM_DI uint32_t pack_u(uint32_t y, double) { return y; }
+RBM_DI float unpack_u(uint32_t v, float) { return __uint_as_float(v); }
+RBM_DI double unpack_u(uint32_t v, double) { return ((double)v + 0.5) * (1.0 / 4294967296.0); }
+
+// -2 s_p from the word of a configuration with N <= 32, shifted so that bit
+// p is its sign bit: the FP32 value is assembled from bits (+2 or -2).
+RBM_DI float flip_factor(uint32_t top) { return __uint_as_float((top & 0x80000000u) | 0x40000000u); }
+template <typename T> RBM_DI T flip_factor_t(uint32_t top) { return (top & 0x80000000u) ? T(-2) : T(2); }
+template <> RBM_DI float flip_factor_t<float>(uint32_t top) { return flip_factor(top); }
+
+// NARROW: N <= 32, the configuration is one 32-bit word.
+template <typename T, int JPL, bool WLDS, bool NARROW>
 __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint64_t* __restrict__ spins_out,
                                                 uint32_t* __restrict__ counts) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -153,13 +176,17 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
     uint64_t s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
     uint64_t s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
     mask_words(s0, s1, N);
+    uint32_t w32 = (uint32_t)s0;
 
     cx<T> th[JPL];
     init_angles<T, JPL>(md, Wsrc, s0, s1, lane, th);
-    T lcr[JPL];
+    // Re lc(theta) summed over this lane's units (padded units included: they
+    // stay at theta = 0 and cancel in every difference); a proposal's
+    // log-ratio is the row sum of (the same sum at theta') - lane_lc.
+    T lane_lc = logcosh_re(th[0]);
 #pragma unroll
-    for (int u = 0; u < JPL; u++) lcr[u] = logcosh_re(th[u]);
-    cx<T> vis = visible_sum(md, s0, s1);
+    for (int u = 1; u < JPL; u++) lane_lc += logcosh_re(th[u]);
+    const cx<T>* __restrict__ Wl = Wsrc + lane;
 
     const uint64_t n_therm = (uint64_t)mp.therm * N;
     const uint64_t n_sweep = (uint64_t)mp.sweeps * N;
@@ -173,8 +200,8 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
     auto measure_now = [&]() {
         if (valid && lane == 0) {
             const int64_t idx = (int64_t)sample * mp.chains + chain;
-            spins_out[2 * idx] = s0;
-            spins_out[2 * idx + 1] = s1;
+            spins_out[2 * idx] = NARROW ? (uint64_t)w32 : s0;
+            spins_out[2 * idx + 1] = NARROW ? 0u : s1;
         }
         sample++;
         next_measure += n_sweep;
@@ -186,60 +213,72 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
     if (n_sweep == 0 && n_therm == 0)
         while (sample < mp.steps_per_chain) measure_now();
 
+    auto load_row = [&](cx<T> (&w)[JPL], uint32_t dx) {
+#pragma unroll
+        for (int u = 0; u < JPL; u++) w[u] = Wl[(dx & 0x7FFFu) + 16 * u];
+    };
+    // One proposal with draw d = (x, U) and the W row of its site in wr.
+    auto propose = [&](uint2 d, const cx<T> (&wr)[JPL]) {
+        const int p = (int)((d.x >> 16) & 127u);
+        const T f = NARROW ? flip_factor_t<T>(w32 << (d.x >> 24)) : T(-2) * spin_of<T>(s0, s1, p);
+        cx<T> thn[JPL];
+        thn[0] = th[0] + f * wr[0];
+        T lnew = logcosh_re(thn[0]);
+#pragma unroll
+        for (int u = 1; u < JPL; u++) {
+            thn[u] = th[u] + f * wr[u];
+            lnew += logcosh_re(thn[u]);
+        }
+        T delta = row_sum16(lnew - lane_lc);
+        if (md.a) delta += f * md.a[p].re;
+        // u < 1 always, so (ratio > 1 || u <= ratio) is u <= ratio
+        const bool accept = unpack_u(d.y, T(0)) <= ratio_of(delta);
+#pragma unroll
+        for (int u = 0; u < JPL; u++) th[u] = accept ? thn[u] : th[u];
+        lane_lc = accept ? lnew : lane_lc;
+        if (NARROW) {
+            w32 ^= (uint32_t)accept << p;
+        } else {
+            const uint64_t bit = (uint64_t)accept;
+            s0 ^= p < 64 ? bit << (p & 63) : 0u;
+            s1 ^= p >= 64 ? bit << (p & 63) : 0u;
+        }
+        nacc += accept ? 1u : 0u;
+    };
+
     for (uint64_t kb = 0; kb < total; kb += MC_DRAW_BLOCK) {
         __syncthreads();
         {
+            // x = position * ld | position << 16 | (31 - position) << 24
             const u32x4 r = philox((uint32_t)chain, (uint32_t)((kb >> 1) + lane), mp.call, 0u, mp.seed_lo, mp.seed_hi);
-            draws[2 * lane] = make_uint2(r.x % (uint32_t)N, r.y);
-            draws[2 * lane + 1] = make_uint2(r.z % (uint32_t)N, r.w);
+            const uint32_t p0 = r.x % (uint32_t)N, p1 = r.z % (uint32_t)N;
+            draws[2 * lane] = make_uint2(p0 * (uint32_t)ld | p0 << 16 | ((31u - p0) & 31u) << 24, pack_u(r.y, T(0)));
+            draws[2 * lane + 1] = make_uint2(p1 * (uint32_t)ld | p1 << 16 | ((31u - p1) & 31u) << 24, pack_u(r.w, T(0)));
         }
         __syncthreads();
         const int nk = (int)(total - kb < (uint64_t)MC_DRAW_BLOCK ? total - kb : (uint64_t)MC_DRAW_BLOCK);
         int i = 0;
         while (i < nk) {
-            // proposals up to the next measurement point or the end of the block
+            // proposals up to the next measurement point or the end of the block;
+            // the W row of the next proposal is loaded while the current one is
+            // evaluated, alternating between two register sets
             const int stop = next_measure > kb && next_measure - kb < (uint64_t)nk ? (int)(next_measure - kb) : nk;
             uint2 d = draws[i];
-            cx<T> wr[JPL];
-#pragma unroll
-            for (int u = 0; u < JPL; u++) wr[u] = Wsrc[(int)d.x * ld + lane + 16 * u];
-            for (; i < stop; i++) {
-                const uint2 dn = draws[i + 1 < nk ? i + 1 : i];
-                cx<T> wn[JPL];
-#pragma unroll
-                for (int u = 0; u < JPL; u++) wn[u] = Wsrc[(int)dn.x * ld + lane + 16 * u];
-                const int p = (int)d.x;
-                const T f = T(-2) * spin_of<T>(s0, s1, p);
-                cx<T> thn[JPL];
-                T lrn[JPL];
-                T part = T(0);
-                // padded units: theta = theta' = 0, difference 0
-#pragma unroll
-                for (int u = 0; u < JPL; u++) {
-                    thn[u] = th[u] + f * wr[u];
-                    lrn[u] = logcosh_re(thn[u]);
-                    part += lrn[u] - lcr[u];
-                }
-                T delta = row_sum16(part);
-                cx<T> ap = mk(T(0), T(0));
-                if (md.a) { ap = md.a[p]; delta += f * ap.re; }
-                const T ratio = exp_fast(T(2) * delta);
-                const T uu = (T)(((double)d.y + 0.5) * (1.0 / 4294967296.0));
-                const bool accept = (ratio > T(1)) || (uu <= ratio);
-#pragma unroll
-                for (int u = 0; u < JPL; u++) {
-                    th[u] = accept ? thn[u] : th[u];
-                    lcr[u] = accept ? lrn[u] : lcr[u];
-                }
-                const uint64_t bit = (uint64_t)accept;
-                s0 ^= p < 64 ? bit << (p & 63) : 0u;
-                s1 ^= p >= 64 ? bit << (p & 63) : 0u;
-                vis = accept ? vis + f * ap : vis;
-                nacc += accept ? 1u : 0u;
+            cx<T> wa[JPL], wb[JPL];
+            load_row(wa, d.x);
+            while (i < stop) {
+                uint2 dn = draws[i + 1 < nk ? i + 1 : i];
+                load_row(wb, dn.x);
+                propose(d, wa);
                 d = dn;
-#pragma unroll
-                for (int u = 0; u < JPL; u++) wr[u] = wn[u];
+                if (++i >= stop) break;
+                dn = draws[i + 1 < nk ? i + 1 : i];
+                load_row(wa, dn.x);
+                propose(d, wb);
+                d = dn;
+                ++i;
             }
+            if (NARROW) s0 = w32;
             if (kb + (uint64_t)i == next_measure) measure_now();
             if (n_sweep == 0 && kb + (uint64_t)i == n_therm)
                 while (sample < mp.steps_per_chain) measure_now();
@@ -663,15 +702,21 @@ hipError_t launch_mc(const RbmDev<T>& md, const McParams& mp, uint64_t* spins, u
     const int jpl = pick_jpl(md.M);
     const dim3 block(256), grid((mp.chains + 15) / 16);
     const size_t shm = (wlds ? lds_bytes : 0) + MC_DRAW_LDS;
-#define RBM_MC_CASE(J)                                                                                   \
-    case J:                                                                                              \
-        if (wlds) {                                                                                      \
-            if (shm > 65536)                                                                             \
-                (void)hipFuncSetAttribute((const void*)&k_mc_rbm<T, J, true>,                            \
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);          \
-            hipLaunchKernelGGL((k_mc_rbm<T, J, true>), grid, block, shm, stream, md, mp, spins, counts); \
-        }                                                                                                \
-        else hipLaunchKernelGGL((k_mc_rbm<T, J, false>), grid, block, shm, stream, md, mp, spins, counts);     \
+    const bool narrow = md.N <= 32 && wlds;
+#define RBM_MC_CASE(J)                                                                                          \
+    case J:                                                                                                     \
+        if (narrow) {                                                                                           \
+            if (shm > 65536)                                                                                    \
+                (void)hipFuncSetAttribute((const void*)&k_mc_rbm<T, J, true, true>,                             \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                 \
+            hipLaunchKernelGGL((k_mc_rbm<T, J, true, true>), grid, block, shm, stream, md, mp, spins, counts);  \
+        } else if (wlds) {                                                                                      \
+            if (shm > 65536)                                                                                    \
+                (void)hipFuncSetAttribute((const void*)&k_mc_rbm<T, J, true, false>,                            \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                 \
+            hipLaunchKernelGGL((k_mc_rbm<T, J, true, false>), grid, block, shm, stream, md, mp, spins, counts); \
+        } else                                                                                                  \
+            hipLaunchKernelGGL((k_mc_rbm<T, J, false, false>), grid, block, shm, stream, md, mp, spins, counts); \
         break;
     switch (jpl) {
         RBM_JPL_LIST(RBM_MC_CASE)
