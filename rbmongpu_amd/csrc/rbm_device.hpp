@@ -234,20 +234,33 @@ struct OpDev {
     const uint64_t* ent_zy;        // 2 * entries
 };
 
+// Operator tables are read-only for the lifetime of a launch: reading them
+// through the constant address space lets wave-uniform indices become scalar
+// loads.
+#define RBM_CONST __attribute__((address_space(4)))
+template <typename T> RBM_DI T ldc(const T* p, int i) { return ((const RBM_CONST T*)p)[i]; }
+RBM_DI cx<double> ldc(const cx<double>* p, int i) {
+    const RBM_CONST double* q = (const RBM_CONST double*)p;
+    return mk(q[2 * i], q[2 * i + 1]);
+}
+
 RBM_DI cx<double> diag_energy(const OpDev& op, uint64_t s0, uint64_t s1) {
     cx<double> e = mk(0.0, 0.0);
     for (int d = 0; d < op.n_diag; d++) {
-        const double sg = zy_sign(op.diag_zy[2 * d], op.diag_zy[2 * d + 1], s0, s1);
-        e = e + sg * op.diag_coef[d];
+        const double sg = zy_sign(ldc(op.diag_zy, 2 * d), ldc(op.diag_zy, 2 * d + 1), s0, s1);
+        e = e + sg * ldc(op.diag_coef, d);
     }
     return e;
 }
 
 RBM_DI cx<double> term_coef(const OpDev& op, int t, uint64_t s0, uint64_t s1) {
     cx<double> c = mk(0.0, 0.0);
-    for (int e = op.ent_begin[t]; e < op.ent_begin[t + 1]; e++) {
-        const double sg = zy_sign(op.ent_zy[2 * e], op.ent_zy[2 * e + 1], s0, s1);
-        c = c + sg * op.ent_coef[e];
+    const int ee = ldc(op.ent_begin, t + 1);
+    for (int e = ldc(op.ent_begin, t); e < ee; e++) {
+        const uint64_t z0 = ldc(op.ent_zy, 2 * e), z1 = ldc(op.ent_zy, 2 * e + 1);
+        const cx<double> ce = ldc(op.ent_coef, e);
+        // entries without Z or Y factors (uniform test) need no parity
+        c = (z0 | z1) ? c + zy_sign(z0, z1, s0, s1) * ce : c + ce;
     }
     return c;
 }

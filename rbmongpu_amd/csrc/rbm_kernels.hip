@@ -15,6 +15,20 @@
 
 namespace rbmhip {
 
+// -2 s_p from a 32-bit word shifted so that bit p is its sign bit; the FP32
+// value is assembled from bits (+2 or -2).
+RBM_DI float flip_factor(uint32_t top) { return __uint_as_float((top & 0x80000000u) | 0x40000000u); }
+template <typename T> RBM_DI T flip_factor_t(uint32_t top) { return (top & 0x80000000u) ? T(-2) : T(2); }
+template <> RBM_DI float flip_factor_t<float>(uint32_t top) { return flip_factor(top); }
+
+// -2 s_p for a wave-uniform site p (an operator table entry): the word is
+// chosen by uniform tests, with no 64-bit shifts.
+template <typename T> RBM_DI T spin_factor(uint64_t s0, uint64_t s1, int p) {
+    const uint64_t w64 = p < 64 ? s0 : s1;
+    const uint32_t w = (p & 32) ? (uint32_t)(w64 >> 32) : (uint32_t)w64;
+    return flip_factor_t<T>(w << (31 - (p & 31)));
+}
+
 // ---------------------------------------------------------------------------
 // local energy (include/operator/Operator.hpp:90-115, restated on canonical
 // flip-mask terms): E_loc(s) = diag(s) + sum_t c_t(s) exp(log psi(s^m_t) - log psi(s)).
@@ -30,17 +44,17 @@ RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict_
     const cx<T>* __restrict__ Wl = Wsrc + lane;
     cx<double> E = diag_energy(op, s0, s1);
     for (int t = 0; t < op.n_terms; t++) {
-        const int fb = op.flip_begin[t], fe = op.flip_begin[t + 1];
+        const int fb = ldc(op.flip_begin, t), fe = ldc(op.flip_begin, t + 1);
         cx<T> v[JPL];
         {
-            const int pos = op.flips[fb];
-            const T f = T(-2) * spin_of<T>(s0, s1, pos);
+            const int pos = ldc(op.flips, fb);
+            const T f = spin_factor<T>(s0, s1, pos);
 #pragma unroll
             for (int u = 0; u < JPL; u++) v[u] = th[u] + f * Wl[pos * ld + 16 * u];
         }
         for (int q = fb + 1; q < fe; q++) {
-            const int pos = op.flips[q];
-            const T f = T(-2) * spin_of<T>(s0, s1, pos);
+            const int pos = ldc(op.flips, q);
+            const T f = spin_factor<T>(s0, s1, pos);
 #pragma unroll
             for (int u = 0; u < JPL; u++) v[u] = v[u] + f * Wl[pos * ld + 16 * u];
         }
@@ -145,12 +159,6 @@ RBM_DI uint32_t pack_u(uint32_t y, float) { return __float_as_uint((float)(((dou
 RBM_DI uint32_t pack_u(uint32_t y, double) { return y; }
 RBM_DI float unpack_u(uint32_t v, float) { return __uint_as_float(v); }
 RBM_DI double unpack_u(uint32_t v, double) { return ((double)v + 0.5) * (1.0 / 4294967296.0); }
-
-// -2 s_p from the word of a configuration with N <= 32, shifted so that bit
-// p is its sign bit: the FP32 value is assembled from bits (+2 or -2).
-RBM_DI float flip_factor(uint32_t top) { return __uint_as_float((top & 0x80000000u) | 0x40000000u); }
-template <typename T> RBM_DI T flip_factor_t(uint32_t top) { return (top & 0x80000000u) ? T(-2) : T(2); }
-template <> RBM_DI float flip_factor_t<float>(uint32_t top) { return flip_factor(top); }
 
 // NARROW: N <= 32, the configuration is one 32-bit word.
 template <typename T, int JPL, bool WLDS, bool NARROW>

@@ -78,6 +78,25 @@ def test_log_psi_shapes(precision, N, M):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("N,M", [(31, 20), (33, 17), (64, 40), (65, 33), (100, 48), (128, 16)])
+def test_local_energy_wide(precision, N, M):
+    """Flip sites and Z/Y factors in every 32-bit quarter of the two spin words."""
+    print(f"[local energy wide] precision={precision} N={N} M={M}", flush=True)
+    b, W, a = random_rbm(N, M, seed=N + 7 * M, scale=0.05, visible=True)
+    coeffs, types, idx = random_operator(N, 40, 5, seed=N)
+    tfim = HAM.tfim_1d(N, 1.0, 0.7)
+    tab = (np.concatenate([coeffs, tfim[0]]),
+           np.concatenate([types, np.pad(tfim[1], ((0, 0), (0, types.shape[1] - tfim[1].shape[1])))]),
+           np.concatenate([idx, np.pad(tfim[2], ((0, 0), (0, idx.shape[1] - tfim[2].shape[1])), constant_values=-1)]))
+    rng = np.random.default_rng(N)
+    configs = [int.from_bytes(rng.bytes(16), "little") & ((1 << N) - 1) for _ in range(200)]
+    words = O.spins_words(configs)
+    ref = O.local_energy(O.RBM(b, W, a), O.Operator(*tab), words)
+    got = R.local_energy_batch(gpu_psi(b, W, a, precision=precision), gpu_op(tab), words)
+    assert rel_err(got, ref) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("kind", ["tfim", "heis", "random", "repeated", "continuation", "visible"])
 def test_local_energy_operators(precision, kind):
     print(f"[local energy] precision={precision} kind={kind}", flush=True)
