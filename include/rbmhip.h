@@ -40,6 +40,7 @@ extern "C" {
 typedef struct rbmhip_psi rbmhip_psi;
 typedef struct rbmhip_op rbmhip_op;
 typedef struct rbmhip_ens rbmhip_ens;
+typedef struct rbmhip_made rbmhip_made;
 
 /* ---- library / device ---------------------------------------------------- */
 const char* rbmhip_last_error(void);
@@ -152,6 +153,50 @@ int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out);
  * of psi_prime) is written when want_grad != 0. */
 int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
                        int want_grad, double nu, double* distance, double* grad);
+
+/* ---- learned proposal: masked autoregressive network (MADE) -------------- */
+/* No counterpart in the reference.  A MADE (Germain et al. 2015) over x_i = 1
+ * for spin up: a = W1 x + b1, h = relu(a), l = W2 h + b2,
+ * q(x) = prod_j sigmoid(l_j)^x_j (1 - sigmoid(l_j))^(1 - x_j).  Hidden unit k has
+ * degree m_k = 1 + k mod (N - 1); W1[k][i] is used iff i < m_k, W2[j][k] iff
+ * m_k <= j, so l_j depends on x_0 .. x_{j-1}.  Parameters (FP32, flat):
+ * [W1 (H x N), b1 (H), W2 (N x H), b2 (N)]; masked entries are kept at zero.
+ * params == NULL draws the seeded initialisation: weight k (flat index) is
+ * (2u - 1) / sqrt(fan_in), u = (x + 0.5) 2^-32 with x the first word of
+ * Philox4x32-10(key = seed, ctr = (k, 0, 0, 3)); biases are zero.  N <= 128,
+ * H <= 512, and 4 N round_up(H, 64) bytes (the bfloat16 weights) must fit in LDS. */
+int rbmhip_made_create(int N, int H, const float* params, uint64_t seed, rbmhip_made** out);
+int rbmhip_made_num_params(const rbmhip_made* made, int64_t* out);
+int rbmhip_made_get_params(rbmhip_made* made, float* out);
+int rbmhip_made_set_params(rbmhip_made* made, const float* params);
+/* loss = -(1/n) sum_s log q(x_s) over the n configurations and its gradient
+ * (num_params floats), FP32 MFMA forward and backward passes. */
+int rbmhip_made_loss_grad(rbmhip_made* made, const uint64_t* spins, int64_t n, double* loss, float* grad);
+/* `steps` Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of
+ * `batch` configurations drawn with replacement from the n given ones (index
+ * floor(u n), u from Philox(seed, ctr = (step, row, 0, 4))); batch == n uses all
+ * n in order.  losses (may be NULL) receives the loss of each step's batch. */
+int rbmhip_made_train(rbmhip_made* made, const uint64_t* spins, int64_t n, int steps, int batch, double lr,
+                      uint64_t seed, double* losses);
+int rbmhip_made_reset_optimizer(rbmhip_made* made);
+/* log q of n configurations with the sampler's arithmetic: bfloat16 weights,
+ * FP32 hidden pre-activations, bfloat16 relu(a) in the logit dot products. */
+int rbmhip_made_log_q(rbmhip_made* made, const uint64_t* spins, int64_t n, double* out);
+/* n autoregressive samples and their log q (log_q may be NULL) */
+int rbmhip_made_sample(rbmhip_made* made, int64_t n, uint64_t seed, uint64_t* spins, double* log_q);
+int rbmhip_made_destroy(rbmhip_made* made);
+/* Independent Metropolis-Hastings ensemble with the network as proposal; it is
+ * used wherever a MonteCarloLoop is (expectation, gradient, distance).  Each
+ * chain starts from a proposal, takes num_thermalization_steps unmeasured
+ * proposals, then records one sample per proposal; a proposal s' is accepted
+ * iff u <= |psi(s')|^2 q(s) / (|psi(s)|^2 q(s')).  The network must outlive
+ * the ensemble. */
+int rbmhip_learned_create(rbmhip_made* made, uint32_t num_samples, uint32_t num_thermalization_steps,
+                          uint32_t num_markov_chains, uint64_t seed, rbmhip_ens** out);
+/* Proposals of the last call: *n = chains * (num_thermalization_steps + steps + 1),
+ * proposal t of chain c at index t * chains + c, with its log q and log psi
+ * (complex).  Call with NULL arrays to read n first. */
+int rbmhip_learned_last_proposals(const rbmhip_ens* ens, int64_t* n, uint64_t* spins, double* log_q, double* log_psi);
 
 /* ---- kernel timing (HIP events on the library stream) -------------------- */
 /* wait for all work on the library stream of the current device */

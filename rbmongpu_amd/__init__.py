@@ -7,6 +7,10 @@ ExactSummation, ExpectationValue, HilbertSpaceDistance) and functions
 (psi_O_k_vector, get_S_matrix, activation_function, setDevice, ...), backed by
 hand-written HIP kernels in librbmhip.so.  The compiled module is required:
 there is no CPU fallback.
+
+Beyond the reference: MADE, a masked autoregressive proposal network trained
+on the GPU, and LearnedMonteCarloLoop, an independent Metropolis-Hastings
+ensemble that proposes from it and is accepted with the exact |psi|^2 ratio.
 """
 from __future__ import annotations
 
@@ -19,7 +23,7 @@ try:
         Psi, PsiDeep, Operator, Spins, MonteCarloLoop, ExactSummation, ExpectationValue,
         HilbertSpaceDistance, psi_O_k_vector, get_S_matrix, activation_function, activation_derivative,
         setDevice, start_profiling, stop_profiling, device_count, timing_enable, timing_reset, timing_query,
-        log_psi_batch, local_energy_batch, synchronize,
+        log_psi_batch, local_energy_batch, synchronize, MADE, LearnedMonteCarloLoop,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(

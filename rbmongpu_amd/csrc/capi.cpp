@@ -22,6 +22,7 @@
 
 #include "rbm_kernels.hpp"
 #include "deep_kernels.hpp"
+#include "made_kernels.hpp"
 
 using cd = std::complex<double>;
 
@@ -446,10 +447,157 @@ static OpDev empty_op() {
 }
 
 // ---------------------------------------------------------------------------
+// masked autoregressive proposal (MADE)
+// ---------------------------------------------------------------------------
+// Device parameters are one padded FP32 block [W1 (Hp x Np), b1 (Hp), W2 (Np x Hp),
+// b2 (Np)] so that Adam runs over a single array; masked and padded entries
+// are zero and receive zero gradients, so they stay zero.
+struct rbmhip_made {
+    MadeShape s{};
+    int device = -1;
+    int64_t off_b1 = 0, off_W2 = 0, off_b2 = 0, padded = 0;
+    DevBuf params, grad, m, v;
+    int t = 0;                  // Adam steps taken
+    DevBuf W1t_bf, W2_bf;       // bfloat16 inference copies
+    bool inf_valid = false;
+    ~rbmhip_made() {
+        params.release(); grad.release(); m.release(); v.release();
+        W1t_bf.release(); W2_bf.release();
+    }
+    int64_t num_params() const { return 2 * (int64_t)s.N * s.H + s.H + s.N; }
+    float* P() { return reinterpret_cast<float*>(params.p); }
+    float* G() { return reinterpret_cast<float*>(grad.p); }
+};
+
+static void made_check_device(Ctx& c, const rbmhip_made* md) {
+    RBM_REQUIRE(c.device == md->device, "the proposal network lives on another device");
+}
+
+// host flat layout [W1 (H x N), b1, W2 (N x H), b2] <-> padded device layout, masks applied
+static std::vector<float> made_pad_params(const rbmhip_made* md, const float* flat) {
+    const int N = md->s.N, H = md->s.H, Np = md->s.Np, Hp = md->s.Hp;
+    std::vector<float> out((size_t)md->padded, 0.0f);
+    const float* W1 = flat;
+    const float* b1 = W1 + (size_t)H * N;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + (size_t)N * H;
+    for (int k = 0; k < H; k++) {
+        const int deg = made_degree(k, N);
+        for (int i = 0; i < N; i++) out[(size_t)k * Np + i] = i < deg ? W1[(size_t)k * N + i] : 0.0f;
+        out[md->off_b1 + k] = b1[k];
+    }
+    for (int j = 0; j < N; j++) {
+        for (int k = 0; k < H; k++) out[md->off_W2 + (size_t)j * Hp + k] = made_degree(k, N) <= j ? W2[(size_t)j * H + k] : 0.0f;
+        out[md->off_b2 + j] = b2[j];
+    }
+    return out;
+}
+
+static std::vector<float> made_unpad_params(const rbmhip_made* md, const std::vector<float>& dev) {
+    const int N = md->s.N, H = md->s.H, Np = md->s.Np, Hp = md->s.Hp;
+    std::vector<float> out((size_t)md->num_params());
+    float* W1 = out.data();
+    float* b1 = W1 + (size_t)H * N;
+    float* W2 = b1 + H;
+    float* b2 = W2 + (size_t)N * H;
+    for (int k = 0; k < H; k++) {
+        for (int i = 0; i < N; i++) W1[(size_t)k * N + i] = dev[(size_t)k * Np + i];
+        b1[k] = dev[md->off_b1 + k];
+    }
+    for (int j = 0; j < N; j++) {
+        for (int k = 0; k < H; k++) W2[(size_t)j * H + k] = dev[md->off_W2 + (size_t)j * Hp + k];
+        b2[j] = dev[md->off_b2 + j];
+    }
+    return out;
+}
+
+// Seeded initialisation: parameter k (flat host index) draws Philox(seed; ctr =
+// (k, 0, 0, 3)) word x, u = (x + 0.5) 2^-32; weights (2u - 1) / sqrt(fan_in)
+// (fan_in N for W1, H for W2), biases zero; masks applied afterwards.
+static std::vector<float> made_init_params(int N, int H, uint64_t seed) {
+    const int64_t P = 2 * (int64_t)N * H + H + N;
+    std::vector<float> out((size_t)P, 0.0f);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const int64_t w1 = (int64_t)H * N, w2_begin = w1 + H, w2_end = w2_begin + (int64_t)N * H;
+    for (int64_t k = 0; k < P; k++) {
+        const bool is_w1 = k < w1, is_w2 = k >= w2_begin && k < w2_end;
+        if (!is_w1 && !is_w2) continue;
+        const u32x4 r = philox((uint32_t)k, 0u, 0u, 3u, k0, k1);
+        const double u = ((double)r.x + 0.5) * (1.0 / 4294967296.0);
+        const double fan = is_w1 ? (double)N : (double)H;
+        out[(size_t)k] = (float)((2.0 * u - 1.0) / std::sqrt(fan));
+    }
+    return out;
+}
+
+static MadeInf made_inf(Ctx& c, rbmhip_made* md) {
+    made_check_device(c, md);
+    const size_t bytes = (size_t)md->s.N * md->s.Hp * sizeof(uint16_t);
+    md->W1t_bf.ensure(bytes);
+    md->W2_bf.ensure(bytes);
+    if (!md->inf_valid) {
+        const float* P = md->P();
+        c.launch("made_inf_weights", [&] {
+            return launch_made_inference_weights(md->s, P, P + md->off_W2, reinterpret_cast<uint16_t*>(md->W1t_bf.p),
+                                                 reinterpret_cast<uint16_t*>(md->W2_bf.p), c.stream);
+        });
+        md->inf_valid = true;
+    }
+    MadeInf mi;
+    mi.N = md->s.N; mi.H = md->s.H; mi.Hp = md->s.Hp;
+    mi.W1t = reinterpret_cast<const uint16_t*>(md->W1t_bf.p);
+    mi.W2 = reinterpret_cast<const uint16_t*>(md->W2_bf.p);
+    mi.b1 = md->P() + md->off_b1;
+    mi.b2 = md->P() + md->off_b2;
+    return mi;
+}
+
+// Forward and backward pass of loss = -(1/B) sum_s log q(x_s) over B rows of X
+// (Bp = padded rows).  Leaves the gradient in md->grad and the per-tile log-q
+// partials in `partials`.
+static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const float* X, double* partials) {
+    const int N = md->s.N, H = md->s.H, Np = md->s.Np, Hp = md->s.Hp;
+    float* Apre = reinterpret_cast<float*>(c.buf("made_apre", (size_t)Bp * Hp * sizeof(float)));
+    float* Hh = reinterpret_cast<float*>(c.buf("made_h", (size_t)Bp * Hp * sizeof(float)));
+    float* dL = reinterpret_cast<float*>(c.buf("made_dl", (size_t)Bp * Np * sizeof(float)));
+    float* dH = reinterpret_cast<float*>(c.buf("made_dh", (size_t)Bp * Hp * sizeof(float)));
+    const float* P = md->P();
+    float* G = md->G();
+    const float* W1 = P;
+    const float* W2 = P + md->off_W2;
+    GemmEpiArgs e{};
+    // a = X W1^T + b1, h = relu(a)
+    e.bias = P + md->off_b1; e.C2 = Hh;
+    c.launch("made_gemm", [&] { return launch_gemm(EPI_BIAS_RELU, Bp, Hp, Np, X, Np, 1, W1, 1, Np, Apre, Hp, e, c.stream); });
+    // l = h W2^T + b2 -> dl = (sigmoid(l) - x) / B
+    e = GemmEpiArgs{};
+    e.bias = P + md->off_b2; e.aux = X; e.ld_aux = Np; e.n_live = N; e.m_live = B; e.inv_batch = 1.0f / (float)B;
+    e.partials = partials;
+    c.launch("made_gemm", [&] { return launch_gemm(EPI_LOGITS, Bp, Np, Hp, Hh, Hp, 1, W2, 1, Hp, dL, Np, e, c.stream); });
+    // dW2 = dl^T h (masked)
+    e = GemmEpiArgs{};
+    e.m_live = N; e.n_live = H;
+    c.launch("made_gemm", [&] { return launch_gemm(EPI_MASK_W2, Np, Hp, Bp, dL, 1, Np, Hh, Hp, 1, G + md->off_W2, Hp, e, c.stream); });
+    // dh = (dl W2) * [a > 0]
+    e = GemmEpiArgs{};
+    e.aux = Apre; e.ld_aux = Hp;
+    c.launch("made_gemm", [&] { return launch_gemm(EPI_RELU_GRAD, Bp, Hp, Np, dL, Np, 1, W2, Hp, 1, dH, Hp, e, c.stream); });
+    // dW1 = dh^T X (masked)
+    e = GemmEpiArgs{};
+    e.m_live = H; e.n_live = N;
+    c.launch("made_gemm", [&] { return launch_gemm(EPI_MASK_W1, Hp, Np, Bp, dH, 1, Hp, X, Np, 1, G, Np, e, c.stream); });
+    c.launch("made_colsum", [&] { return launch_colsum(Bp, Hp, H, dH, G + md->off_b1, c.stream); });
+    c.launch("made_colsum", [&] { return launch_colsum(Bp, Np, N, dL, G + md->off_b2, c.stream); });
+}
+
+static int made_loss_tiles(const rbmhip_made* md, int Bp) { return (md->s.Np / 64) * (Bp / 64); }
+
+// ---------------------------------------------------------------------------
 // ensembles
 // ---------------------------------------------------------------------------
 struct rbmhip_ens {
-    int kind = 0;  // 0 ExactSummation, 1 MonteCarloLoop
+    int kind = 0;  // 0 ExactSummation, 1 MonteCarloLoop, 2 independent MH with a MADE proposal
+    rbmhip_made* made = nullptr;  // kind 2 (not owned)
     int N = 0;
     bool has_sector = false;
     int sector = 0;
@@ -464,9 +612,13 @@ struct rbmhip_ens {
     int rec_device = -1;
     int64_t rec_n = 0;
     DevBuf spins, lp, el;
+    // proposals of the last learned-proposal call
+    int64_t prop_n = 0;
+    DevBuf prop_spins, prop_lq, prop_lp;
     ~rbmhip_ens() {
         for (auto& kv : configs_dev) kv.second.release();
         spins.release(); lp.release(); el.release();
+        prop_spins.release(); prop_lq.release(); prop_lp.release();
     }
     int64_t num_steps() const { return kind == 0 ? num_configs : (int64_t)num_samples; }
 };
@@ -505,17 +657,17 @@ static void check_op_fits(const rbmhip_psi* psi, const rbmhip_op* op) {
 // index_base + idx).
 template <typename T>
 static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const uint64_t* configs,
-                       uint64_t index_base, const Records& rec) {
+                       uint64_t index_base, const Records& rec, const char* label = "eval") {
     const OpDev od = op ? op_dev(c, op) : empty_op();
     const int max_blocks = c.num_cus * 8;
     if (psi->kind == 0) {
         const RbmDev<T> md = rbm_dev<T>(c, psi);
         const size_t lds = rbm_w_bytes<T>(psi);
         const bool wlds = lds <= c.max_lds;
-        c.launch("eval", [&] { return launch_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, wlds, lds, max_blocks, c.stream); });
+        c.launch(label, [&] { return launch_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, wlds, lds, max_blocks, c.stream); });
     } else {
         const DeepDev<T> md = deep_dev<T>(c, psi);
-        c.launch("deep_eval", [&] { return launch_deep_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, max_blocks, c.stream); });
+        c.launch(label[0] == 'e' ? "deep_eval" : label, [&] { return launch_deep_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, max_blocks, c.stream); });
     }
 }
 
@@ -523,14 +675,14 @@ static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const 
 // ExactSummation the samples are the basis (or sector) states.
 template <typename T>
 static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, bool want_spins, bool want_tanh) {
-    RBM_REQUIRE(e->N == psi->N || e->kind == 1, "ensemble and wavefunction have different N");
+    RBM_REQUIRE(e->N == psi->N || e->kind != 0, "ensemble and wavefunction have different N");
     check_op_fits(psi, op);
     Samples s;
     s.n = e->num_steps();
     RBM_REQUIRE(s.n > 0, "the ensemble has no samples");
     const size_t tsz = psi->kind == 0 ? (size_t)psi->M : (size_t)psi->num_params() - 2 * (size_t)psi->N;
     if (want_tanh) s.tanh = c.buf("tanh", (size_t)s.n * tsz * sizeof(cx<T>));
-    if (e->kind == 1) {
+    if (e->kind != 0) {
         RBM_REQUIRE(e->num_chains > 0 && e->num_samples % e->num_chains == 0,
                     "num_samples must be a positive multiple of num_markov_chains (MonteCarloLoop.cu:78)");
         e->spins.ensure((size_t)s.n * 2 * sizeof(uint64_t));
@@ -551,7 +703,26 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
         mp.call = e->call++;
         uint32_t* counts = reinterpret_cast<uint32_t*>(c.buf("counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
         const OpDev od = op ? op_dev(c, op) : empty_op();
-        if (psi->kind == 0) {
+        if (e->kind == 2) {
+            // independent proposals do not depend on the chain: draw all of them,
+            // evaluate log psi in one batch, then run the accept scan per chain
+            RBM_REQUIRE(e->made && e->made->s.N == psi->N, "the proposal network and the wavefunction have different N");
+            const int64_t P = (int64_t)mp.chains * (mp.therm + mp.steps_per_chain + 1);
+            uint64_t* prop = reinterpret_cast<uint64_t*>(e->prop_spins.ensure((size_t)P * 2 * sizeof(uint64_t)));
+            double* plq = reinterpret_cast<double*>(e->prop_lq.ensure((size_t)P * sizeof(double)));
+            cx<double>* plp = reinterpret_cast<cx<double>*>(e->prop_lp.ensure((size_t)P * sizeof(cx<double>)));
+            e->prop_n = P;
+            const MadeInf mi = made_inf(c, e->made);
+            c.launch("made_sample", [&] { return launch_made_sample(mi, P, mp.seed_lo, mp.seed_hi, mp.call, prop, plq, c.stream); });
+            Records prec{nullptr, plp, nullptr, nullptr};
+            eval_model<T>(c, psi, nullptr, P, prop, 0, prec, "proposal_eval");
+            c.launch("imh", [&] {
+                return launch_imh(mp.chains, mp.therm, mp.steps_per_chain, prop, plp, plq, mp.seed_lo, mp.seed_hi, mp.call,
+                                  s.spins, counts, c.stream);
+            });
+            Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
+            eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
+        } else if (psi->kind == 0) {
             // sampling records configurations; measurement runs at full
             // occupancy in the evaluation kernel on the recorded samples
             const RbmDev<T> md = rbm_dev<T>(c, psi);
@@ -982,7 +1153,7 @@ int rbmhip_mc_acceptance(const rbmhip_ens* ens, uint64_t* acceptances, uint64_t*
 int rbmhip_mc_last_samples(const rbmhip_ens* ens, uint64_t* spins, double* log_psi, double* e_loc) {
     return guard([&] {
         check_ens(ens);
-        RBM_REQUIRE(ens->kind == 1 && ens->rec_n > 0, "no Monte Carlo samples recorded yet");
+        RBM_REQUIRE(ens->kind != 0 && ens->rec_n > 0, "no Monte Carlo samples recorded yet");
         Ctx& c = ctx();
         RBM_REQUIRE(c.device == ens->rec_device, "samples were recorded on another device");
         if (spins) c.d2h(spins, ens->spins.p, (size_t)ens->rec_n * 2 * sizeof(uint64_t));
@@ -1154,7 +1325,7 @@ int rbmhip_expectation_list(rbmhip_psi* psi, rbmhip_op* const* ops, int n_ops, r
             // one pass of the ensemble, then every operator on the same samples
             // (ExpectationValue.cu:448-518)
             Samples s = sample<T>(c, psi, nullptr, ens, true, false);
-            const uint64_t* configs = ens->kind == 1 ? s.spins : ens_configs(c, ens);
+            const uint64_t* configs = ens->kind != 0 ? s.spins : ens_configs(c, ens);
             cx<double>* el = reinterpret_cast<cx<double>*>(c.buf("el_list", (size_t)s.n * sizeof(cx<double>)));
             cx<double>* lp2 = reinterpret_cast<cx<double>*>(c.buf("lp_list", (size_t)s.n * sizeof(cx<double>)));
             const double inv = 1.0 / (double)ens->num_steps();
@@ -1257,7 +1428,7 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
             using T = decltype(tag);
             s = sample<T>(c, psi, op, ens, true, false);
         });
-        const uint64_t* configs = ens->kind == 1 ? s.spins : (ens->has_sector ? ens_configs(c, ens) : nullptr);
+        const uint64_t* configs = ens->kind != 0 ? s.spins : (ens->has_sector ? ens_configs(c, ens) : nullptr);
         // HilbertSpaceDistance.cu:40-44: log psi'(s) on the same samples (and tanh for O_k(psi'))
         cx<double>* lp2 = reinterpret_cast<cx<double>*>(c.buf("lp2", (size_t)s.n * sizeof(cx<double>)));
         void* tanh2 = nullptr;
@@ -1336,6 +1507,217 @@ int rbmhip_timing_query(const char* name, double* total_ms, int64_t* launches) {
         auto it = c.totals.find(name);
         if (total_ms) *total_ms = it == c.totals.end() ? 0.0 : it->second.first;
         if (launches) *launches = it == c.totals.end() ? 0 : it->second.second;
+    });
+}
+
+// ---- masked autoregressive proposal ---------------------------------------------
+int rbmhip_made_create(int N, int H, const float* params, uint64_t seed, rbmhip_made** out) {
+    return guard([&] {
+        RBM_REQUIRE(out, "null output");
+        RBM_REQUIRE(N >= 1 && N <= 128, "the proposal network needs 1 <= N <= 128");
+        RBM_REQUIRE(H >= 1 && made_pad(H) <= 512, "the proposal network needs 1 <= hidden units <= 512");
+        Ctx& c = ctx();
+        auto md = std::make_unique<rbmhip_made>();
+        md->s.N = N; md->s.H = H; md->s.Np = made_pad(N); md->s.Hp = made_pad(H);
+        RBM_REQUIRE((size_t)2 * N * md->s.Hp * sizeof(uint16_t) <= 160 * 1024,
+                    "the bfloat16 weights of this network do not fit in one workgroup's LDS (N * hidden too large)");
+        const int Np = md->s.Np, Hp = md->s.Hp;
+        md->off_b1 = (int64_t)Hp * Np;
+        md->off_W2 = md->off_b1 + Hp;
+        md->off_b2 = md->off_W2 + (int64_t)Np * Hp;
+        md->padded = md->off_b2 + Np;
+        md->device = c.device;
+        const std::vector<float> flat = params ? std::vector<float>(params, params + md->num_params())
+                                               : made_init_params(N, H, seed);
+        const std::vector<float> dev = made_pad_params(md.get(), flat.data());
+        const size_t bytes = (size_t)md->padded * sizeof(float);
+        md->params.ensure(bytes); md->grad.ensure(bytes); md->m.ensure(bytes); md->v.ensure(bytes);
+        c.h2d(md->params.p, dev.data(), bytes);
+        RBM_HIPCHK(hipMemsetAsync(md->grad.p, 0, bytes, c.stream));
+        RBM_HIPCHK(hipMemsetAsync(md->m.p, 0, bytes, c.stream));
+        RBM_HIPCHK(hipMemsetAsync(md->v.p, 0, bytes, c.stream));
+        c.sync();
+        *out = md.release();
+    });
+}
+
+int rbmhip_made_num_params(const rbmhip_made* made, int64_t* out) {
+    return guard([&] { RBM_REQUIRE(made && out, "bad arguments"); *out = made->num_params(); });
+}
+
+int rbmhip_made_get_params(rbmhip_made* made, float* out) {
+    return guard([&] {
+        RBM_REQUIRE(made && out, "bad arguments");
+        Ctx& c = ctx();
+        made_check_device(c, made);
+        std::vector<float> dev((size_t)made->padded);
+        c.d2h(dev.data(), made->params.p, dev.size() * sizeof(float));
+        c.sync();
+        const std::vector<float> flat = made_unpad_params(made, dev);
+        std::memcpy(out, flat.data(), flat.size() * sizeof(float));
+    });
+}
+
+int rbmhip_made_set_params(rbmhip_made* made, const float* params) {
+    return guard([&] {
+        RBM_REQUIRE(made && params, "bad arguments");
+        Ctx& c = ctx();
+        made_check_device(c, made);
+        const std::vector<float> dev = made_pad_params(made, params);
+        c.h2d(made->params.p, dev.data(), dev.size() * sizeof(float));
+        c.sync();
+        made->inf_valid = false;
+    });
+}
+
+int rbmhip_made_reset_optimizer(rbmhip_made* made) {
+    return guard([&] {
+        RBM_REQUIRE(made, "null handle");
+        Ctx& c = ctx();
+        made_check_device(c, made);
+        const size_t bytes = (size_t)made->padded * sizeof(float);
+        RBM_HIPCHK(hipMemsetAsync(made->m.p, 0, bytes, c.stream));
+        RBM_HIPCHK(hipMemsetAsync(made->v.p, 0, bytes, c.stream));
+        c.sync();
+        made->t = 0;
+    });
+}
+
+int rbmhip_made_loss_grad(rbmhip_made* made, const uint64_t* spins, int64_t n, double* loss, float* grad) {
+    return guard([&] {
+        RBM_REQUIRE(made && spins && n > 0, "bad arguments");
+        RBM_REQUIRE(n <= (1 << 24), "batch too large");
+        Ctx& c = ctx();
+        made_check_device(c, made);
+        const int B = (int)n, Bp = made_pad(B);
+        uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("made_in", (size_t)n * 2 * sizeof(uint64_t)));
+        c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));
+        float* X = reinterpret_cast<float*>(c.buf("made_x", (size_t)Bp * made->s.Np * sizeof(float)));
+        c.launch("made_gather", [&] { return launch_made_gather(Bp, B, made->s.N, made->s.Np, ds, nullptr, X, c.stream); });
+        const int tiles = made_loss_tiles(made, Bp);
+        double* part = reinterpret_cast<double*>(c.buf("made_part", (size_t)tiles * sizeof(double)));
+        made_forward_backward(c, made, B, Bp, X, part);
+        std::vector<double> hp((size_t)tiles);
+        std::vector<float> dev((size_t)made->padded);
+        c.d2h(hp.data(), part, hp.size() * sizeof(double));
+        c.d2h(dev.data(), made->grad.p, dev.size() * sizeof(float));
+        c.sync();
+        double sum = 0.0;
+        for (double v : hp) sum += v;
+        if (loss) *loss = -sum / (double)B;
+        if (grad) {
+            const std::vector<float> flat = made_unpad_params(made, dev);
+            std::memcpy(grad, flat.data(), flat.size() * sizeof(float));
+        }
+    });
+}
+
+int rbmhip_made_train(rbmhip_made* made, const uint64_t* spins, int64_t n, int steps, int batch, double lr,
+                      uint64_t seed, double* losses) {
+    return guard([&] {
+        RBM_REQUIRE(made && spins && n > 0 && steps >= 0 && batch > 0, "bad arguments");
+        RBM_REQUIRE(batch <= (1 << 20), "batch too large");
+        Ctx& c = ctx();
+        made_check_device(c, made);
+        if (steps == 0) return;
+        const int B = batch, Bp = made_pad(B);
+        const bool full = (int64_t)batch == n;   // the whole sample set, in order, every step
+        uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("made_train_in", (size_t)n * 2 * sizeof(uint64_t)));
+        c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));
+        float* X = reinterpret_cast<float*>(c.buf("made_x", (size_t)Bp * made->s.Np * sizeof(float)));
+        int64_t* idx = full ? nullptr : reinterpret_cast<int64_t*>(c.buf("made_idx", (size_t)B * sizeof(int64_t)));
+        const int tiles = made_loss_tiles(made, Bp);
+        double* part = reinterpret_cast<double*>(c.buf("made_train_part", (size_t)steps * tiles * sizeof(double)));
+        for (int st = 0; st < steps; st++) {
+            if (!full) {
+                c.launch("made_batch", [&] {
+                    return launch_made_batch_index(B, n, (uint32_t)(made->t + st), (uint32_t)seed, (uint32_t)(seed >> 32), idx, c.stream);
+                });
+            }
+            c.launch("made_gather", [&] { return launch_made_gather(Bp, B, made->s.N, made->s.Np, ds, idx, X, c.stream); });
+            made_forward_backward(c, made, B, Bp, X, part + (size_t)st * tiles);
+            const int t = made->t + st + 1;
+            c.launch("made_adam", [&] {
+                return launch_adam(made->padded, made->P(), made->G(), reinterpret_cast<float*>(made->m.p),
+                                   reinterpret_cast<float*>(made->v.p), t, (float)lr, 0.9f, 0.999f, 1e-8f, c.stream);
+            });
+        }
+        made->t += steps;
+        made->inf_valid = false;
+        std::vector<double> hp((size_t)steps * tiles);
+        c.d2h(hp.data(), part, hp.size() * sizeof(double));
+        c.sync();
+        if (losses)
+            for (int st = 0; st < steps; st++) {
+                double sum = 0.0;
+                for (int k = 0; k < tiles; k++) sum += hp[(size_t)st * tiles + k];
+                losses[st] = -sum / (double)B;
+            }
+    });
+}
+
+int rbmhip_made_log_q(rbmhip_made* made, const uint64_t* spins, int64_t n, double* out) {
+    return guard([&] {
+        RBM_REQUIRE(made && n >= 0 && (n == 0 || (spins && out)), "bad arguments");
+        if (n == 0) return;
+        Ctx& c = ctx();
+        const MadeInf mi = made_inf(c, made);
+        uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("in_spins", (size_t)n * 2 * sizeof(uint64_t)));
+        double* lq = reinterpret_cast<double*>(c.buf("made_lq", (size_t)n * sizeof(double)));
+        c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));
+        c.launch("made_log_q", [&] { return launch_made_log_q(mi, n, ds, lq, c.stream); });
+        c.d2h(out, lq, (size_t)n * sizeof(double));
+        c.sync();
+    });
+}
+
+int rbmhip_made_sample(rbmhip_made* made, int64_t n, uint64_t seed, uint64_t* spins, double* log_q) {
+    return guard([&] {
+        RBM_REQUIRE(made && n >= 0 && (n == 0 || spins), "bad arguments");
+        if (n == 0) return;
+        Ctx& c = ctx();
+        const MadeInf mi = made_inf(c, made);
+        uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("made_out_spins", (size_t)n * 2 * sizeof(uint64_t)));
+        double* lq = reinterpret_cast<double*>(c.buf("made_lq", (size_t)n * sizeof(double)));
+        c.launch("made_sample", [&] { return launch_made_sample(mi, n, (uint32_t)seed, (uint32_t)(seed >> 32), 0u, ds, lq, c.stream); });
+        c.d2h(spins, ds, (size_t)n * 2 * sizeof(uint64_t));
+        if (log_q) c.d2h(log_q, lq, (size_t)n * sizeof(double));
+        c.sync();
+    });
+}
+
+int rbmhip_learned_last_proposals(const rbmhip_ens* ens, int64_t* n, uint64_t* spins, double* log_q, double* log_psi) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 2, "not a learned-proposal ensemble");
+        if (n) *n = ens->prop_n;
+        if (ens->prop_n == 0 || (!spins && !log_q && !log_psi)) return;
+        Ctx& c = ctx();
+        RBM_REQUIRE(c.device == ens->rec_device, "proposals were drawn on another device");
+        if (spins) c.d2h(spins, ens->prop_spins.p, (size_t)ens->prop_n * 2 * sizeof(uint64_t));
+        if (log_q) c.d2h(log_q, ens->prop_lq.p, (size_t)ens->prop_n * sizeof(double));
+        if (log_psi) c.d2h(log_psi, ens->prop_lp.p, (size_t)ens->prop_n * sizeof(cx<double>));
+        c.sync();
+    });
+}
+
+int rbmhip_made_destroy(rbmhip_made* made) {
+    return guard([&] { delete made; });
+}
+
+int rbmhip_learned_create(rbmhip_made* made, uint32_t num_samples, uint32_t num_thermalization_steps,
+                          uint32_t num_markov_chains, uint64_t seed, rbmhip_ens** out) {
+    return guard([&] {
+        RBM_REQUIRE(made && out, "bad arguments");
+        RBM_REQUIRE(num_markov_chains > 0, "num_markov_chains must be > 0");
+        RBM_REQUIRE(num_samples % num_markov_chains == 0, "num_samples must be a multiple of num_markov_chains");
+        auto e = std::make_unique<rbmhip_ens>();
+        e->kind = 2;
+        e->made = made;
+        e->N = made->s.N;
+        e->num_samples = num_samples; e->num_sweeps = 1; e->num_therm = num_thermalization_steps;
+        e->num_chains = num_markov_chains; e->seed = seed;
+        *out = e.release();
     });
 }
 

@@ -1,0 +1,464 @@
+// gfx950 kernels of the masked autoregressive proposal (MADE):
+//   k_gemm<EPI>           FP32 MFMA GEMM (v_mfma_f32_16x16x4f32) with fused epilogues
+//                         for the training forward and backward passes
+//   k_made_gather         spin words -> 0/1 input rows
+//   k_made_batch_index    Philox-drawn minibatch indices
+//   k_colsum, k_adam      bias gradients and the Adam update
+//   k_made_inf_weights    bfloat16 inference copies of W1 (transposed) and W2
+//   k_made_sample<HPL,S>  autoregressive sampling / scoring, bfloat16 weights in LDS
+//   k_imh                 independent Metropolis-Hastings over precomputed proposals
+#include "made_kernels.hpp"
+
+namespace rbmhip {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// ---------------------------------------------------------------------------
+// GEMM.  A 256-thread workgroup computes a 64 x 64 tile of C as 2 x 2 waves of
+// 32 x 32, each 2 x 2 MFMA 16x16x4 tiles; K advances 16 at a time through LDS.
+// MFMA operand layout (16x16x4 f32): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator element v of lane l is D[4 (l / 16) + v][l % 16].
+// ---------------------------------------------------------------------------
+constexpr int GT = 64, GK = 16;
+
+RBM_DI float sigmoidf_acc(float l) { return 1.0f / (1.0f + expf(-l)); }
+RBM_DI float softplusf_acc(float l) { return fmaxf(l, 0.0f) + log1pf(expf(-fabsf(l))); }
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* __restrict__ A, long sam, long sak,
+                                              const float* __restrict__ B, long sbk, long sbn, float* __restrict__ C,
+                                              long ldc, GemmEpiArgs e) {
+    __shared__ float As[GK][GT + 4];
+    __shared__ float Bs[GK][GT + 4];
+    __shared__ double red[256];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int idx = tid + 256 * r;
+            int mm, kk;
+            if (sak == 1) { kk = idx & 15; mm = idx >> 4; } else { mm = idx & 63; kk = idx >> 6; }
+            As[kk][mm] = A[(long)(m0 + mm) * sam + (long)(k0 + kk) * sak];
+            int nn, kb;
+            if (sbn == 1) { nn = idx & 63; kb = idx >> 6; } else { kb = idx & 15; nn = idx >> 4; }
+            Bs[kb][nn] = B[(long)(k0 + kb) * sbk + (long)(n0 + nn) * sbn];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kq = 0; kq < GK; kq += 4) {
+            float af[2], bf[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) af[i] = As[kq + (lane >> 4)][wm + 16 * i + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; j++) bf[j] = Bs[kq + (lane >> 4)][wn + 16 * j + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    double part = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+                const int n = n0 + wn + 16 * j + (lane & 15);
+                const float x = acc[i][j][v];
+                float* out = C + (long)m * ldc + n;
+                if (EPI == EPI_STORE) {
+                    *out = x;
+                } else if (EPI == EPI_BIAS_RELU) {
+                    const float a = x + e.bias[n];
+                    *out = a;
+                    e.C2[(long)m * ldc + n] = fmaxf(a, 0.0f);
+                } else if (EPI == EPI_LOGITS) {
+                    const float l = x + e.bias[n];
+                    const float xv = e.aux[(long)m * e.ld_aux + n];
+                    const bool live = m < e.m_live && n < e.n_live;
+                    *out = live ? (sigmoidf_acc(l) - xv) * e.inv_batch : 0.0f;
+                    if (live) part += (double)(xv * l) - (double)softplusf_acc(l);
+                } else if (EPI == EPI_MASK_W2) {
+                    const bool ok = m < e.m_live && n < e.n_live && made_degree(n, e.m_live) <= m;
+                    *out = ok ? x : 0.0f;
+                } else if (EPI == EPI_MASK_W1) {
+                    const bool ok = m < e.m_live && n < e.n_live && n < made_degree(m, e.n_live);
+                    *out = ok ? x : 0.0f;
+                } else if (EPI == EPI_RELU_GRAD) {
+                    *out = e.aux[(long)m * e.ld_aux + n] > 0.0f ? x : 0.0f;
+                }
+            }
+    if (EPI == EPI_LOGITS) {
+        red[tid] = part;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int k = 0; k < 256; k++) s += red[k];
+            e.partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+        }
+    }
+}
+
+hipError_t launch_gemm(int epi, int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk,
+                       long sbn, float* C, long ldc, const GemmEpiArgs& e, hipStream_t stream) {
+    if (M % GT || N % GT || K % GK || M <= 0 || N <= 0 || K <= 0) return hipErrorInvalidValue;
+    const dim3 grid(N / GT, M / GT), block(256);
+#define RBM_GEMM_CASE(E) \
+    case E: hipLaunchKernelGGL((k_gemm<E>), grid, block, 0, stream, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, e); break;
+    switch (epi) {
+        RBM_GEMM_CASE(EPI_STORE)
+        RBM_GEMM_CASE(EPI_BIAS_RELU)
+        RBM_GEMM_CASE(EPI_LOGITS)
+        RBM_GEMM_CASE(EPI_MASK_W2)
+        RBM_GEMM_CASE(EPI_MASK_W1)
+        RBM_GEMM_CASE(EPI_RELU_GRAD)
+        default: return hipErrorInvalidValue;
+    }
+#undef RBM_GEMM_CASE
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Training helpers
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_made_gather(int64_t rows, int n_live, int N, int Np,
+                                                     const uint64_t* __restrict__ spins,
+                                                     const int64_t* __restrict__ idx, float* __restrict__ X) {
+    const int64_t total = rows * Np;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = t / Np;
+        const int i = (int)(t - s * Np);
+        float v = 0.0f;
+        if (s < n_live && i < N) {
+            const int64_t src = idx ? idx[s] : s;
+            v = (float)bit_of(spins[2 * src], spins[2 * src + 1], i);
+        }
+        X[t] = v;
+    }
+}
+
+hipError_t launch_made_gather(int64_t rows, int n_live, int N, int Np, const uint64_t* spins, const int64_t* idx,
+                              float* X, hipStream_t stream) {
+    const int64_t total = rows * Np;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_made_gather, dim3((unsigned)blocks), dim3(256), 0, stream, rows, n_live, N, Np, spins, idx, X);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_made_batch_index(int B, int64_t n, uint32_t step, uint32_t seed_lo,
+                                                          uint32_t seed_hi, int64_t* __restrict__ idx) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    const u32x4 r = philox(step, (uint32_t)s, 0u, 4u, seed_lo, seed_hi);
+    const double u = ((double)r.x + 0.5) * (1.0 / 4294967296.0);
+    int64_t k = (int64_t)(u * (double)n);
+    idx[s] = k < n ? k : n - 1;
+}
+
+hipError_t launch_made_batch_index(int B, int64_t n, uint32_t step, uint32_t seed_lo, uint32_t seed_hi, int64_t* idx,
+                                   hipStream_t stream) {
+    hipLaunchKernelGGL(k_made_batch_index, dim3((B + 255) / 256), dim3(256), 0, stream, B, n, step, seed_lo, seed_hi,
+                       idx);
+    return hipGetLastError();
+}
+
+// one thread per column; rows summed in order
+__global__ __launch_bounds__(256) void k_colsum(int rows, int cols, int n_live, const float* __restrict__ G,
+                                                float* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    if (c < n_live)
+        for (int r = 0; r < rows; r++) s += G[(long)r * cols + c];
+    out[c] = s;
+}
+
+hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_colsum, dim3((cols + 255) / 256), dim3(256), 0, stream, rows, cols, n_live, G, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, float c1, float c2,
+                                              float lr, float b1, float b2, float eps) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const float gk = g[k];
+        const float mk_ = b1 * m[k] + (1.0f - b1) * gk;
+        const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+        m[k] = mk_;
+        v[k] = vk;
+        p[k] -= lr * (mk_ * c1) / (sqrtf(vk * c2) + eps);
+    }
+}
+
+hipError_t launch_adam(int64_t n, float* p, const float* g, float* m, float* v, int t, float lr, float b1, float b2,
+                       float eps, hipStream_t stream) {
+    const float c1 = (float)(1.0 / (1.0 - pow((double)b1, t)));
+    const float c2 = (float)(1.0 / (1.0 - pow((double)b2, t)));
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, stream, n, p, g, m, v, c1, c2, lr, b1, b2, eps);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_made_inf_weights(MadeShape s, const float* __restrict__ W1,
+                                                          const float* __restrict__ W2, uint16_t* __restrict__ W1t,
+                                                          uint16_t* __restrict__ W2b) {
+    const int64_t total = (int64_t)s.N * s.Hp;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / s.Hp), k = (int)(t % s.Hp);
+        const __bf16 a = (__bf16)W1[(long)k * s.Np + i];
+        const __bf16 b = (__bf16)W2[(long)i * s.Hp + k];
+        W1t[t] = __builtin_bit_cast(uint16_t, a);
+        W2b[t] = __builtin_bit_cast(uint16_t, b);
+    }
+}
+
+hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, const float* W2, uint16_t* W1t_bf,
+                                         uint16_t* W2_bf, hipStream_t stream) {
+    hipLaunchKernelGGL(k_made_inf_weights, dim3(256), dim3(256), 0, stream, s, W1, W2, W1t_bf, W2_bf);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Autoregressive sampling (SCORE = false) and scoring (SCORE = true).
+// A 16-lane row owns one configuration; lane l holds hidden units
+// [l HPL, (l + 1) HPL) as FP32 pre-activations a and bfloat16 pairs of relu(a).
+// Per position j:
+//   l_j = b2_j + sum_k W2[j][k] relu(a_k)   (v_dot2c_f32_bf16, then a DPP row sum)
+//   x_j drawn (or read), log q += log sigmoid(+-l_j)
+//   if x_j = 1: a += W1[:, j] (FP32 adds of the bfloat16 weights), repack relu(a)
+// Both weight matrices sit in LDS as bfloat16 (N x Hp each).
+// Uniforms: lane l of the row computes Philox blocks l and 16 + l of the
+// configuration; position j reads word j % 4 of block j / 4 from its lane.
+// ---------------------------------------------------------------------------
+RBM_DI float dot2_bf16(uint32_t w, uint32_t h, float c) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, h), c, false);
+}
+RBM_DI uint32_t pack_relu_bf16(float lo, float hi) {
+    bf16x2 v;
+    v.x = (__bf16)fmaxf(lo, 0.0f);
+    v.y = (__bf16)fmaxf(hi, 0.0f);
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+int made_hidden_per_lane(int Hp) { return Hp / 16; }
+
+template <int HPL, bool SCORE>
+__global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint32_t seed_lo, uint32_t seed_hi,
+                                                      uint32_t call, const uint64_t* __restrict__ in_spins,
+                                                      uint64_t* __restrict__ out_spins, double* __restrict__ log_q) {
+    static_assert(HPL % 4 == 0, "hidden units per lane must be a multiple of 4");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int N = m.N, Hp = m.Hp;
+    uint16_t* sW1t = reinterpret_cast<uint16_t*>(smem_raw);
+    uint16_t* sW2 = sW1t + (size_t)N * Hp;
+    {
+        const uint4* g1 = reinterpret_cast<const uint4*>(m.W1t);
+        const uint4* g2 = reinterpret_cast<const uint4*>(m.W2);
+        uint4* l1 = reinterpret_cast<uint4*>(sW1t);
+        uint4* l2 = reinterpret_cast<uint4*>(sW2);
+        const int nv = N * Hp / 8;
+        for (int k = threadIdx.x; k < nv; k += blockDim.x) { l1[k] = g1[k]; l2[k] = g2[k]; }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 15, rows = blockDim.x >> 4;
+    const int row_base = (threadIdx.x & 63) & ~15;
+    const int k0 = lane * HPL;
+    float b1v[HPL];
+#pragma unroll
+    for (int u = 0; u < HPL; u++) b1v[u] = m.b1[k0 + u];
+
+    for (int64_t g = (int64_t)blockIdx.x * rows + (threadIdx.x >> 4); g < n; g += (int64_t)gridDim.x * rows) {
+        uint32_t uw[8];
+        uint64_t x0 = 0, x1 = 0;
+        if (SCORE) {
+            x0 = in_spins[2 * g];
+            x1 = in_spins[2 * g + 1];
+        } else {
+            const u32x4 r0 = philox((uint32_t)g, (uint32_t)(g >> 32), call, 0x100u + lane, seed_lo, seed_hi);
+            uw[0] = r0.x; uw[1] = r0.y; uw[2] = r0.z; uw[3] = r0.w;
+            if (N > 64) {
+                const u32x4 r1 = philox((uint32_t)g, (uint32_t)(g >> 32), call, 0x110u + lane, seed_lo, seed_hi);
+                uw[4] = r1.x; uw[5] = r1.y; uw[6] = r1.z; uw[7] = r1.w;
+            } else {
+                uw[4] = uw[5] = uw[6] = uw[7] = 0u;
+            }
+        }
+        float a[HPL];
+        uint32_t hb[HPL / 2];
+#pragma unroll
+        for (int u = 0; u < HPL; u++) a[u] = b1v[u];
+#pragma unroll
+        for (int u = 0; u < HPL / 2; u++) hb[u] = pack_relu_bf16(a[2 * u], a[2 * u + 1]);
+        double lq = 0.0;
+        for (int jb = 0; jb < N; jb += 4) {
+            uint32_t cur[4];
+            if (!SCORE) {
+                const int b = jb >> 2;
+                const int src = row_base | (b & 15);
+#pragma unroll
+                for (int q = 0; q < 4; q++) cur[q] = (uint32_t)__shfl((int)((b >> 4) ? uw[4 + q] : uw[q]), src, 64);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = jb + q;
+                if (j >= N) break;
+                const uint2* w2 = reinterpret_cast<const uint2*>(sW2 + (size_t)j * Hp + k0);
+                float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+                for (int c = 0; c < HPL / 4; c++) {
+                    const uint2 w = w2[c];
+                    s0 = dot2_bf16(w.x, hb[2 * c], s0);
+                    s1 = dot2_bf16(w.y, hb[2 * c + 1], s1);
+                }
+                const float l = row_sum16(s0 + s1) + m.b2[j];
+                const float e = __expf(-fabsf(l));
+                const float L1 = __logf(1.0f + e);
+                bool x;
+                if (SCORE) {
+                    x = bit_of(x0, x1, j) != 0;
+                } else {
+                    const float r = __builtin_amdgcn_rcpf(1.0f + e);
+                    const float p = l >= 0.0f ? r : e * r;
+                    const float u = __builtin_fmaf((float)cur[q], 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+                    x = u < p;
+                    if (x) {
+                        if (j < 64) x0 |= (uint64_t)1 << j; else x1 |= (uint64_t)1 << (j - 64);
+                    }
+                }
+                // log sigmoid(l) = -(max(-l, 0) + log(1 + e^-|l|)), log(1 - sigmoid(l)) = -(max(l, 0) + ...)
+                lq -= (double)((x ? fmaxf(-l, 0.0f) : fmaxf(l, 0.0f)) + L1);
+                if (x) {
+                    const uint2* w1 = reinterpret_cast<const uint2*>(sW1t + (size_t)j * Hp + k0);
+#pragma unroll
+                    for (int c = 0; c < HPL / 4; c++) {
+                        const uint2 w = w1[c];
+                        a[4 * c + 0] += __uint_as_float(w.x << 16);
+                        a[4 * c + 1] += __uint_as_float(w.x & 0xFFFF0000u);
+                        a[4 * c + 2] += __uint_as_float(w.y << 16);
+                        a[4 * c + 3] += __uint_as_float(w.y & 0xFFFF0000u);
+                    }
+#pragma unroll
+                    for (int u = 0; u < HPL / 2; u++) hb[u] = pack_relu_bf16(a[2 * u], a[2 * u + 1]);
+                }
+            }
+        }
+        if (lane == 0) {
+            if (!SCORE) {
+                out_spins[2 * g] = x0;
+                out_spins[2 * g + 1] = x1;
+            }
+            log_q[g] = lq;
+        }
+    }
+}
+
+#define RBM_HPL_LIST(X) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32)
+
+size_t made_sampler_lds(const MadeInf& m) { return (size_t)2 * m.N * m.Hp * sizeof(uint16_t); }
+
+static hipError_t launch_made_common(const MadeInf& m, int64_t n, uint32_t seed_lo, uint32_t seed_hi, uint32_t call,
+                                     const uint64_t* in_spins, uint64_t* out_spins, double* log_q, bool score,
+                                     hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const size_t lds = made_sampler_lds(m);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int hpl = made_hidden_per_lane(m.Hp);
+    const int rows = 64;
+    int64_t blocks = (n + rows - 1) / rows;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (blocks > cus) blocks = cus;
+    const dim3 grid((unsigned)blocks), block(rows * 16);
+#define RBM_MADE_CASE(H)                                                                                           \
+    case H:                                                                                                        \
+        if (score) {                                                                                               \
+            (void)hipFuncSetAttribute((const void*)&k_made_sample<H, true>,                                        \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                        \
+            hipLaunchKernelGGL((k_made_sample<H, true>), grid, block, lds, stream, m, n, seed_lo, seed_hi, call,    \
+                               in_spins, out_spins, log_q);                                                         \
+        } else {                                                                                                   \
+            (void)hipFuncSetAttribute((const void*)&k_made_sample<H, false>,                                       \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                        \
+            hipLaunchKernelGGL((k_made_sample<H, false>), grid, block, lds, stream, m, n, seed_lo, seed_hi, call,   \
+                               in_spins, out_spins, log_q);                                                         \
+        }                                                                                                          \
+        break;
+    switch (hpl) {
+        RBM_HPL_LIST(RBM_MADE_CASE)
+        default: return hipErrorInvalidValue;
+    }
+#undef RBM_MADE_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_made_sample(const MadeInf& m, int64_t n, uint32_t seed_lo, uint32_t seed_hi, uint32_t call,
+                              uint64_t* spins, double* log_q, hipStream_t stream) {
+    return launch_made_common(m, n, seed_lo, seed_hi, call, nullptr, spins, log_q, false, stream);
+}
+
+hipError_t launch_made_log_q(const MadeInf& m, int64_t n, const uint64_t* spins, double* log_q, hipStream_t stream) {
+    return launch_made_common(m, n, 0, 0, 0, spins, nullptr, log_q, true, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Independent Metropolis-Hastings, one thread per chain.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_imh(int chains, int therm, int steps, const uint64_t* __restrict__ prop,
+                                             const cx<double>* __restrict__ lp, const double* __restrict__ lq,
+                                             uint32_t seed_lo, uint32_t seed_hi, uint32_t call,
+                                             uint64_t* __restrict__ out, uint32_t* __restrict__ counts) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= chains) return;
+    int64_t cur = c;
+    double lpc = lp[cur].re, lqc = lq[cur];
+    uint32_t acc = 0;
+    const int total = therm + steps;
+    for (int t = 1; t <= total; t++) {
+        const int64_t g = (int64_t)t * chains + c;
+        const double lpg = lp[g].re, lqg = lq[g];
+        const double r = 2.0 * (lpg - lpc) + lqc - lqg;
+        const u32x4 R = philox((uint32_t)c, (uint32_t)t, call, 2u, seed_lo, seed_hi);
+        const double u = ((double)R.x + 0.5) * (1.0 / 4294967296.0);
+        if (u <= exp(r)) {
+            cur = g;
+            lpc = lpg;
+            lqc = lqg;
+            acc++;
+        }
+        if (t > therm) {
+            const int64_t k = (int64_t)(t - therm - 1) * chains + c;
+            out[2 * k] = prop[2 * cur];
+            out[2 * k + 1] = prop[2 * cur + 1];
+        }
+    }
+    if (counts) {
+        counts[2 * c] = acc;
+        counts[2 * c + 1] = (uint32_t)total - acc;
+    }
+}
+
+hipError_t launch_imh(int chains, int therm, int steps, const uint64_t* prop_spins, const cx<double>* log_psi,
+                      const double* log_q, uint32_t seed_lo, uint32_t seed_hi, uint32_t call, uint64_t* out_spins,
+                      uint32_t* counts, hipStream_t stream) {
+    hipLaunchKernelGGL(k_imh, dim3((chains + 255) / 256), dim3(256), 0, stream, chains, therm, steps, prop_spins,
+                       log_psi, log_q, seed_lo, seed_hi, call, out_spins, counts);
+    return hipGetLastError();
+}
+
+}  // namespace rbmhip

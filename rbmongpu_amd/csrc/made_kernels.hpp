@@ -1,0 +1,95 @@
+// Masked autoregressive proposal (MADE, Germain et al. 2015) on gfx950:
+// FP32 MFMA training kernels and a bfloat16-weight sampler.  The reference
+// has no counterpart; the model is defined in the C ABI header.
+//
+// Device layout (padded to multiples of 64 so every GEMM tile is full):
+//   Np = round_up(N, 64), Hp = round_up(H, 64)
+//   W1  Hp x Np  FP32 (zero outside the mask and the padding)
+//   b1  Hp,  W2  Np x Hp,  b2  Np
+// Inference copies: W1t_bf  N x Hp bfloat16 (row i = column i of W1),
+//                   W2_bf   N x Hp bfloat16 (row j of W2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rbm_device.hpp"
+
+namespace rbmhip {
+
+struct MadeShape {
+    int N, H, Np, Hp;
+};
+
+inline int made_pad(int x) { return (x + 63) / 64 * 64; }
+
+// degree of hidden unit k: 1 + k mod (N - 1) (0 for N = 1)
+__host__ __device__ inline int made_degree(int k, int N) { return N > 1 ? 1 + k % (N - 1) : 0; }
+
+enum GemmEpi {
+    EPI_STORE = 0,       // C = acc
+    EPI_BIAS_RELU = 1,   // C = acc + bias[n] (pre-activation), C2 = relu(C)
+    EPI_LOGITS = 2,      // l = acc + bias[n]; C = (sigmoid(l) - x) / B on live entries, 0 elsewhere; log-q partials
+    EPI_MASK_W2 = 3,     // C (Np x Hp) = acc where deg(n) <= m < N and n < H, else 0
+    EPI_MASK_W1 = 4,     // C (Hp x Np) = acc where n < deg(m), m < H and n < N, else 0
+    EPI_RELU_GRAD = 5,   // C = acc where aux[m][n] > 0, else 0
+};
+
+struct GemmEpiArgs {
+    const float* bias;   // EPI_BIAS_RELU, EPI_LOGITS
+    float* C2;           // EPI_BIAS_RELU
+    const float* aux;    // EPI_LOGITS: X (B x Np); EPI_RELU_GRAD: pre-activation
+    long ld_aux;
+    int n_live, m_live;  // live columns / rows (N or H, real batch)
+    float inv_batch;
+    double* partials;    // EPI_LOGITS: one log-q partial per workgroup
+};
+
+// C[m][n] = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn];
+// M, N multiples of 64, K a multiple of 16.
+hipError_t launch_gemm(int epi, int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk,
+                       long sbn, float* C, long ldc, const GemmEpiArgs& e, hipStream_t stream);
+
+// X[s][i] = bit i of spins[idx[s]] (or of spins[s] without idx) for i < N, 0 for the padding
+// and for rows s >= n_live.
+hipError_t launch_made_gather(int64_t rows, int n_live, int N, int Np, const uint64_t* spins, const int64_t* idx,
+                              float* X, hipStream_t stream);
+// batch indices: idx[s] = floor(u * n) with u from Philox(seed; ctr = (step, s, 0, 4))
+hipError_t launch_made_batch_index(int B, int64_t n, uint32_t step, uint32_t seed_lo, uint32_t seed_hi, int64_t* idx,
+                                   hipStream_t stream);
+// out[n] = sum_m G[m][n] over rows m < rows (ld = cols), fixed order
+hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, hipStream_t stream);
+// Adam (Kingma & Ba 2015) over n parameters; t is the 1-based step
+hipError_t launch_adam(int64_t n, float* p, const float* g, float* m, float* v, int t, float lr, float b1, float b2,
+                       float eps, hipStream_t stream);
+// bfloat16 inference copies of W1 (transposed) and W2
+hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, const float* W2, uint16_t* W1t_bf,
+                                         uint16_t* W2_bf, hipStream_t stream);
+
+struct MadeInf {
+    int N, H, Hp;
+    const uint16_t* W1t;   // N x Hp
+    const uint16_t* W2;    // N x Hp
+    const float* b1;       // Hp (zero padded)
+    const float* b2;       // >= N
+};
+size_t made_sampler_lds(const MadeInf& m);
+int made_hidden_per_lane(int Hp);
+// Autoregressive sampling of n configurations: proposal g draws its uniforms
+// from Philox(seed; ctr = (g_lo, g_hi, call, 0x100 + j / 4)), word j % 4,
+// u = (w + 0.5) 2^-32 in FP32, x_j = 1 iff u < sigmoid(l_j).
+hipError_t launch_made_sample(const MadeInf& m, int64_t n, uint32_t seed_lo, uint32_t seed_hi, uint32_t call,
+                              uint64_t* spins, double* log_q, hipStream_t stream);
+// log q of given configurations with the sampler's arithmetic
+hipError_t launch_made_log_q(const MadeInf& m, int64_t n, const uint64_t* spins, double* log_q, hipStream_t stream);
+
+// Independent Metropolis-Hastings over precomputed proposals: chain c uses
+// proposals t * chains + c for t = 0 .. therm + steps; proposal 0 starts the
+// chain and t >= 1 is accepted iff u <= exp(2 (Re lp' - Re lp) + lq - lq'),
+// u from Philox(seed; ctr = (c, t, call, 2)) in FP64.  The configuration held
+// after measured proposal therm + k is written to sample k * chains + c.
+hipError_t launch_imh(int chains, int therm, int steps, const uint64_t* prop_spins, const cx<double>* log_psi,
+                      const double* log_q, uint32_t seed_lo, uint32_t seed_hi, uint32_t call, uint64_t* out_spins,
+                      uint32_t* counts, hipStream_t stream);
+
+}  // namespace rbmhip

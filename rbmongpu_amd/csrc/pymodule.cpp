@@ -231,6 +231,26 @@ struct EnsBase {
 struct MonteCarloLoop : EnsBase {};
 struct ExactSummation : EnsBase {};
 
+// Masked autoregressive proposal network (no reference counterpart).
+struct MADE {
+    rbmhip_made* h = nullptr;
+    int N = 0, H = 0;
+    ~MADE() { if (h) rbmhip_made_destroy(h); }
+    int64_t num_params() const { int64_t n = 0; check(rbmhip_made_num_params(h, &n)); return n; }
+};
+// Independent Metropolis-Hastings with a MADE proposal; keeps its network alive.
+struct LearnedMonteCarloLoop : EnsBase {
+    py::object made;
+};
+
+using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using u64arr = py::array_t<uint64_t, py::array::c_style | py::array::forcecast>;
+
+static int64_t spin_rows(const u64arr& spins) {
+    if (spins.ndim() != 2 || spins.shape(1) != 2) throw py::value_error("spins must be an (n, 2) uint64 array");
+    return (int64_t)spins.shape(0);
+}
+
 // Spins (include/Spins.h:225-333), one 64-bit word as bound by the reference.
 struct Spins {
     uint64_t c = 0;
@@ -634,6 +654,112 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
             return py::make_tuple(spins, lp, el);
         });
 
+    py::class_<MADE>(m, "MADE")
+        .def(py::init([](int N, int hidden, uint64_t seed, py::object params) {
+                 auto md = std::make_unique<MADE>();
+                 md->N = N; md->H = hidden;
+                 if (params.is_none()) {
+                     check(rbmhip_made_create(N, hidden, nullptr, seed, &md->h));
+                 } else {
+                     farr p = params.cast<farr>();
+                     if (p.size() != 2 * (py::ssize_t)N * hidden + hidden + N) throw py::value_error("params has the wrong size");
+                     check(rbmhip_made_create(N, hidden, p.data(), seed, &md->h));
+                 }
+                 return md;
+             }),
+             py::arg("N"), py::arg("hidden") = 256, py::kw_only(), py::arg("seed") = 0, py::arg("params") = py::none())
+        .def_readonly("N", &MADE::N)
+        .def_readonly("hidden", &MADE::H)
+        .def_property_readonly("num_params", &MADE::num_params)
+        .def_property("params",
+                      [](const MADE& md) {
+                          farr out((py::ssize_t)md.num_params());
+                          check(rbmhip_made_get_params(md.h, out.mutable_data()));
+                          return out;
+                      },
+                      [](MADE& md, const farr& p) {
+                          if (p.size() != md.num_params()) throw py::value_error("params has the wrong size");
+                          check(rbmhip_made_set_params(md.h, p.data()));
+                      })
+        .def("loss_and_gradient", [](MADE& md, const u64arr& spins) {
+            const int64_t n = spin_rows(spins);
+            farr g((py::ssize_t)md.num_params());
+            double loss = 0.0;
+            {
+                py::gil_scoped_release nogil;
+                check(rbmhip_made_loss_grad(md.h, spins.data(), n, &loss, g.mutable_data()));
+            }
+            return py::make_tuple(loss, g);
+        })
+        .def("train", [](MADE& md, const u64arr& spins, int steps, int batch, double lr, uint64_t seed) {
+                 const int64_t n = spin_rows(spins);
+                 darr losses((py::ssize_t)steps);
+                 {
+                     py::gil_scoped_release nogil;
+                     check(rbmhip_made_train(md.h, spins.data(), n, steps, batch, lr, seed, losses.mutable_data()));
+                 }
+                 return losses;
+             },
+             py::arg("spins"), py::arg("steps"), py::arg("batch") = 4096, py::arg("lr") = 1e-3, py::arg("seed") = 0)
+        .def("reset_optimizer", [](MADE& md) { check(rbmhip_made_reset_optimizer(md.h)); })
+        .def("log_q", [](MADE& md, const u64arr& spins) {
+            const int64_t n = spin_rows(spins);
+            darr out((py::ssize_t)n);
+            {
+                py::gil_scoped_release nogil;
+                check(rbmhip_made_log_q(md.h, spins.data(), n, out.mutable_data()));
+            }
+            return out;
+        })
+        .def("sample", [](MADE& md, int64_t n, uint64_t seed) {
+                 py::array_t<uint64_t> spins({(py::ssize_t)n, (py::ssize_t)2});
+                 darr lq((py::ssize_t)n);
+                 {
+                     py::gil_scoped_release nogil;
+                     check(rbmhip_made_sample(md.h, n, seed, spins.mutable_data(), lq.mutable_data()));
+                 }
+                 return py::make_tuple(spins, lq);
+             },
+             py::arg("n"), py::arg("seed") = 0);
+
+    py::class_<LearnedMonteCarloLoop>(m, "LearnedMonteCarloLoop")
+        .def(py::init([](py::object made, unsigned num_samples, unsigned num_therm, unsigned num_chains, bool gpu,
+                         uint64_t seed) {
+                 require_gpu(gpu);
+                 MADE& md = made.cast<MADE&>();
+                 auto e = std::make_unique<LearnedMonteCarloLoop>();
+                 e->gpu = gpu;
+                 e->made = made;
+                 check(rbmhip_learned_create(md.h, num_samples, num_therm, num_chains, seed, &e->h));
+                 return e;
+             }),
+             py::arg("made"), py::arg("num_samples"), py::arg("num_thermalization_steps"), py::arg("num_markov_chains"),
+             py::arg("gpu"), py::kw_only(), py::arg("seed") = 0)
+        .def_property_readonly("num_steps", &LearnedMonteCarloLoop::num_steps)
+        .def_property_readonly("acceptance_rate", [](const LearnedMonteCarloLoop& e) {
+            uint64_t a = 0, r = 0;
+            check(rbmhip_mc_acceptance(e.h, &a, &r));
+            return (float)a / (float)(a + r);
+        })
+        .def("last_proposals", [](const LearnedMonteCarloLoop& e) {
+            int64_t n = 0;
+            check(rbmhip_learned_last_proposals(e.h, &n, nullptr, nullptr, nullptr));
+            py::array_t<uint64_t> spins({(py::ssize_t)n, (py::ssize_t)2});
+            darr lq((py::ssize_t)n);
+            carr lp((py::ssize_t)n);
+            check(rbmhip_learned_last_proposals(e.h, &n, spins.mutable_data(), lq.mutable_data(),
+                                                reinterpret_cast<double*>(lp.mutable_data())));
+            return py::make_tuple(spins, lq, lp);
+        })
+        .def("last_samples", [](const LearnedMonteCarloLoop& e) {
+            const uint64_t n = e.num_steps();
+            py::array_t<uint64_t> spins({(py::ssize_t)n, (py::ssize_t)2});
+            carr lp((py::ssize_t)n), el((py::ssize_t)n);
+            check(rbmhip_mc_last_samples(e.h, spins.mutable_data(), reinterpret_cast<double*>(lp.mutable_data()),
+                                         reinterpret_cast<double*>(el.mutable_data())));
+            return py::make_tuple(spins, lp, el);
+        });
+
     py::class_<ExactSummation>(m, "ExactSummation")
         .def(py::init([](unsigned N, bool gpu) {
             require_gpu(gpu);
@@ -665,6 +791,8 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
     RBM_EV_BIND(Psi, ExactSummation)
     RBM_EV_BIND(PsiDeep, MonteCarloLoop)
     RBM_EV_BIND(PsiDeep, ExactSummation)
+    RBM_EV_BIND(Psi, LearnedMonteCarloLoop)
+    RBM_EV_BIND(PsiDeep, LearnedMonteCarloLoop)
 #undef RBM_EV_BIND
 
     auto hs = py::class_<HilbertSpaceDistance>(m, "HilbertSpaceDistance");
@@ -681,6 +809,8 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
     RBM_HS_BIND(Psi, ExactSummation)
     RBM_HS_BIND(PsiDeep, MonteCarloLoop)
     RBM_HS_BIND(PsiDeep, ExactSummation)
+    RBM_HS_BIND(Psi, LearnedMonteCarloLoop)
+    RBM_HS_BIND(PsiDeep, LearnedMonteCarloLoop)
 #undef RBM_HS_BIND
 
     m.def("log_psi_batch", &log_psi_batch<Psi>, "log psi of (n, 2) uint64 spin words");
