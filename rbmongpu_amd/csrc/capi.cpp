@@ -18,6 +18,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rbm_kernels.hpp"
@@ -714,8 +715,19 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
             e->prop_n = P;
             const MadeInf mi = made_inf(c, e->made);
             c.launch("made_sample", [&] { return launch_made_sample(mi, P, mp.seed_lo, mp.seed_hi, mp.call, prop, plq, c.stream); });
-            Records prec{nullptr, plp, nullptr, nullptr};
-            eval_model<T>(c, psi, nullptr, P, prop, 0, prec, "proposal_eval");
+            bool done = false;
+            if constexpr (std::is_same<T, float>::value) {
+                if (psi->kind == 0 && !psi->has_a) {
+                    // log psi of independent configurations: one GEMM over all proposals
+                    const RbmDev<float> md = rbm_dev<float>(c, psi);
+                    c.launch("proposal_eval", [&] { return launch_theta_logpsi(md, P, prop, plp, c.stream); });
+                    done = true;
+                }
+            }
+            if (!done) {
+                Records prec{nullptr, plp, nullptr, nullptr};
+                eval_model<T>(c, psi, nullptr, P, prop, 0, prec, "proposal_eval");
+            }
             c.launch("imh", [&] {
                 return launch_imh(mp.chains, mp.therm, mp.steps_per_chain, prop, plp, plq, mp.seed_lo, mp.seed_hi, mp.call,
                                   s.spins, counts, c.stream);

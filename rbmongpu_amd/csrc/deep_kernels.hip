@@ -94,7 +94,7 @@ RBM_DI cx<double> local_energy_deep(const DeepDev<T>& md, const OpDev& op, uint6
         uint64_t p0 = s0, p1 = s1;
         for (int q = op.flip_begin[t]; q < op.flip_begin[t + 1]; q++) flip_bit(p0, p1, op.flips[q]);
         const cx<T> lpp = deep_log_psi(md, p0, p1, g, lane);
-        E = E + term_coef(op, t, s0, s1) * to_d(cexp(lpp - lp));
+        E = E + term_coef(op, t, s0, s1) * cexp_wide(lpp - lp);
     }
     return E;
 }

@@ -55,12 +55,29 @@ RBM_DI cx<float> cexp(cx<float> z) {
     const float e = __expf(z.re);
     return mk(e * __builtin_amdgcn_cosf(rev), e * __builtin_amdgcn_sinf(rev));
 }
+// exp(z) delivered in FP64 with FP64's exponent range: exp(Re z) = 2^k 2^f
+// with k = floor(Re z log2 e), 2^f from v_exp_f32 and 2^k applied by
+// v_ldexp_f64, so a ratio psi(s')/psi(s) beyond FP32's range (Re z > 88)
+// stays finite.  Relative accuracy as cexp above.
+RBM_DI cx<double> cexp_wide(cx<float> z) {
+    const float t = z.re * 1.4426950408889634f;
+    const float k = __builtin_floorf(t);
+    const float m = __builtin_amdgcn_exp2f(t - k);
+    const float kk = __builtin_rintf(z.im * 0.15915494309189535f);
+    float r = __builtin_fmaf(-kk, 6.28318548202514648f, z.im);
+    r = __builtin_fmaf(-kk, -1.7484555e-07f, r);
+    const float rev = r * 0.15915494309189535f;
+    const int e = (int)__builtin_fmaxf(__builtin_fminf(k, 2000.0f), -2000.0f);
+    return mk(__builtin_ldexp((double)(m * __builtin_amdgcn_cosf(rev)), e),
+              __builtin_ldexp((double)(m * __builtin_amdgcn_sinf(rev)), e));
+}
 RBM_DI cx<double> cexp(cx<double> z) {
     double s, c;
     sincos(z.im, &s, &c);
     const double e = exp(z.re);
     return mk(e * c, e * s);
 }
+RBM_DI cx<double> cexp_wide(cx<double> z) { return cexp(z); }
 
 // Shared denominator of the approximant:
 // 9.19376335670885 + z (s 10.218.. + z (7.7714.. + z (s 3.7466.. + z)))

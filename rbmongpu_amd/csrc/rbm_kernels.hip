@@ -69,7 +69,7 @@ RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict_
             }
         }
         const cx<double> c = term_coef(op, t, s0, s1);
-        E = E + c * to_d(cexp(d));
+        E = E + c * cexp_wide(d);
     }
     return E;
 }
@@ -804,6 +804,119 @@ hipError_t launch_s_matrix_ref(int64_t n, int N, int M, int has_a, const void* T
     if (g > 8192) g = 8192;
     hipLaunchKernelGGL((k_s_matrix_ref<T>), dim3((unsigned)g), dim3(256), 0, stream, n, N, M, has_a,
                        reinterpret_cast<const cx<T>*>(Trec), spins, inv, S);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// log psi of n configurations as an FP32 MFMA GEMM (RBM without visible bias):
+// theta = b + W^T s over 64 configurations per wave, the hidden units in
+// chunks of 32 whose W columns are staged in LDS, my_logcosh summed over the
+// units in the epilogue (Psi::log_psi_s, include/quantum_state/Psi.hpp:55-63).
+// MFMA 16x16x4 f32: lane l supplies A[l % 16][l / 16] (spin s_k of
+// configuration row l % 16, k = k0 + l / 16, as +-1) and B[l / 16][l % 16]
+// (W[k][unit]); accumulator element v of lane l is D[4 (l / 16) + v][l % 16].
+// ---------------------------------------------------------------------------
+constexpr int TL_ROWS = 64, TL_UNITS = 32, TL_LDS_STRIDE = TL_UNITS + 1;
+
+__global__ __launch_bounds__(256) void k_theta_logpsi(RbmDev<float> md, int64_t n, const uint64_t* __restrict__ spins,
+                                                      cx<double>* __restrict__ lp) {
+    __shared__ float2 Ws[128 * TL_LDS_STRIDE];
+    const int N = md.N, M = md.M, ld = md.ld;
+    const int Kp = (N + 3) & ~3;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * TL_ROWS;
+    // spin words of the configuration of A-operand row (lane % 16) in each row tile
+    uint32_t w[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int64_t p = base + 16 * r + (lane & 15);
+        uint64_t s0 = 0, s1 = 0;
+        if (p < n) { s0 = spins[2 * p]; s1 = spins[2 * p + 1]; }
+        w[r][0] = (uint32_t)s0; w[r][1] = (uint32_t)(s0 >> 32);
+        w[r][2] = (uint32_t)s1; w[r][3] = (uint32_t)(s1 >> 32);
+    }
+    float lre[4][4], lim[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) { lre[r][v] = 0.0f; lim[r][v] = 0.0f; }
+    const int kq = lane >> 4;
+    for (int u0 = 0; u0 < ld; u0 += TL_UNITS) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < Kp * TL_UNITS; t += blockDim.x) {
+            const int k = t / TL_UNITS, j = t % TL_UNITS;
+            float2 x = make_float2(0.0f, 0.0f);
+            if (k < N && u0 + j < ld) {
+                const cx<float> c = md.W[k * ld + u0 + j];
+                x = make_float2(c.re, c.im);
+            }
+            Ws[k * TL_LDS_STRIDE + j] = x;
+        }
+        __syncthreads();
+        floatx4 acc[4][2][2];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                acc[r][c][0] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc[r][c][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+            }
+        for (int k0 = 0; k0 < Kp; k0 += 4) {
+            const int k = k0 + kq;
+            const int word = k0 >> 5, sh = k & 31;   // k0 .. k0 + 3 share one 32-bit word
+            float a[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint32_t x = word == 0 ? w[r][0] : word == 1 ? w[r][1] : word == 2 ? w[r][2] : w[r][3];
+                a[r] = ((x >> sh) & 1u) ? 1.0f : -1.0f;
+            }
+            float bre[2], bim[2];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float2 x = Ws[k * TL_LDS_STRIDE + 16 * c + (lane & 15)];
+                bre[c] = x.x;
+                bim[c] = x.y;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    acc[r][c][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bre[c], acc[r][c][0], 0, 0, 0);
+                    acc[r][c][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bim[c], acc[r][c][1], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int j = u0 + 16 * c + (lane & 15);
+            if (j < M) {
+                const cx<float> bj = md.b[j];
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const cx<float> lc = logcosh(mk(acc[r][c][0][v] + bj.re, acc[r][c][1][v] + bj.im));
+                        lre[r][v] += lc.re;
+                        lim[r][v] += lc.im;
+                    }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float sr = row_sum16(lre[r][v]);
+            const float si = row_sum16(lim[r][v]);
+            const int64_t p = base + 16 * r + 4 * kq + v;
+            if ((lane & 15) == 0 && p < n) lp[p] = mk((double)sr, (double)si);
+        }
+}
+
+hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_t* spins, cx<double>* lp,
+                               hipStream_t stream) {
+    if (md.a || md.N > 128 || n <= 0) return hipErrorInvalidValue;
+    const int64_t blocks = (n + 4 * TL_ROWS - 1) / (4 * TL_ROWS);
+    hipLaunchKernelGGL(k_theta_logpsi, dim3((unsigned)blocks), dim3(256), 0, stream, md, n, spins, lp);
     return hipGetLastError();
 }
 

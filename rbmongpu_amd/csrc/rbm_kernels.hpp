@@ -89,5 +89,9 @@ template <typename T>
 hipError_t launch_s_matrix_ref(int64_t n, int N, int M, int has_a, const void* Trec, const uint64_t* spins, double inv,
                                cx<double>* S, hipStream_t stream);
 hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, int deriv, hipStream_t stream);
+// log psi of n configurations of an FP32 RBM without visible bias (N <= 128) as
+// an MFMA GEMM theta = b + W^T s with a log-cosh epilogue.
+hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_t* spins, cx<double>* lp,
+                               hipStream_t stream);
 
 }  // namespace rbmhip

@@ -232,3 +232,20 @@ def test_learned_energy_and_gradient_match_exact_summation():
     assert abs(E.real - E_ex.real) < 3 * se, (E, E_ex, se)
     assert ens.acceptance_rate > 0.3
     assert rel_err(g, g_ex) < 0.1
+
+
+@pytest.mark.parametrize("N,M", [(100, 200), (37, 50), (128, 16), (5, 3)])
+def test_proposal_log_psi_on_mfma_matches_oracle(N, M):
+    """complex64 RBMs score the proposals with the MFMA theta GEMM."""
+    print(f"[proposal log psi] N={N} M={M}", flush=True)
+    b, W, _ = random_rbm(N, M, seed=N + M, scale=0.05)
+    psi = gpu_psi(b, W, precision=32)
+    md = R.MADE(N, 64, seed=1)
+    chains = 256
+    ens = R.LearnedMonteCarloLoop(md, chains * 3, 2, chains, True, seed=1)
+    R.ExpectationValue(True)(psi, R.Operator(*HAM.tfim_1d(N), True), ens)
+    prop, _, plp = ens.last_proposals()
+    ref = O.log_psi(O.RBM(b, W), prop)
+    assert rel_err(plp, ref) < 1e-5
+    # the same configurations through the evaluation kernel
+    assert rel_err(R.log_psi_batch(psi, prop), plp) < 1e-5

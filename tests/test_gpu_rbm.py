@@ -78,6 +78,22 @@ def test_log_psi_shapes(precision, N, M):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+def test_local_energy_ratios_beyond_fp32_range(precision):
+    """psi(s')/psi(s) up to e^400: complex64 kernels still return finite E_loc
+    (the ratio's exponent is applied in FP64)."""
+    N, M = 6, 3
+    rng = np.random.default_rng(5)
+    W = 40.0 * rng.normal(size=(N, M)) + 1j * rng.normal(size=(N, M))
+    b = rng.normal(size=M) + 1j * rng.normal(size=M)
+    tab = HAM.tfim_1d(N, 1.0, 0.5)
+    ref = O.local_energy(O.RBM(b, W), O.Operator(*tab), basis_words(N))
+    assert np.abs(ref).max() > np.exp(90.0)
+    got = R.local_energy_batch(gpu_psi(b, W, precision=precision), gpu_op(tab), basis_words(N))
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - ref) / np.abs(ref)) < TOL[precision] * 10
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("N,M", [(31, 20), (33, 17), (64, 40), (65, 33), (100, 48), (128, 16)])
 def test_local_energy_wide(precision, N, M):
     """Flip sites and Z/Y factors in every 32-bit quarter of the two spin words."""
