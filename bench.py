@@ -1,21 +1,36 @@
 #!/usr/bin/env python3
-"""Benchmark: effective samples/sec including E_loc and O_k on one MI355X.
+"""Benchmark: effective samples/sec including E_loc and O_k on one MI355X,
+learned (MADE independent Metropolis-Hastings) against plain single-spin-flip
+proposals.
 
-Workload (the plain-method GPU configuration): 1-D transverse-field Ising
-chain N=20, periodic, J=h=1; RBM with M=80 hidden units, complex64 parameters
-drawn i.i.d. N(0, 0.01^2) (seed 1236); 4096 Metropolis single-spin-flip
-chains, 200 thermalisation sweeps and 1000 measurement sweeps (one sample per
-sweep of N proposals).  One step is one ExpectationValue.gradient call: the
-full sampling pass, E_loc of every sample, O_k = my_tanh(theta) and the
-FP64 gradient contraction.
+Default workload (--config c5, the learned-proposal configuration): 2-D
+transverse-field Ising model on a 10x10 periodic lattice, J = 1, h = 3.04;
+RBM with M = 200 hidden units, complex64 parameters drawn i.i.d. N(0, 0.01^2)
+(seed 1238), then 50 Adam steps (lr 1e-2) on the energy gradient from 8192
+chains x 10 samples.  Both samplers then run 8192 chains:
+  plain    MonteCarloLoop: 200 thermalisation sweeps, one sample per sweep of N
+           single-spin-flip proposals;
+  learned  MADE 100 -> 256 -> 100 trained for 2000 Adam steps (lr 1e-3,
+           batch 4096) on 10^6 samples of the plain chain, used as an
+           independent Metropolis-Hastings proposal (20 thermalisation
+           proposals, one sample per proposal).
+One step is one ExpectationValue.gradient call: sampling, E_loc of every
+sample, O_k and the FP64 gradient contraction.  MADE training is timed
+separately (made_train_s) and is not in the step.
 
-value = chains * samples_per_chain / (2 tau_int(E_loc)) / seconds_per_step,
-with tau_int from Sokal's automatic windowing (c = 5) on the per-chain E_loc
-series.  The roofline entry prices the dominant kernel (the Metropolis
-kernel, FP32 vector ALU bound) with algorithmic FLOP from rbmongpu_amd/flops.py
-over its average launch time measured with HIP events on the library stream.
-The CPU baseline times the FP64 oracle (the reference's algorithm, with its
-per-string O_k(s') estimator) on one host core over a bounded sample.
+value = chains * samples_per_chain / (2 tau_int(E_loc)) / seconds_per_step for
+the learned sampler, with tau_int from Sokal's automatic windowing (c = 5) on
+the per-chain E_loc series; the plain sampler is measured the same way in the
+same run (plain.value), and the two energies are compared in units of their
+combined standard error.
+
+--config c2 measures the plain GPU configuration (1-D TFIM N = 20, M = 80,
+4096 chains x (200 + 1000 sweeps)) instead.
+
+roofline prices the dominant kernel of the measured step with algorithmic FLOP
+from rbmongpu_amd/flops.py over its average launch time from HIP events on the
+library stream.  cpu_baseline times the FP64 oracle (the reference's
+algorithm, one host core) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -28,17 +43,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-N, M = 20, 80
-CHAINS, SAMPLES_PER_CHAIN, THERM, SWEEPS = 4096, 1000, 200, 1
-SEED = 1236
 PEAK_FP32_TFLOPS = 157.3
-
-
-def make_params():
-    rng = np.random.default_rng(SEED)
-    b = 0.01 * (rng.normal(size=M) + 1j * rng.normal(size=M))
-    W = 0.01 * (rng.normal(size=(N, M)) + 1j * rng.normal(size=(N, M)))
-    return b, W
+METRIC = "effective samples/sec incl. E_loc+O_k on 1 GPU: learned vs single-flip proposal"
 
 
 def tau_int(series, c=5.0):
@@ -49,6 +55,8 @@ def tau_int(series, c=5.0):
     T = x.shape[0]
     f = np.fft.rfft(x, n=2 * T, axis=0)
     acov = np.fft.irfft(f * np.conj(f), axis=0)[:T].mean(axis=1) / T
+    if acov[0] <= 0:
+        return 0.5
     rho = acov / acov[0]
     tau = 0.5
     for w in range(1, T):
@@ -59,11 +67,11 @@ def tau_int(series, c=5.0):
 
 
 def kernel_source_sha():
-    """Hash of the sampler's source: a stored traffic measurement is used only
-    for the kernel it was taken on."""
+    """Hash of the kernels' sources: a stored traffic measurement is used only
+    for the kernels it was taken on."""
     import hashlib
     h = hashlib.sha256()
-    for f in ("rbm_kernels.hip", "rbm_device.hpp", "rbm_kernels.hpp"):
+    for f in ("rbm_kernels.hip", "rbm_device.hpp", "rbm_kernels.hpp", "made_kernels.hip", "made_kernels.hpp"):
         h.update(open(os.path.join(ROOT, "rbmongpu_amd", "csrc", f), "rb").read())
     return h.hexdigest()[:16]
 
@@ -82,142 +90,257 @@ def copy_calibration(nbytes=1 << 30):
     hip.hipFree(b)
 
 
-def cpu_baseline(budget_s):
-    """The oracle's Metropolis + E_loc + O_k gradient on one core, chains of
-    the same workload (200 thermalisation sweeps, then measured sweeps)."""
+def stored_traffic(kernel_key):
+    """HBM bytes per launch of a kernel from the committed PMC summary, if it
+    was taken on the current sources."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    t = json.load(open(path))
+    if t.get("kernel_source_sha") != kernel_source_sha():
+        return None
+    return t.get("bytes_per_launch", {}).get(kernel_key)
+
+
+def rbm_params(N, M, seed):
+    rng = np.random.default_rng(seed)
+    b = 0.01 * (rng.normal(size=M) + 1j * rng.normal(size=M))
+    W = 0.01 * (rng.normal(size=(N, M)) + 1j * rng.normal(size=(N, M)))
+    return b, W
+
+
+def chain_stats(el, samples_per_chain, chains):
+    series = el.real.reshape(samples_per_chain, chains)
+    per_chain = series.mean(axis=0)
+    return tau_int(series), float(per_chain.mean()), float(per_chain.std(ddof=1) / np.sqrt(chains))
+
+
+def time_steps(R, ev, psi, op, ens, steps, warmup, kernels):
+    for _ in range(warmup):
+        ev.gradient(psi, op, ens)
+    R.synchronize()
+    R.timing_enable(True)
+    R.timing_reset()
+    R.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g, E = ev.gradient(psi, op, ens)
+    R.synchronize()
+    wall = time.perf_counter() - t0
+    R.timing_enable(False)
+    kms = {k: R.timing_query(k) for k in kernels}
+    return wall / steps, E, kms
+
+
+def cpu_baseline(psi_b, psi_W, tab, N, therm, budget_s):
+    """The oracle's Metropolis + E_loc + O_k gradient (the reference algorithm,
+    FP64, per-string O_k(s') estimator) on one core: chains of the same
+    workload (thermalisation, then measured sweeps), as many samples as fit
+    the time budget."""
     from oracle import oracle as O
-    from rbmongpu_amd import hamiltonians as HAM
-    b, W = make_params()
-    psi = O.RBM(b, W)
-    op = O.Operator(*HAM.tfim_1d(N))
-    # calibrate: 1 chain, thermalisation + a few samples
-    per = 4
+    psi = O.RBM(psi_b, psi_W)
+    op = O.Operator(*tab)
     t0 = time.perf_counter()
-    O.gradient(psi, op, O.MonteCarlo(N, per, SWEEPS, THERM, 1, seed=1), mode=0)
+    O.gradient(psi, op, O.MonteCarlo(N, 2, 1, therm, 1, seed=1), mode=0)
     t_small = time.perf_counter() - t0
-    per2 = 24
     t0 = time.perf_counter()
-    O.gradient(psi, op, O.MonteCarlo(N, per2, SWEEPS, THERM, 1, seed=2), mode=0)
+    O.gradient(psi, op, O.MonteCarlo(N, 6, 1, therm, 1, seed=2), mode=0)
     t_big = time.perf_counter() - t0
-    t_sample = max((t_big - t_small) / (per2 - per), 1e-9)
-    t_therm = max(t_small - per * t_sample, 0.0)
-    # bounded run: as many whole chains (therm + samples) as fit the budget
-    spc = max(1, int((budget_s / 4 - t_therm) / t_sample))
-    chains = 4
+    t_sample = max((t_big - t_small) / 4, 1e-9)
+    t_therm = max(t_small - 2 * t_sample, 0.0)
+    chains = 2
+    spc = max(1, int((budget_s / chains - t_therm) / t_sample))
     t0 = time.perf_counter()
-    O.gradient(psi, op, O.MonteCarlo(N, spc * chains, SWEEPS, THERM, chains, seed=3), mode=0)
+    O.gradient(psi, op, O.MonteCarlo(N, spc * chains, 1, therm, chains, seed=3), mode=0)
     wall = time.perf_counter() - t0
     return chains * spc / wall, chains, spc, wall
+
+
+# ---------------------------------------------------------------------------
+def run_c5(args, R, HAM, F):
+    L, N, M, C, H = 10, 100, 200, 8192, 256
+    spc = args.samples_per_chain or 500
+    tab = HAM.tfim_2d(L, L, 1.0, 3.04)
+    b, W = rbm_params(N, M, 1238)
+    psi = R.Psi(np.zeros(N), np.zeros(N), b, W, 1.0, False, True, precision=args.precision)
+    op = R.Operator(*tab, True)
+    ev = R.ExpectationValue(True)
+    # 50 Adam steps on the energy gradient (b and W; the first 2N entries are alpha, beta)
+    t0 = time.perf_counter()
+    mc = R.MonteCarloLoop(C * 10, 1, 20, C, True, seed=11)
+    p = psi.params
+    m = np.zeros(p.shape, complex)
+    v = np.zeros(p.shape)
+    for t in range(1, 51):
+        g, E_pre = ev.gradient(psi, op, mc)
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * np.abs(g) ** 2
+        p = p - 1e-2 * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+        p[:2 * N] = 0
+        psi.params = p
+    R.synchronize()
+    pretrain_s = time.perf_counter() - t0
+
+    kernels = ("mc", "eval", "coef", "scalar_reduce", "ok_reduce", "partials_sum", "made_sample", "proposal_eval", "imh")
+    plain = R.MonteCarloLoop(C * spc, 1, 200, C, True, seed=21)
+    p_ms, p_E, p_k = time_steps(R, ev, psi, op, plain, args.steps, args.warmup, kernels)
+    _, _, el = plain.last_samples()
+    p_tau, p_mean, p_se = chain_stats(el, spc, C)
+    p_ess = C * spc / (2 * p_tau) / p_ms
+
+    # 10^6 training samples from the plain chain, then the proposal network
+    data_mc = R.MonteCarloLoop(C * 123, 1, 200, C, True, seed=31)
+    ev(psi, op, data_mc)
+    data, _, _ = data_mc.last_samples()
+    made = R.MADE(N, H, seed=5)
+    R.synchronize()
+    t0 = time.perf_counter()
+    losses = made.train(data, 2000, batch=4096, lr=1e-3, seed=6)
+    R.synchronize()
+    made_s = time.perf_counter() - t0
+
+    learned = R.LearnedMonteCarloLoop(made, C * spc, 20, C, True, seed=41)
+    l_ms, l_E, l_k = time_steps(R, ev, psi, op, learned, args.steps, args.warmup, kernels)
+    _, _, el = learned.last_samples()
+    l_tau, l_mean, l_se = chain_stats(el, spc, C)
+    l_ess = C * spc / (2 * l_tau) / l_ms
+
+    # dominant kernel of the learned step: E_loc / O_k evaluation of the samples
+    ev_ms, ev_n = l_k["eval"]
+    ev_avg = ev_ms / max(ev_n, 1)
+    runs = learned.num_runs   # configurations measured per call: one per run of equal samples
+    terms = N
+    ev_flop = F.eval_flops(N, M, terms, runs)
+    achieved = ev_flop / (ev_avg / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": l_ess,
+        "unit": "effective samples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": l_ms * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64" if args.precision == 32 else "c128",
+        "data": "synthetic (seeded N(0, 0.01^2) RBM, 50 Adam steps; proposal network trained on the plain chain)",
+        "config": {"workload": "2-D TFIM 10x10 periodic J=1 h=3.04, RBM M=200 after 50 Adam steps; learned "
+                               "(MADE 100-256-100 independent MH) vs plain single-spin-flip, 8192 chains, "
+                               f"{spc} samples per chain, gradient step (sampling + E_loc + O_k + FP64 contraction)",
+                   "chains": C, "samples_per_chain": spc, "N": N, "M": M, "made_hidden": H,
+                   "precision": args.precision},
+        "speedup_vs_plain": l_ess / p_ess,
+        "energy_agreement_sigma": abs(l_E.real - p_E.real) / float(np.hypot(l_se, p_se)),
+        "learned": {"value": l_ess, "ms_per_step": l_ms * 1e3, "tau_int": l_tau, "energy": l_E.real,
+                    "energy_se": l_se, "acceptance_rate": learned.acceptance_rate,
+                    "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in l_k.items() if v[1]},
+                    "made_train_s": made_s, "made_loss_first_last": [float(losses[0]), float(losses[-1])]},
+        "plain": {"value": p_ess, "ms_per_step": p_ms * 1e3, "tau_int": p_tau, "energy": p_E.real, "energy_se": p_se,
+                  "acceptance_rate": plain.acceptance_rate,
+                  "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in p_k.items() if v[1]}},
+        "pretrain": {"seconds": pretrain_s, "energy_per_site": E_pre.real / N},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": stored_traffic("k_eval_rbm@c5"),
+                     "kernel": "k_eval_rbm (E_loc, log psi and my_tanh of one configuration per run)",
+                     "flop_per_launch": ev_flop, "avg_launch_ms": ev_avg,
+                     "algorithmic_bytes_per_launch": F.eval_bytes(M, runs)},
+    }
+    if not args.no_cpu_baseline:
+        cps, cchains, cspc, cwall = cpu_baseline(*R_params_of(psi, N, M), tab, N, 200, args.cpu_budget)
+        out["cpu_baseline"] = {
+            "value": cps / (2.0 * p_tau), "unit": "effective samples/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle, plain single-spin-flip chain with the reference algorithm (per-string O_k(s') "
+                      f"gradient estimator) on the same trained RBM and Hamiltonian: {cchains} chains x (200 "
+                      f"thermalisation + {cspc} measured sweeps) in {cwall:.1f} s on one core; raw {cps:.2f} "
+                      f"samples/s divided by 2 tau_int of the GPU plain chain"}
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+def R_params_of(psi, N, M):
+    b = np.asarray(psi.b)
+    W = np.asarray(psi.W).reshape(N, M)
+    return b, W
+
+
+def run_c2(args, R, HAM, F):
+    N, M, C, THERM = 20, 80, 4096, 200
+    spc = args.samples_per_chain or 1000
+    b, W = rbm_params(N, M, 1236)
+    psi = R.Psi(np.zeros(N), np.zeros(N), b, W, 1.0, False, True, precision=args.precision)
+    tab = HAM.tfim_1d(N, 1.0, 1.0)
+    op = R.Operator(*tab, True)
+    mc = R.MonteCarloLoop(C * spc, 1, THERM, C, True)
+    ev = R.ExpectationValue(True)
+    kernels = ("mc", "eval", "coef", "scalar_reduce", "ok_reduce", "partials_sum")
+    ms, E, kms = time_steps(R, ev, psi, op, mc, args.steps, args.warmup, kernels)
+    _, _, el = mc.last_samples()
+    tau, mean, se = chain_stats(el, spc, C)
+    ess = C * spc / (2 * tau) / ms
+    mc_ms, mc_n = kms["mc"]
+    mc_avg = mc_ms / max(mc_n, 1)
+    flop = F.mc_launch_flops(N, M, C, spc, THERM, 1)
+    achieved = flop / (mc_avg / 1e3) / 1e12
+    ev_ms, ev_n = kms["eval"]
+    ev_avg = ev_ms / max(ev_n, 1)
+    ev_flops = F.eval_flops(N, M, N, C * spc)
+    out = {
+        "metric": METRIC + " (plain single-flip configuration)",
+        "value": ess, "unit": "effective samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64" if args.precision == 32 else "c128",
+        "data": "synthetic (seeded N(0, 0.01^2) RBM parameters)",
+        "config": {"workload": "1-D TFIM N=20 J=h=1 periodic, RBM M=80, Metropolis single-spin-flip, "
+                               f"{C} chains x ({THERM} therm + {spc} measurement sweeps), gradient step "
+                               "(sampling + E_loc + O_k + FP64 gradient contraction)",
+                   "chains": C, "samples_per_chain": spc, "thermalization_sweeps": THERM, "N": N, "M": M,
+                   "precision": args.precision},
+        "tau_int": tau, "energy": E.real, "energy_se": se, "samples_per_s": C * spc / ms,
+        "acceptance_rate": mc.acceptance_rate,
+        "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": stored_traffic("k_mc_rbm@c2"),
+                     "kernel": "k_mc_rbm", "flop_per_launch": flop, "avg_launch_ms": mc_avg,
+                     "algorithmic_bytes_per_launch": F.mc_launch_bytes(C, spc)},
+        "eval_kernel": {"kernel": "k_eval_rbm (log psi, E_loc, my_tanh of the recorded samples)",
+                        "avg_launch_ms": ev_avg, "flop_per_launch": ev_flops,
+                        "achieved_tflops": ev_flops / (ev_avg / 1e3) / 1e12 if ev_avg > 0 else None},
+    }
+    if not args.no_cpu_baseline:
+        cps, cchains, cspc, cwall = cpu_baseline(b, W, tab, N, THERM, args.cpu_budget)
+        out["cpu_baseline"] = {
+            "value": cps / (2.0 * tau), "unit": "effective samples/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (reference algorithm: per-string O_k(s') gradient estimator), {cchains} chains x "
+                      f"({THERM} thermalisation + {cspc} measured sweeps) of the same N=20, M=80 TFIM workload in "
+                      f"{cwall:.1f} s on one core; raw {cps:.1f} samples/s divided by 2 tau_int of the GPU run"}
+    else:
+        out["cpu_baseline"] = None
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", choices=("c5", "c2"), default="c5")
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--samples-per-chain", type=int, default=SAMPLES_PER_CHAIN)
+    ap.add_argument("--samples-per-chain", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for the FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        sys.stderr.write("bench.py measures one GPU: the sampler's chains are independent, and this build runs on a "
-                         "single MI355X (multi-GPU sharding is described in DESIGN.md, not run)\n")
+        sys.stderr.write("bench.py measures one GPU: the chains are independent, and this build runs on a single "
+                         "MI355X (sharding across GPUs is a design note, not run)\n")
         sys.exit(2)
 
     import rbmongpu_amd as R
     from rbmongpu_amd import flops as F
     from rbmongpu_amd import hamiltonians as HAM
 
-    spc = args.samples_per_chain
-    b, W = make_params()
-    psi = R.Psi(np.zeros(N), np.zeros(N), b, W, 1.0, False, True, precision=args.precision)
-    op = R.Operator(*HAM.tfim_1d(N, 1.0, 1.0), True)
-    mc = R.MonteCarloLoop(CHAINS * spc, SWEEPS, THERM, CHAINS, True)
-    ev = R.ExpectationValue(True)
-    for _ in range(args.warmup):
-        ev.gradient(psi, op, mc)
-    R.synchronize()
-    R.timing_enable(True)
-    R.timing_reset()
-    R.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        grad, E = ev.gradient(psi, op, mc)
-    R.synchronize()
-    wall = time.perf_counter() - t0
-    R.timing_enable(False)
-    ms_per_step = 1e3 * wall / args.steps
-    kernel_ms = {k: R.timing_query(k) for k in ("mc", "eval", "coef", "scalar_reduce", "ok_reduce", "partials_sum")}
-    mc_ms, mc_launches = kernel_ms["mc"]
-    mc_avg_ms = mc_ms / max(mc_launches, 1)
-
-    _, _, el = mc.last_samples()
-    series = el.real.reshape(spc, CHAINS)
-    tau = tau_int(series)
-    samples = CHAINS * spc
-    ess_per_s = samples / (2.0 * tau) / (ms_per_step / 1e3)
-
-    terms = N  # N single-flip X terms; the N ZZ strings are diagonal
-    launch_flops = F.mc_launch_flops(N, M, CHAINS, spc, THERM, SWEEPS)
-    achieved = launch_flops / (mc_avg_ms / 1e3) / 1e12
-    traffic = F.mc_launch_bytes(CHAINS, spc)
-    ev_ms, ev_launches = kernel_ms["eval"]
-    ev_avg_ms = ev_ms / max(ev_launches, 1)
-    ev_flops = F.eval_flops(N, M, terms, CHAINS * spc)
-
+    out = run_c5(args, R, HAM, F) if args.config == "c5" else run_c2(args, R, HAM, F)
     if args.copy_calibration:
         copy_calibration()
-
-    traffic_meas = None
-    tpath = os.path.join(ROOT, "profiles", "mc_rbm_c2_traffic.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath))
-        if t.get("kernel_source_sha") == kernel_source_sha():
-            traffic_meas = t["bytes_per_launch"]
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cps, cchains, cspc, cwall = cpu_baseline(args.cpu_budget)
-        cpu = {"value": cps / (2.0 * tau), "unit": "effective samples/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 oracle (reference algorithm: per-string O_k(s') gradient estimator), {cchains} chains x "
-                         f"({THERM} thermalisation + {cspc} measured sweeps) of the same N=20, M=80 TFIM workload in "
-                         f"{cwall:.1f} s on one core; raw {cps:.1f} samples/s divided by 2 tau_int of the GPU run"}
-
-    out = {
-        "metric": "effective samples/sec incl. E_loc+O_k on 1 GPU (plain single-flip proposal)",
-        "value": ess_per_s,
-        "unit": "effective samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "c64" if args.precision == 32 else "c128",
-        "data": "synthetic (seeded N(0, 0.01^2) RBM parameters)",
-        "config": {"workload": "1-D TFIM N=20 J=h=1 periodic, RBM M=80, Metropolis single-spin-flip, "
-                               f"{CHAINS} chains x ({THERM} therm + {spc} measurement sweeps), gradient step "
-                               "(sampling + E_loc + O_k + FP64 gradient contraction)",
-                   "chains": CHAINS, "samples_per_chain": spc, "thermalization_sweeps": THERM,
-                   "N": N, "M": M, "precision": args.precision},
-        "tau_int": tau,
-        "samples_per_s": samples / (ms_per_step / 1e3),
-        "acceptance_rate": mc.acceptance_rate,
-        "energy": E.real,
-        "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kernel_ms.items()},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_meas,
-                     "kernel": "k_mc_rbm", "flop_per_launch": launch_flops, "avg_launch_ms": mc_avg_ms,
-                     "algorithmic_bytes_per_launch": traffic},
-        "eval_kernel": {"kernel": "k_eval_rbm (log psi, E_loc, my_tanh of the recorded samples)",
-                        "avg_launch_ms": ev_avg_ms, "flop_per_launch": ev_flops,
-                        "achieved_tflops": ev_flops / (ev_avg_ms / 1e3) / 1e12 if ev_avg_ms > 0 else None},
-        "cpu_baseline": cpu,
-    }
     print(json.dumps(out))
 
 

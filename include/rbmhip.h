@@ -197,6 +197,10 @@ int rbmhip_learned_create(rbmhip_made* made, uint32_t num_samples, uint32_t num_
  * proposal t of chain c at index t * chains + c, with its log q and log psi
  * (complex).  Call with NULL arrays to read n first. */
 int rbmhip_learned_last_proposals(const rbmhip_ens* ens, int64_t* n, uint64_t* spins, double* log_q, double* log_psi);
+/* Number of measured configurations of the last call: a rejection repeats the
+ * held configuration, and each run of equal samples is evaluated once and
+ * weighted by its length. */
+int rbmhip_learned_num_runs(const rbmhip_ens* ens, int64_t* out);
 
 /* ---- kernel timing (HIP events on the library stream) -------------------- */
 /* wait for all work on the library stream of the current device */

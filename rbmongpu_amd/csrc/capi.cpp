@@ -613,13 +613,13 @@ struct rbmhip_ens {
     int rec_device = -1;
     int64_t rec_n = 0;
     DevBuf spins, lp, el;
-    // proposals of the last learned-proposal call
-    int64_t prop_n = 0;
-    DevBuf prop_spins, prop_lq, prop_lp;
+    // proposals of the last learned-proposal call, and its runs of equal samples
+    int64_t prop_n = 0, run_n = 0;
+    DevBuf prop_spins, prop_lq, prop_lp, run_mult, sample_run;
     ~rbmhip_ens() {
         for (auto& kv : configs_dev) kv.second.release();
         spins.release(); lp.release(); el.release();
-        prop_spins.release(); prop_lq.release(); prop_lp.release();
+        prop_spins.release(); prop_lq.release(); prop_lp.release(); run_mult.release(); sample_run.release();
     }
     int64_t num_steps() const { return kind == 0 ? num_configs : (int64_t)num_samples; }
 };
@@ -643,6 +643,7 @@ struct Samples {
     cx<double>* lp = nullptr;
     cx<double>* el = nullptr;
     void* tanh = nullptr;
+    const double* mult = nullptr;   // per-configuration multiplicity (runs of a learned-proposal chain)
 };
 
 template <typename T>
@@ -728,10 +729,32 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
                 Records prec{nullptr, plp, nullptr, nullptr};
                 eval_model<T>(c, psi, nullptr, P, prop, 0, prec, "proposal_eval");
             }
+            // accept scan: a rejection repeats the held configuration, so each
+            // chain's samples are runs of equal configurations; every run is
+            // measured once and weighted by its length
+            uint32_t* runs = reinterpret_cast<uint32_t*>(c.buf("imh_runs", (size_t)mp.chains * sizeof(uint32_t)));
             c.launch("imh", [&] {
-                return launch_imh(mp.chains, mp.therm, mp.steps_per_chain, prop, plp, plq, mp.seed_lo, mp.seed_hi, mp.call,
-                                  s.spins, counts, c.stream);
+                return launch_imh(mp.chains, mp.therm, mp.steps_per_chain, prop, plp, plq, mp.seed_lo, mp.seed_hi, mp.call, 0,
+                                  runs, nullptr, nullptr, nullptr, nullptr, counts, c.stream);
             });
+            std::vector<uint32_t> hr((size_t)mp.chains);
+            c.d2h(hr.data(), runs, hr.size() * sizeof(uint32_t));
+            c.sync();
+            std::vector<int64_t> off((size_t)mp.chains);
+            int64_t U = 0;
+            for (int i = 0; i < mp.chains; i++) { off[(size_t)i] = U; U += hr[(size_t)i]; }
+            int64_t* doff = reinterpret_cast<int64_t*>(c.buf("imh_off", off.size() * sizeof(int64_t)));
+            c.h2d(doff, off.data(), off.size() * sizeof(int64_t));
+            double* mult = reinterpret_cast<double*>(e->run_mult.ensure((size_t)U * sizeof(double)));
+            int64_t* srun = reinterpret_cast<int64_t*>(e->sample_run.ensure((size_t)s.n * sizeof(int64_t)));
+            c.launch("imh", [&] {
+                return launch_imh(mp.chains, mp.therm, mp.steps_per_chain, prop, plp, plq, mp.seed_lo, mp.seed_hi, mp.call, 1,
+                                  nullptr, doff, s.spins, mult, srun, nullptr, c.stream);
+            });
+            e->run_n = U;
+            e->rec_n = U;
+            s.n = U;
+            s.mult = mult;
             Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
             eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
         } else if (psi->kind == 0) {
@@ -874,7 +897,7 @@ static Scalars energy_scalars(Ctx& c, rbmhip_psi* psi, const Samples& s, rbmhip_
     double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
     CoefArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.n = s.n; a.mode = 0; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs;
+    a.n = s.n; a.mode = 0; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs; a.mult = s.mult;
     a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.el = s.el; a.coef = coef; a.sc = sc;
     c.launch("coef", [&] { return launch_coef(a, c.stream); });
     return reduce_scalars(c, s.n, sc);
@@ -1168,6 +1191,25 @@ int rbmhip_mc_last_samples(const rbmhip_ens* ens, uint64_t* spins, double* log_p
         RBM_REQUIRE(ens->kind != 0 && ens->rec_n > 0, "no Monte Carlo samples recorded yet");
         Ctx& c = ctx();
         RBM_REQUIRE(c.device == ens->rec_device, "samples were recorded on another device");
+        if (ens->kind == 2) {
+            // one record per run: expand to one row per sample
+            const int64_t U = ens->run_n, n = ens->num_steps();
+            std::vector<int64_t> run((size_t)n);
+            std::vector<uint64_t> rs((size_t)U * 2);
+            std::vector<cx<double>> rl((size_t)U), re((size_t)U);
+            c.d2h(run.data(), ens->sample_run.p, run.size() * sizeof(int64_t));
+            c.d2h(rs.data(), ens->spins.p, rs.size() * sizeof(uint64_t));
+            c.d2h(rl.data(), ens->lp.p, rl.size() * sizeof(cx<double>));
+            c.d2h(re.data(), ens->el.p, re.size() * sizeof(cx<double>));
+            c.sync();
+            for (int64_t k = 0; k < n; k++) {
+                const int64_t r = run[(size_t)k];
+                if (spins) { spins[2 * k] = rs[2 * r]; spins[2 * k + 1] = rs[2 * r + 1]; }
+                if (log_psi) { log_psi[2 * k] = rl[r].re; log_psi[2 * k + 1] = rl[r].im; }
+                if (e_loc) { e_loc[2 * k] = re[r].re; e_loc[2 * k + 1] = re[r].im; }
+            }
+            return;
+        }
         if (spins) c.d2h(spins, ens->spins.p, (size_t)ens->rec_n * 2 * sizeof(uint64_t));
         if (log_psi) c.d2h(log_psi, ens->lp.p, (size_t)ens->rec_n * sizeof(cx<double>));
         if (e_loc) c.d2h(e_loc, ens->el.p, (size_t)ens->rec_n * sizeof(cx<double>));
@@ -1392,7 +1434,7 @@ int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* gra
             cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * sizeof(cx<double>)));
             CoefArgs a;
             std::memset(&a, 0, sizeof(a));
-            a.n = s.n; a.mode = 2; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs;
+            a.n = s.n; a.mode = 2; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs; a.mult = s.mult;
             a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.el = s.el; a.coef = coef;
             a.e_mean = mk(E.real(), E.imag());
             c.launch("coef", [&] { return launch_coef(a, c.stream); });
@@ -1460,7 +1502,7 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
         double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
         CoefArgs a;
         std::memset(&a, 0, sizeof(a));
-        a.n = s.n; a.mode = 1; a.exact = ens->kind == 0; a.is_unitary = is_unitary ? 1 : 0;
+        a.n = s.n; a.mode = 1; a.exact = ens->kind == 0; a.is_unitary = is_unitary ? 1 : 0; a.mult = s.mult;
         a.num_configs = (double)ens->num_configs; a.log_prefactor = std::log(psi->prefactor);
         a.lp = s.lp; a.el = s.el; a.lp2 = lp2; a.sc = sc;
         c.launch("coef", [&] { return launch_coef(a, c.stream); });
@@ -1710,6 +1752,14 @@ int rbmhip_learned_last_proposals(const rbmhip_ens* ens, int64_t* n, uint64_t* s
         if (log_q) c.d2h(log_q, ens->prop_lq.p, (size_t)ens->prop_n * sizeof(double));
         if (log_psi) c.d2h(log_psi, ens->prop_lp.p, (size_t)ens->prop_n * sizeof(cx<double>));
         c.sync();
+    });
+}
+
+int rbmhip_learned_num_runs(const rbmhip_ens* ens, int64_t* out) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 2 && out, "bad arguments");
+        *out = ens->run_n;
     });
 }
 

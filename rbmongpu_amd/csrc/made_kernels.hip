@@ -417,47 +417,73 @@ hipError_t launch_made_log_q(const MadeInf& m, int64_t n, const uint64_t* spins,
 }
 
 // ---------------------------------------------------------------------------
-// Independent Metropolis-Hastings, one thread per chain.
+// Independent Metropolis-Hastings, one thread per chain.  A rejected proposal
+// repeats the held configuration, so the measured samples of a chain form runs
+// of equal configurations.  Pass 0 counts the runs of each chain; pass 1 (with
+// run offsets) writes each run's configuration once with its length as the
+// multiplicity, and the run of every sample.  Both passes replay the same
+// decisions.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_imh(int chains, int therm, int steps, const uint64_t* __restrict__ prop,
                                              const cx<double>* __restrict__ lp, const double* __restrict__ lq,
-                                             uint32_t seed_lo, uint32_t seed_hi, uint32_t call,
-                                             uint64_t* __restrict__ out, uint32_t* __restrict__ counts) {
+                                             uint32_t seed_lo, uint32_t seed_hi, uint32_t call, int pass,
+                                             uint32_t* __restrict__ runs, const int64_t* __restrict__ run_offset,
+                                             uint64_t* __restrict__ run_spins, double* __restrict__ run_mult,
+                                             int64_t* __restrict__ sample_run, uint32_t* __restrict__ counts) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= chains) return;
     int64_t cur = c;
     double lpc = lp[cur].re, lqc = lq[cur];
-    uint32_t acc = 0;
+    uint32_t acc = 0, nrun = 0, len = 0;
     const int total = therm + steps;
+    const int64_t off = pass ? run_offset[c] : 0;
     for (int t = 1; t <= total; t++) {
         const int64_t g = (int64_t)t * chains + c;
         const double lpg = lp[g].re, lqg = lq[g];
         const double r = 2.0 * (lpg - lpc) + lqc - lqg;
         const u32x4 R = philox((uint32_t)c, (uint32_t)t, call, 2u, seed_lo, seed_hi);
         const double u = ((double)R.x + 0.5) * (1.0 / 4294967296.0);
-        if (u <= exp(r)) {
+        const bool accept = u <= exp(r);
+        if (accept) {
             cur = g;
             lpc = lpg;
             lqc = lqg;
             acc++;
         }
         if (t > therm) {
-            const int64_t k = (int64_t)(t - therm - 1) * chains + c;
-            out[2 * k] = prop[2 * cur];
-            out[2 * k + 1] = prop[2 * cur + 1];
+            const int k = t - therm - 1;
+            if (k == 0 || accept) {
+                // a new run starts: close the previous one
+                if (pass && nrun > 0) run_mult[off + nrun - 1] = (double)len;
+                if (pass) {
+                    run_spins[2 * (off + nrun)] = prop[2 * cur];
+                    run_spins[2 * (off + nrun) + 1] = prop[2 * cur + 1];
+                }
+                nrun++;
+                len = 0;
+            }
+            len++;
+            if (pass) sample_run[(int64_t)k * chains + c] = off + nrun - 1;
         }
     }
-    if (counts) {
-        counts[2 * c] = acc;
-        counts[2 * c + 1] = (uint32_t)total - acc;
+    if (pass) {
+        if (nrun > 0) run_mult[off + nrun - 1] = (double)len;
+    } else {
+        runs[c] = nrun;
+        if (counts) {
+            counts[2 * c] = acc;
+            counts[2 * c + 1] = (uint32_t)total - acc;
+        }
     }
 }
 
 hipError_t launch_imh(int chains, int therm, int steps, const uint64_t* prop_spins, const cx<double>* log_psi,
-                      const double* log_q, uint32_t seed_lo, uint32_t seed_hi, uint32_t call, uint64_t* out_spins,
+                      const double* log_q, uint32_t seed_lo, uint32_t seed_hi, uint32_t call, int pass, uint32_t* runs,
+                      const int64_t* run_offset, uint64_t* run_spins, double* run_mult, int64_t* sample_run,
                       uint32_t* counts, hipStream_t stream) {
     hipLaunchKernelGGL(k_imh, dim3((chains + 255) / 256), dim3(256), 0, stream, chains, therm, steps, prop_spins,
-                       log_psi, log_q, seed_lo, seed_hi, call, out_spins, counts);
+                       log_psi, log_q, seed_lo, seed_hi, call, pass, runs, run_offset, run_spins, run_mult, sample_run,
+                       counts);
     return hipGetLastError();
 }
 

@@ -86,10 +86,16 @@ hipError_t launch_made_log_q(const MadeInf& m, int64_t n, const uint64_t* spins,
 // Independent Metropolis-Hastings over precomputed proposals: chain c uses
 // proposals t * chains + c for t = 0 .. therm + steps; proposal 0 starts the
 // chain and t >= 1 is accepted iff u <= exp(2 (Re lp' - Re lp) + lq - lq'),
-// u from Philox(seed; ctr = (c, t, call, 2)) in FP64.  The configuration held
-// after measured proposal therm + k is written to sample k * chains + c.
+// u from Philox(seed; ctr = (c, t, call, 2)) in FP64.  Measured sample k of
+// chain c is the configuration held after proposal therm + 1 + k.
+// pass 0: runs[c] = number of runs of equal consecutive samples, counts[2c],
+//         counts[2c + 1] = acceptances, rejections;
+// pass 1: with run_offset[c] (exclusive prefix of runs), run r of chain c is
+//         written at run_offset[c] + r: its configuration and its length
+//         (run_mult); sample_run[k * chains + c] = the run holding sample k.
 hipError_t launch_imh(int chains, int therm, int steps, const uint64_t* prop_spins, const cx<double>* log_psi,
-                      const double* log_q, uint32_t seed_lo, uint32_t seed_hi, uint32_t call, uint64_t* out_spins,
+                      const double* log_q, uint32_t seed_lo, uint32_t seed_hi, uint32_t call, int pass, uint32_t* runs,
+                      const int64_t* run_offset, uint64_t* run_spins, double* run_mult, int64_t* sample_run,
                       uint32_t* counts, hipStream_t stream);
 
 }  // namespace rbmhip

@@ -741,6 +741,11 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
             check(rbmhip_mc_acceptance(e.h, &a, &r));
             return (float)a / (float)(a + r);
         })
+        .def_property_readonly("num_runs", [](const LearnedMonteCarloLoop& e) {
+            int64_t n = 0;
+            check(rbmhip_learned_num_runs(e.h, &n));
+            return n;
+        })
         .def("last_proposals", [](const LearnedMonteCarloLoop& e) {
             int64_t n = 0;
             check(rbmhip_learned_last_proposals(e.h, &n, nullptr, nullptr, nullptr));

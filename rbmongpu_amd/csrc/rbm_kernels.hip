@@ -335,7 +335,8 @@ __global__ __launch_bounds__(256) void k_eval_rbm(RbmDev<T> md, OpDev op, int ha
 
 // ---------------------------------------------------------------------------
 // Per-sample coefficients and scalar records (FP64).
-//   weight w = exact ? num_configs * exp(2 (log prefactor + Re log psi)) : 1
+//   weight w = mult[s] for runs of equal samples, else
+//            exact ? num_configs * exp(2 (log prefactor + Re log psi)) : 1
 //     (ExactSummation.hpp:75, Psi.hpp:133-136)
 //   mode 0 (energy):   coef0 = w, coef1 = w E;  sc = {w, w E, w |E|^2}
 //   mode 1 (distance): HilbertSpaceDistance.cu:83-105 with log psi' in lp2:
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(256) void k_eval_rbm(RbmDev<T> md, OpDev op, int ha
 __global__ __launch_bounds__(256) void k_coef(CoefArgs a) {
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)gridDim.x * blockDim.x) {
         const cx<double> lp = a.lp[s];
-        const double w = a.exact ? a.num_configs * exp(2.0 * (a.log_prefactor + lp.re)) : 1.0;
+        const double w = a.mult ? a.mult[s] : a.exact ? a.num_configs * exp(2.0 * (a.log_prefactor + lp.re)) : 1.0;
         const cx<double> E = a.el ? a.el[s] : mk(0.0, 0.0);
         if (a.mode == 0 || a.mode == 2) {
             if (a.mode == 2) {

@@ -42,6 +42,7 @@ struct CoefArgs {
     int64_t n;
     int mode;            // 0 energy, 1 Hilbert-space distance, 2 centred energy, 3 centred distance
     int exact;           // ExactSummation weights
+    const double* mult;  // per-configuration multiplicity weights (runs of equal samples), may be null
     int is_unitary;
     double num_configs;
     double log_prefactor;
