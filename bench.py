@@ -102,6 +102,11 @@ def stored_traffic(kernel_key):
     return t.get("bytes_per_launch", {}).get(kernel_key)
 
 
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
 def rbm_params(N, M, seed):
     rng = np.random.default_rng(seed)
     b = 0.01 * (rng.normal(size=M) + 1j * rng.normal(size=M))
@@ -166,6 +171,7 @@ def run_c5(args, R, HAM, F):
     op = R.Operator(*tab, True)
     ev = R.ExpectationValue(True)
     # 50 Adam steps on the energy gradient (b and W; the first 2N entries are alpha, beta)
+    log("c5: pre-training the RBM (50 Adam steps)")
     t0 = time.perf_counter()
     mc = R.MonteCarloLoop(C * 10, 1, 20, C, True, seed=11)
     p = psi.params
@@ -182,6 +188,7 @@ def run_c5(args, R, HAM, F):
     pretrain_s = time.perf_counter() - t0
 
     kernels = ("mc", "eval", "coef", "scalar_reduce", "ok_reduce", "partials_sum", "made_sample", "proposal_eval", "imh")
+    log("c5: plain single-flip steps")
     plain = R.MonteCarloLoop(C * spc, 1, 200, C, True, seed=21)
     p_ms, p_E, p_k = time_steps(R, ev, psi, op, plain, args.steps, args.warmup, kernels)
     _, _, el = plain.last_samples()
@@ -189,16 +196,27 @@ def run_c5(args, R, HAM, F):
     p_ess = C * spc / (2 * p_tau) / p_ms
 
     # 10^6 training samples from the plain chain, then the proposal network
+    log("c5: training data and proposal network")
     data_mc = R.MonteCarloLoop(C * 123, 1, 200, C, True, seed=31)
     ev(psi, op, data_mc)
     data, _, _ = data_mc.last_samples()
-    made = R.MADE(N, H, seed=5)
-    R.synchronize()
-    t0 = time.perf_counter()
-    losses = made.train(data, 2000, batch=4096, lr=1e-3, seed=6)
-    R.synchronize()
-    made_s = time.perf_counter() - t0
+    log(f"c5: {data.shape[0]} training samples")
+    if args.made_params:
+        # a network trained by an earlier run of this workload (profiling runs)
+        made = R.MADE(N, H, params=np.load(args.made_params))
+        made_s, losses = None, [float("nan"), float("nan")]
+    else:
+        made = R.MADE(N, H, seed=5)
+        R.synchronize()
+        log("c5: training the proposal network")
+        t0 = time.perf_counter()
+        losses = made.train(data, 2000, batch=4096, lr=1e-3, seed=6)
+        R.synchronize()
+        made_s = time.perf_counter() - t0
+        if args.save_made_params:
+            np.save(args.save_made_params, made.params)
 
+    log("c5: learned-proposal steps")
     learned = R.LearnedMonteCarloLoop(made, C * spc, 20, C, True, seed=41)
     l_ms, l_E, l_k = time_steps(R, ev, psi, op, learned, args.steps, args.warmup, kernels)
     _, _, el = learned.last_samples()
@@ -243,6 +261,7 @@ def run_c5(args, R, HAM, F):
                      "algorithmic_bytes_per_launch": F.eval_bytes(M, runs)},
     }
     if not args.no_cpu_baseline:
+        log("c5: CPU baseline (FP64 oracle, one core)")
         cps, cchains, cspc, cwall = cpu_baseline(*R_params_of(psi, N, M), tab, N, 200, args.cpu_budget)
         out["cpu_baseline"] = {
             "value": cps / (2.0 * p_tau), "unit": "effective samples/s", "cores": 1, "kind": "port",
@@ -326,6 +345,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--samples-per-chain", type=int, default=0)
+    ap.add_argument("--made-params", default="", help="c5: load the proposal network (.npy) instead of training it")
+    ap.add_argument("--save-made-params", default="", help="c5: save the trained proposal network (.npy)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for the FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()

@@ -575,10 +575,16 @@ static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const 
     e.bias = P + md->off_b2; e.aux = X; e.ld_aux = Np; e.n_live = N; e.m_live = B; e.inv_batch = 1.0f / (float)B;
     e.partials = partials;
     c.launch("made_gemm", [&] { return launch_gemm(EPI_LOGITS, Bp, Np, Hp, Hh, Hp, 1, W2, 1, Hp, dL, Np, e, c.stream); });
+    // weight gradients: small outputs with the batch as K, split over workgroups
+    int splits = 1;
+    while (splits < 64 && Bp % (16 * splits * 2) == 0 && Bp / (splits * 2) >= 64) splits *= 2;
+    float* scratch = reinterpret_cast<float*>(c.buf("made_splitk", (size_t)splits * Np * Hp * sizeof(float)));
     // dW2 = dl^T h (masked)
     e = GemmEpiArgs{};
     e.m_live = N; e.n_live = H;
-    c.launch("made_gemm", [&] { return launch_gemm(EPI_MASK_W2, Np, Hp, Bp, dL, 1, Np, Hh, Hp, 1, G + md->off_W2, Hp, e, c.stream); });
+    c.launch("made_gemm", [&] {
+        return launch_gemm_splitk(EPI_MASK_W2, Np, Hp, Bp, dL, 1, Np, Hh, Hp, 1, G + md->off_W2, Hp, e, splits, scratch, c.stream);
+    });
     // dh = (dl W2) * [a > 0]
     e = GemmEpiArgs{};
     e.aux = Apre; e.ld_aux = Hp;
@@ -586,7 +592,9 @@ static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const 
     // dW1 = dh^T X (masked)
     e = GemmEpiArgs{};
     e.m_live = H; e.n_live = N;
-    c.launch("made_gemm", [&] { return launch_gemm(EPI_MASK_W1, Hp, Np, Bp, dH, 1, Hp, X, Np, 1, G, Np, e, c.stream); });
+    c.launch("made_gemm", [&] {
+        return launch_gemm_splitk(EPI_MASK_W1, Hp, Np, Bp, dH, 1, Hp, X, Np, 1, G, Np, e, splits, scratch, c.stream);
+    });
     c.launch("made_colsum", [&] { return launch_colsum(Bp, Hp, H, dH, G + md->off_b1, c.stream); });
     c.launch("made_colsum", [&] { return launch_colsum(Bp, Np, N, dL, G + md->off_b2, c.stream); });
 }
@@ -1587,9 +1595,9 @@ int rbmhip_made_create(int N, int H, const float* params, uint64_t seed, rbmhip_
         const size_t bytes = (size_t)md->padded * sizeof(float);
         md->params.ensure(bytes); md->grad.ensure(bytes); md->m.ensure(bytes); md->v.ensure(bytes);
         c.h2d(md->params.p, dev.data(), bytes);
-        RBM_HIPCHK(hipMemsetAsync(md->grad.p, 0, bytes, c.stream));
-        RBM_HIPCHK(hipMemsetAsync(md->m.p, 0, bytes, c.stream));
-        RBM_HIPCHK(hipMemsetAsync(md->v.p, 0, bytes, c.stream));
+        RBM_HIPCHK(launch_zero(md->padded, reinterpret_cast<float*>(md->grad.p), c.stream));
+        RBM_HIPCHK(launch_zero(md->padded, reinterpret_cast<float*>(md->m.p), c.stream));
+        RBM_HIPCHK(launch_zero(md->padded, reinterpret_cast<float*>(md->v.p), c.stream));
         c.sync();
         *out = md.release();
     });
@@ -1629,9 +1637,8 @@ int rbmhip_made_reset_optimizer(rbmhip_made* made) {
         RBM_REQUIRE(made, "null handle");
         Ctx& c = ctx();
         made_check_device(c, made);
-        const size_t bytes = (size_t)made->padded * sizeof(float);
-        RBM_HIPCHK(hipMemsetAsync(made->m.p, 0, bytes, c.stream));
-        RBM_HIPCHK(hipMemsetAsync(made->v.p, 0, bytes, c.stream));
+        RBM_HIPCHK(launch_zero(made->padded, reinterpret_cast<float*>(made->m.p), c.stream));
+        RBM_HIPCHK(launch_zero(made->padded, reinterpret_cast<float*>(made->v.p), c.stream));
         c.sync();
         made->t = 0;
     });

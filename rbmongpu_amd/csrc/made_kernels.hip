@@ -35,12 +35,17 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    // split K: workgroup z covers [z K / gridDim.z, (z + 1) K / gridDim.z) and
+    // writes its partial product at C + z M ldc
+    const int kspan = K / gridDim.z;
+    const int kbeg = blockIdx.z * kspan, kend = kbeg + kspan;
+    C += (long)blockIdx.z * M * ldc;
     floatx4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += GK) {
+    for (int k0 = kbeg; k0 < kend; k0 += GK) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int idx = tid + 256 * r;
@@ -111,6 +116,41 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
     }
 }
 
+// Split-K reduction: C = epilogue(sum over the splits of the partial products).
+template <int EPI>
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int splits, const float* __restrict__ part,
+                                                       float* __restrict__ C, long ldc, GemmEpiArgs e) {
+    const int64_t total = (int64_t)M * N;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int m = (int)(t / N), n = (int)(t % N);
+        float x = 0.0f;
+        for (int z = 0; z < splits; z++) x += part[(int64_t)z * total + t];
+        bool ok = true;
+        if (EPI == EPI_MASK_W2) ok = m < e.m_live && n < e.n_live && made_degree(n, e.m_live) <= m;
+        if (EPI == EPI_MASK_W1) ok = m < e.m_live && n < e.n_live && n < made_degree(m, e.n_live);
+        C[(long)m * ldc + n] = ok ? x : 0.0f;
+    }
+}
+
+hipError_t launch_gemm_splitk(int epi, int M, int N, int K, const float* A, long sam, long sak, const float* B,
+                              long sbk, long sbn, float* C, long ldc, const GemmEpiArgs& e, int splits, float* scratch,
+                              hipStream_t stream) {
+    if (M % GT || N % GT || M <= 0 || N <= 0 || K <= 0 || splits < 1 || K % (GK * splits)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_gemm<EPI_STORE>), dim3(N / GT, M / GT, splits), dim3(256), 0, stream, M, N, K, A, sam, sak, B,
+                       sbk, sbn, scratch, (long)N, e);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    const int64_t total = (int64_t)M * N;
+    const unsigned blocks = (unsigned)((total + 255) / 256 < 2048 ? (total + 255) / 256 : 2048);
+    switch (epi) {
+        case EPI_STORE: hipLaunchKernelGGL((k_splitk_reduce<EPI_STORE>), dim3(blocks), dim3(256), 0, stream, M, N, splits, scratch, C, ldc, e); break;
+        case EPI_MASK_W1: hipLaunchKernelGGL((k_splitk_reduce<EPI_MASK_W1>), dim3(blocks), dim3(256), 0, stream, M, N, splits, scratch, C, ldc, e); break;
+        case EPI_MASK_W2: hipLaunchKernelGGL((k_splitk_reduce<EPI_MASK_W2>), dim3(blocks), dim3(256), 0, stream, M, N, splits, scratch, C, ldc, e); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_gemm(int epi, int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk,
                        long sbn, float* C, long ldc, const GemmEpiArgs& e, hipStream_t stream) {
     if (M % GT || N % GT || K % GK || M <= 0 || N <= 0 || K <= 0) return hipErrorInvalidValue;
@@ -175,19 +215,39 @@ hipError_t launch_made_batch_index(int B, int64_t n, uint32_t step, uint32_t see
     return hipGetLastError();
 }
 
-// one thread per column; rows summed in order
-__global__ __launch_bounds__(256) void k_colsum(int rows, int cols, int n_live, const float* __restrict__ G,
-                                                float* __restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// Column sums: a 1024-thread workgroup owns 64 columns; 16 row groups stride
+// over the rows (coalesced 64-wide reads) and are added in a fixed order.
+__global__ __launch_bounds__(1024) void k_colsum(int rows, int cols, int n_live, const float* __restrict__ G,
+                                                 float* __restrict__ out) {
+    __shared__ float red[16][64];
+    const int cl = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cl;
     float s = 0.0f;
-    if (c < n_live)
-        for (int r = 0; r < rows; r++) s += G[(long)r * cols + c];
-    out[c] = s;
+    if (c < cols && c < n_live)
+        for (int r = grp; r < rows; r += 16) s += G[(long)r * cols + c];
+    red[grp][cl] = s;
+    __syncthreads();
+    if (grp == 0 && c < cols) {
+        float t = 0.0f;
+        for (int g = 0; g < 16; g++) t += red[g][cl];
+        out[c] = t;
+    }
 }
 
 hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, hipStream_t stream) {
-    hipLaunchKernelGGL(k_colsum, dim3((cols + 255) / 256), dim3(256), 0, stream, rows, cols, n_live, G, out);
+    hipLaunchKernelGGL(k_colsum, dim3((cols + 63) / 64), dim3(1024), 0, stream, rows, cols, n_live, G, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_zero(int64_t n, float* __restrict__ p) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = 0.0f;
+}
+
+hipError_t launch_zero(int64_t n, float* p, hipStream_t stream) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, stream, n, p);
     return hipGetLastError();
 }
 

@@ -50,6 +50,15 @@ struct GemmEpiArgs {
 hipError_t launch_gemm(int epi, int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk,
                        long sbn, float* C, long ldc, const GemmEpiArgs& e, hipStream_t stream);
 
+// The same product with K split over `splits` workgroups (K a multiple of
+// 16 * splits): partial products go to scratch (splits x M x N floats) and a
+// second kernel adds them in order and applies EPI_STORE, EPI_MASK_W1 or
+// EPI_MASK_W2.  For the weight gradients, whose M x N is small and K (the batch)
+// long.
+hipError_t launch_gemm_splitk(int epi, int M, int N, int K, const float* A, long sam, long sak, const float* B,
+                              long sbk, long sbn, float* C, long ldc, const GemmEpiArgs& e, int splits, float* scratch,
+                              hipStream_t stream);
+
 // X[s][i] = bit i of spins[idx[s]] (or of spins[s] without idx) for i < N, 0 for the padding
 // and for rows s >= n_live.
 hipError_t launch_made_gather(int64_t rows, int n_live, int N, int Np, const uint64_t* spins, const int64_t* idx,
@@ -59,6 +68,9 @@ hipError_t launch_made_batch_index(int B, int64_t n, uint32_t step, uint32_t see
                                    hipStream_t stream);
 // out[n] = sum_m G[m][n] over rows m < rows (ld = cols), fixed order
 hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, hipStream_t stream);
+// p[0 .. n) = 0 (a fill kernel, so the optimiser state can be cleared inside
+// captured work)
+hipError_t launch_zero(int64_t n, float* p, hipStream_t stream);
 // Adam (Kingma & Ba 2015) over n parameters; t is the 1-based step
 hipError_t launch_adam(int64_t n, float* p, const float* g, float* m, float* v, int t, float lr, float b1, float b2,
                        float eps, hipStream_t stream);
