@@ -498,14 +498,24 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // RT x CT output tiles per wavefront, fixed at compile time so the K loop has
 // no branches and the next K-step's loads overlap the current MFMAs.  Rows
 // past 2(N+1) and columns past 2(M+1) are padding and contribute zero.
+//
+// Workgroups are numbered so that all tile groups of one sample chunk land on
+// the same XCD (workgroup b runs on XCD b mod 8) and start together: they read
+// the same my_tanh records at about the same time, so the repeats come from
+// that XCD's L2 instead of HBM.
 template <int RT, int CT>
 __global__ __launch_bounds__(64) void k_ok_mfma(int64_t n, int N, int M, const float* __restrict__ Tf,
                                                 const uint64_t* __restrict__ spins,
-                                                const cx<double>* __restrict__ coef, int64_t chunk, int rows_p,
-                                                int cols_p, float* __restrict__ partials) {
+                                                const cx<double>* __restrict__ coef, int64_t chunk, int nchunks,
+                                                int ry, int cz, int rows_p, int cols_p, float* __restrict__ partials) {
     const int lane = threadIdx.x;
     const int kk = lane >> 4, ii = lane & 15;
-    const int rt0 = blockIdx.y * RT, ct0 = blockIdx.z * CT;
+    const int G = ry * cz;
+    const int slot = blockIdx.x >> 3;
+    const int chunk_id = (slot / G) * 8 + (blockIdx.x & 7);
+    const int tile = slot % G;
+    if (chunk_id >= nchunks) return;
+    const int rt0 = (tile % ry) * RT, ct0 = (tile / ry) * CT;
     floatx4 acc[RT][CT];
 #pragma unroll
     for (int a = 0; a < RT; a++)
@@ -522,7 +532,7 @@ __global__ __launch_bounds__(64) void k_ok_mfma(int64_t n, int N, int M, const f
         cload[b] = j < M ? 1.f : 0.f;                    // take the loaded value
         cfix[b] = (j == M && (q & 1) == 0) ? 1.f : 0.f;  // the constant column
     }
-    const int64_t s_begin = (int64_t)blockIdx.x * chunk;
+    const int64_t s_begin = (int64_t)chunk_id * chunk;
     const int64_t s_end = s_begin + chunk < n ? s_begin + chunk : n;
     const int twoM = 2 * M;
 #pragma unroll 2
@@ -549,7 +559,7 @@ __global__ __launch_bounds__(64) void k_ok_mfma(int64_t n, int N, int M, const f
 #pragma unroll
             for (int b = 0; b < CT; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    float* out = partials + (int64_t)blockIdx.x * rows_p * cols_p;
+    float* out = partials + (int64_t)chunk_id * rows_p * cols_p;
 #pragma unroll
     for (int a = 0; a < RT; a++)
 #pragma unroll
@@ -614,13 +624,14 @@ hipError_t launch_ok_mfma(int64_t n, int N, int M, const void* Trec, const uint6
     int RT, CT, rp, cp;
     mfma_tiling(N, M, RT, CT, rp, cp);
     if (rp != rows_p || cp != cols_p) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)nchunks, (unsigned)(rows_p / (16 * RT)), (unsigned)(cols_p / (16 * CT)));
+    const int ry = rows_p / (16 * RT), cz = cols_p / (16 * CT);
+    const dim3 grid((unsigned)(8 * ((nchunks + 7) / 8) * ry * cz));
     const float* Tf = reinterpret_cast<const float*>(Trec);
-#define RBM_MF_CASE(R_, C_)                                                                                       \
-    if (RT == R_ && CT == C_) {                                                                                   \
-        hipLaunchKernelGGL((k_ok_mfma<R_, C_>), grid, dim3(64), 0, stream, n, N, M, Tf, spins, coef, chunk, rows_p, \
-                           cols_p, partials);                                                                     \
-        return hipGetLastError();                                                                                 \
+#define RBM_MF_CASE(R_, C_)                                                                                        \
+    if (RT == R_ && CT == C_) {                                                                                    \
+        hipLaunchKernelGGL((k_ok_mfma<R_, C_>), grid, dim3(64), 0, stream, n, N, M, Tf, spins, coef, chunk, nchunks, \
+                           ry, cz, rows_p, cols_p, partials);                                                      \
+        return hipGetLastError();                                                                                  \
     }
 #define RBM_MF_ROW(R_) RBM_MF_CASE(R_, 4) RBM_MF_CASE(R_, 8) RBM_MF_CASE(R_, 11) RBM_MF_CASE(R_, 13)
     RBM_MF_ROW(1) RBM_MF_ROW(2) RBM_MF_ROW(3) RBM_MF_ROW(4)
