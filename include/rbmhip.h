@@ -148,6 +148,22 @@ int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* gra
 /* get_S_matrix(Psi, ExactSummation): source/network_functions/S_matrix.cu:10-58
  * (num_params x num_params complex, unweighted and uncentred as in the reference) */
 int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out);
+/* Centred, weighted S-matrix S_kl = <conj(O_k - <O_k>) (O_l - <O_l>)> over the
+ * parameters that have an O_k (the num_params - 2N entries of `params` after
+ * alpha and beta), P x P complex, FP32 MFMA products accumulated in FP64.  No
+ * reference counterpart: the reference's S (get_S_matrix above) is unweighted
+ * and uncentred and feeds no solver. */
+int rbmhip_s_matrix_centred(rbmhip_psi* psi, rbmhip_ens* ens, double* out);
+/* x = A^{-1} b for an n x n complex Hermitian positive-definite A (row-major,
+ * lower triangle used), FP64 Cholesky on the GPU.  RBMHIP_ERR_NUMERIC if A is
+ * not positive definite.  No reference counterpart. */
+int rbmhip_cholesky_solve(int n, const double* A, const double* b, double* x);
+/* Stochastic-reconfiguration step: update = (S + eps I)^{-1} g with S the
+ * centred S-matrix and g the gradient of rbmhip_gradient, from one pass of the
+ * ensemble; update has num_params complex values (zero for alpha and beta).
+ * No reference counterpart (the reference's driver, pyRBMonGPU's
+ * gradient_descent.py, has no S solve). */
+int rbmhip_sr_update(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double eps, double* update, double* expectation);
 /* HilbertSpaceDistance::distance / gradient (free_quantum_axis = false):
  * source/network_functions/HilbertSpaceDistance.cu:193-269.  grad (num_params
  * of psi_prime) is written when want_grad != 0. */

@@ -24,6 +24,7 @@ try:
         HilbertSpaceDistance, psi_O_k_vector, get_S_matrix, activation_function, activation_derivative,
         setDevice, start_profiling, stop_profiling, device_count, timing_enable, timing_reset, timing_query,
         log_psi_batch, local_energy_batch, synchronize, MADE, LearnedMonteCarloLoop,
+        get_S_matrix_centred, sr_update, cholesky_solve,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(

@@ -24,6 +24,7 @@
 #include "rbm_kernels.hpp"
 #include "deep_kernels.hpp"
 #include "made_kernels.hpp"
+#include "linalg_kernels.hpp"
 
 using cd = std::complex<double>;
 
@@ -914,6 +915,63 @@ static Scalars energy_scalars(Ctx& c, rbmhip_psi* psi, const Samples& s, rbmhip_
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// Centred, weighted S over the parameters with an O_k (the `params` entries
+// after alpha and beta), FP32 MFMA Gram blocks accumulated in FP64; returns a
+// device array of P x P complex128 (eps added on the diagonal).
+template <typename T>
+static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, const Samples& s, double eps) {
+    RBM_REQUIRE(psi->kind == 0, "the centred S-matrix is built for Psi");
+    const int N = psi->N, M = psi->M;
+    const int P = (int)(psi->num_params() - 2 * (int64_t)N);
+    const double n = (double)ens->num_steps();
+    cx<double>* w = reinterpret_cast<cx<double>*>(c.buf("s_w", (size_t)s.n * sizeof(cx<double>)));
+    CoefArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = s.n; a.mode = 4; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs; a.mult = s.mult;
+    a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.coef = w;
+    c.launch("coef", [&] { return launch_coef(a, c.stream); });
+    const std::vector<cd> osum = contract_params<T>(c, psi, s.n, s.tanh, s.spins, w);
+    std::vector<cx<float>> obar((size_t)P);
+    for (int k = 0; k < P; k++) {
+        // the contraction gives sum_s w conj(O_k)
+        const cd o = std::conj(osum[(size_t)(2 * N + k)]) / n;
+        obar[(size_t)k] = mk((float)o.real(), (float)o.imag());
+    }
+    cx<float>* dob = reinterpret_cast<cx<float>*>(c.buf("s_obar", (size_t)P * sizeof(cx<float>)));
+    c.h2d(dob, obar.data(), obar.size() * sizeof(cx<float>));
+    const int ld2 = made_pad(2 * P);
+    int64_t nb = ((int64_t)(512ll << 20) / ((int64_t)ld2 * 4)) / 64 * 64;
+    nb = std::max<int64_t>(64, std::min<int64_t>(nb, (s.n + 63) / 64 * 64));
+    float* O = reinterpret_cast<float*>(c.buf("s_rows", (size_t)nb * ld2 * sizeof(float)));
+    float* G = reinterpret_cast<float*>(c.buf("s_gram", (size_t)ld2 * ld2 * sizeof(float)));
+    double* Gd = reinterpret_cast<double*>(c.buf("s_gram64", (size_t)ld2 * ld2 * sizeof(double)));
+    c.launch("s_matrix", [&] { return launch_zero((int64_t)2 * ld2 * ld2, reinterpret_cast<float*>(Gd), c.stream); });
+    for (int64_t s0 = 0; s0 < s.n; s0 += nb) {
+        const int64_t cur = std::min<int64_t>(nb, s.n - s0), curp = (cur + 63) / 64 * 64;
+        c.launch("s_matrix", [&] {
+            return launch_o_realified<T>(s0, cur, curp, N, M, P, ld2, s.tanh, s.spins, w, dob, O, c.stream);
+        });
+        GemmEpiArgs e{};
+        c.launch("s_matrix", [&] { return launch_gemm(EPI_STORE, ld2, ld2, (int)curp, O, 1, ld2, O, ld2, 1, G, ld2, e, c.stream); });
+        c.launch("s_matrix", [&] { return launch_accum_f64((int64_t)ld2 * ld2, G, Gd, c.stream); });
+    }
+    cx<double>* S = reinterpret_cast<cx<double>*>(c.buf("s_centred", (size_t)P * P * sizeof(cx<double>)));
+    c.launch("s_matrix", [&] { return launch_s_assemble(P, ld2, 1.0 / n, eps, Gd, S, c.stream); });
+    return S;
+}
+
+// Cholesky-factor the device matrix A (n x n, lower triangle) and solve A x = x.
+static void cholesky_solve_device(Ctx& c, int n, cx<double>* A, cx<double>* x) {
+    int* info = reinterpret_cast<int*>(c.buf("chol_info", 64));
+    c.launch("cholesky", [&] { return launch_zero(16, reinterpret_cast<float*>(info), c.stream); });
+    c.launch("cholesky", [&] { return launch_cholesky(n, A, info, c.stream); });
+    int h = 0;
+    c.d2h(&h, info, sizeof(int));
+    c.sync();
+    if (h) throw Error(RBMHIP_ERR_NUMERIC, "matrix is not positive definite (pivot " + std::to_string(h) + ")");
+    c.launch("cholesky_solve", [&] { return launch_cholesky_solve(n, A, x, c.stream); });
+}
+
 extern "C" {
 
 const char* rbmhip_last_error(void) { return g_err.c_str(); }
@@ -1474,6 +1532,82 @@ int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
             c.launch("s_matrix", [&] { return launch_s_matrix_ref<T>(s.n, psi->N, psi->M, psi->has_a, s.tanh, s.spins, inv, S, c.stream); });
             c.d2h(out, S, (size_t)(P * P) * sizeof(cx<double>));
             c.sync();
+        });
+    });
+}
+
+int rbmhip_s_matrix_centred(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(out, "null output");
+        Ctx& c = ctx();
+        const int64_t P = psi->num_params() - 2 * (int64_t)psi->N;
+        RBM_REQUIRE(P <= 16384, "S matrix larger than 16384 x 16384");
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, nullptr, ens, true, true);
+            cx<double>* S = s_matrix_centred<T>(c, psi, ens, s, 0.0);
+            c.d2h(out, S, (size_t)(P * P) * sizeof(cx<double>));
+            c.sync();
+        });
+    });
+}
+
+int rbmhip_cholesky_solve(int n, const double* A, const double* b, double* x) {
+    return guard([&] {
+        RBM_REQUIRE(n > 0 && A && b && x, "bad arguments");
+        Ctx& c = ctx();
+        cx<double>* dA = reinterpret_cast<cx<double>*>(c.buf("chol_A", (size_t)n * n * sizeof(cx<double>)));
+        cx<double>* dx = reinterpret_cast<cx<double>*>(c.buf("chol_x", (size_t)n * sizeof(cx<double>)));
+        c.h2d(dA, A, (size_t)n * n * sizeof(cx<double>));
+        c.h2d(dx, b, (size_t)n * sizeof(cx<double>));
+        cholesky_solve_device(c, n, dA, dx);
+        c.d2h(x, dx, (size_t)n * sizeof(cx<double>));
+        c.sync();
+    });
+}
+
+int rbmhip_sr_update(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double eps, double* update, double* expectation) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(op && update && expectation, "bad arguments");
+        RBM_REQUIRE(eps >= 0.0, "eps must be >= 0");
+        if (!op->hermitian)
+            throw Error(RBMHIP_ERR_UNSUPPORTED, "stochastic reconfiguration needs a Hermitian operator");
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, op, ens, true, true);
+            Scalars r = energy_scalars<T>(c, psi, s, ens, nullptr);
+            const int64_t Pall = psi->num_params();
+            const int N = psi->N;
+            const int P = (int)(Pall - 2 * (int64_t)N);
+            const double n = (double)ens->num_steps();
+            const cd E(r.v[1] / n, r.v[2] / n);
+            cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * sizeof(cx<double>)));
+            CoefArgs a;
+            std::memset(&a, 0, sizeof(a));
+            a.n = s.n; a.mode = 2; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs; a.mult = s.mult;
+            a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.el = s.el; a.coef = coef;
+            a.e_mean = mk(E.real(), E.imag());
+            c.launch("coef", [&] { return launch_coef(a, c.stream); });
+            const std::vector<cd> R = contract_params<T>(c, psi, s.n, s.tanh, s.spins, coef);
+            std::vector<cd> g((size_t)P);
+            for (int k = 0; k < P; k++) g[(size_t)k] = (2.0 / n) * R[(size_t)(2 * N + k)];
+            cx<double>* S = s_matrix_centred<T>(c, psi, ens, s, eps);
+            cx<double>* dx = reinterpret_cast<cx<double>*>(c.buf("sr_x", (size_t)P * sizeof(cx<double>)));
+            c.h2d(dx, g.data(), g.size() * sizeof(cx<double>));
+            cholesky_solve_device(c, P, S, dx);
+            std::vector<cd> x((size_t)P);
+            c.d2h(x.data(), dx, x.size() * sizeof(cx<double>));
+            c.sync();
+            for (int64_t k = 0; k < Pall; k++) {
+                const cd v = k < 2 * N ? cd(0, 0) : x[(size_t)(k - 2 * N)];
+                update[2 * k] = v.real();
+                update[2 * k + 1] = v.imag();
+            }
+            expectation[0] = E.real();
+            expectation[1] = E.imag();
         });
     });
 }

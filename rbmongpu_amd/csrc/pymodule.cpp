@@ -39,7 +39,7 @@ static void check(int rc) {
         PyErr_SetString(PyExc_NotImplementedError, msg.c_str());
         throw py::error_already_set();
     }
-    if (rc == RBMHIP_ERR_ARG) throw py::value_error(msg);
+    if (rc == RBMHIP_ERR_ARG || rc == RBMHIP_ERR_NUMERIC) throw py::value_error(msg);
     throw std::runtime_error(msg);
 }
 
@@ -833,6 +833,44 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
             check(rbmhip_s_matrix(psi.h, ens.h, sp));
         }
         return S;
+    });
+    // Stochastic reconfiguration (no reference counterpart): centred S, FP64
+    // Cholesky solve, and one SR update.
+#define RBM_SR_BIND(E)                                                                                            \
+    m.def("get_S_matrix_centred", [](const Psi& psi, E& ens) {                                                   \
+        const int64_t P = psi.num_params() - 2 * (int64_t)psi.N;                                                 \
+        carr S({(py::ssize_t)P, (py::ssize_t)P});                                                                \
+        double* sp = reinterpret_cast<double*>(S.mutable_data());                                                \
+        {                                                                                                         \
+            py::gil_scoped_release nogil;                                                                         \
+            check(rbmhip_s_matrix_centred(psi.h, ens.h, sp));                                                    \
+        }                                                                                                         \
+        return S;                                                                                                 \
+    }, py::arg("psi"), py::arg("spin_ensemble"));                                                                 \
+    m.def("sr_update", [](const Psi& psi, const Operator& op, E& ens, double eps) {                              \
+        carr g((py::ssize_t)psi.num_params());                                                                    \
+        double e[2];                                                                                              \
+        {                                                                                                         \
+            py::gil_scoped_release nogil;                                                                         \
+            check(rbmhip_sr_update(psi.h, op.h, ens.h, eps, reinterpret_cast<double*>(g.mutable_data()), e));    \
+        }                                                                                                         \
+        return py::make_tuple(g, cd(e[0], e[1]));                                                                 \
+    }, py::arg("psi"), py::arg("operator_"), py::arg("spin_ensemble"), py::arg("eps") = 1e-4);
+    RBM_SR_BIND(ExactSummation)
+    RBM_SR_BIND(MonteCarloLoop)
+    RBM_SR_BIND(LearnedMonteCarloLoop)
+#undef RBM_SR_BIND
+    m.def("cholesky_solve", [](const carr& A, const carr& b) {
+        if (A.ndim() != 2 || A.shape(0) != A.shape(1) || b.size() != A.shape(0))
+            throw py::value_error("cholesky_solve needs an n x n matrix and an n-vector");
+        const int n = (int)A.shape(0);
+        carr x((py::ssize_t)n);
+        {
+            py::gil_scoped_release nogil;
+            check(rbmhip_cholesky_solve(n, reinterpret_cast<const double*>(A.data()), reinterpret_cast<const double*>(b.data()),
+                                        reinterpret_cast<double*>(x.mutable_data())));
+        }
+        return x;
     });
     m.def("activation_function", [](cd z) {
         double in[2] = {z.real(), z.imag()}, out[2];

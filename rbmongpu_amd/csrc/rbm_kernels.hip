@@ -347,6 +347,11 @@ __global__ __launch_bounds__(256) void k_coef(CoefArgs a) {
         const cx<double> lp = a.lp[s];
         const double w = a.mult ? a.mult[s] : a.exact ? a.num_configs * exp(2.0 * (a.log_prefactor + lp.re)) : 1.0;
         const cx<double> E = a.el ? a.el[s] : mk(0.0, 0.0);
+        if (a.mode == 4) {
+            // the weight itself: sum_s coef conj(O_k) = n <conj O_k>
+            a.coef[s] = mk(w, 0.0);
+            continue;
+        }
         if (a.mode == 0 || a.mode == 2) {
             if (a.mode == 2) {
                 // w (E - <E>): sum_s coef conj(O_k) = n (<E conj O_k> - <E> conj<O_k>)
@@ -931,6 +936,94 @@ hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_
     hipLaunchKernelGGL(k_theta_logpsi, dim3((unsigned)blocks), dim3(256), 0, stream, md, n, spins, lp);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Centred, weighted S-matrix S_kl = (1/n) sum_s w (O_k - <O_k>)^* (O_l - <O_l>)
+// over the RBM parameters with an O_k (b, W, visible a; order of `params`
+// after alpha and beta).  Rows of a sample batch are materialised "realified"
+// as FP32 [Re O (P) | Im O (P) | 0 padding] scaled by sqrt(w), so that one
+// real GEMM G = O^T O gives S_kl = G[k][l] + G[P+k][P+l] + i (G[k][P+l] - G[P+k][l]).
+// O_k(s): b_j -> T_j, W_ij -> s_i T_j, a_i -> s_i  (Psi.hpp:165-202).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_o_realified(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2,
+                                                     const void* __restrict__ Trec, const uint64_t* __restrict__ spins,
+                                                     const cx<double>* __restrict__ w, const cx<float>* __restrict__ obar,
+                                                     float* __restrict__ out) {
+    const int64_t total = nbp * ld2;
+    const cx<T>* Tr = reinterpret_cast<const cx<T>*>(Trec);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / ld2;
+        const int c = (int)(t - r * ld2);
+        float v = 0.0f;
+        if (r < nb && c < 2 * P) {
+            const int k = c < P ? c : c - P;
+            const int64_t s = s0 + r;
+            const uint64_t x0 = spins[2 * s], x1 = spins[2 * s + 1];
+            cx<float> o;
+            if (k < M) {
+                const cx<T> tj = Tr[s * M + k];
+                o = mk((float)tj.re, (float)tj.im);
+            } else if (k < M + N * M) {
+                const int i = (k - M) / M, j = (k - M) % M;
+                const cx<T> tj = Tr[s * M + j];
+                const float si = spin_of<float>(x0, x1, i);
+                o = mk(si * (float)tj.re, si * (float)tj.im);
+            } else {
+                o = mk(spin_of<float>(x0, x1, k - M - N * M), 0.0f);
+            }
+            const float sw = (float)sqrt(w[s].re);
+            const cx<float> d = o - obar[k];
+            v = sw * (c < P ? d.re : d.im);
+        }
+        out[t] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_accum_f64(int64_t n, const float* __restrict__ g, double* __restrict__ acc) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        acc[t] += (double)g[t];
+}
+
+__global__ __launch_bounds__(256) void k_s_assemble(int P, int ld2, double inv, double eps, const double* __restrict__ G,
+                                                    cx<double>* __restrict__ S) {
+    const int64_t total = (int64_t)P * P;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t / P), l = (int)(t % P);
+        const double rr = G[(int64_t)k * ld2 + l], ii = G[(int64_t)(P + k) * ld2 + P + l];
+        const double ri = G[(int64_t)k * ld2 + P + l], ir = G[(int64_t)(P + k) * ld2 + l];
+        S[t] = mk((rr + ii) * inv + (k == l ? eps : 0.0), (ri - ir) * inv);
+    }
+}
+
+template <typename T>
+hipError_t launch_o_realified(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2, const void* Trec,
+                              const uint64_t* spins, const cx<double>* w, const cx<float>* obar, float* out,
+                              hipStream_t stream) {
+    const int64_t total = nbp * ld2;
+    const unsigned blocks = (unsigned)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
+    hipLaunchKernelGGL((k_o_realified<T>), dim3(blocks), dim3(256), 0, stream, s0, nb, nbp, N, M, P, ld2, Trec, spins, w,
+                       obar, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_accum_f64(int64_t n, const float* g, double* acc, hipStream_t stream) {
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
+    hipLaunchKernelGGL(k_accum_f64, dim3(blocks), dim3(256), 0, stream, n, g, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_s_assemble(int P, int ld2, double inv, double eps, const double* G, cx<double>* S, hipStream_t stream) {
+    const int64_t total = (int64_t)P * P;
+    const unsigned blocks = (unsigned)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
+    hipLaunchKernelGGL(k_s_assemble, dim3(blocks), dim3(256), 0, stream, P, ld2, inv, eps, G, S);
+    return hipGetLastError();
+}
+
+template hipError_t launch_o_realified<float>(int64_t, int64_t, int64_t, int, int, int, int, const void*, const uint64_t*,
+                                              const cx<double>*, const cx<float>*, float*, hipStream_t);
+template hipError_t launch_o_realified<double>(int64_t, int64_t, int64_t, int, int, int, int, const void*,
+                                               const uint64_t*, const cx<double>*, const cx<float>*, float*, hipStream_t);
 
 hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, int deriv, hipStream_t stream) {
     int64_t g = (n + 255) / 256;

@@ -40,7 +40,7 @@ constexpr int OK_ROWS_PER_THREAD = 8;
 
 struct CoefArgs {
     int64_t n;
-    int mode;            // 0 energy, 1 Hilbert-space distance, 2 centred energy, 3 centred distance
+    int mode;            // 0 energy, 1 Hilbert-space distance, 2 centred energy, 3 centred distance, 4 weight
     int exact;           // ExactSummation weights
     const double* mult;  // per-configuration multiplicity weights (runs of equal samples), may be null
     int is_unitary;
@@ -90,6 +90,16 @@ template <typename T>
 hipError_t launch_s_matrix_ref(int64_t n, int N, int M, int has_a, const void* Trec, const uint64_t* spins, double inv,
                                cx<double>* S, hipStream_t stream);
 hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, int deriv, hipStream_t stream);
+// Centred S-matrix pieces: realified, weighted, centred O rows of samples
+// [s0, s0 + nb) (rows nb .. nbp zero; columns [Re (P) | Im (P) | 0 .. ld2));
+// FP64 accumulation of an FP32 Gram block; assembly of S (P x P complex,
+// scaled by inv, eps added on the diagonal).
+template <typename T>
+hipError_t launch_o_realified(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2, const void* Trec,
+                              const uint64_t* spins, const cx<double>* w, const cx<float>* obar, float* out,
+                              hipStream_t stream);
+hipError_t launch_accum_f64(int64_t n, const float* g, double* acc, hipStream_t stream);
+hipError_t launch_s_assemble(int P, int ld2, double inv, double eps, const double* G, cx<double>* S, hipStream_t stream);
 // log psi of n configurations of an FP32 RBM without visible bias (N <= 128) as
 // an MFMA GEMM theta = b + W^T s with a log-cosh epilogue.
 hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_t* spins, cx<double>* lp,
