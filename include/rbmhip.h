@@ -36,6 +36,9 @@ extern "C" {
 
 #define RBMHIP_FP32 32 /* complex64 kernels, FP64 reductions */
 #define RBMHIP_FP64 64 /* complex128 kernels */
+#define RBMHIP_BF16 16 /* PsiDeep only: forward passes as BF16 MFMA layers with FP32
+                          accumulation and activations; O_k by the complex64 kernels on
+                          the bfloat16-rounded weights */
 
 typedef struct rbmhip_psi rbmhip_psi;
 typedef struct rbmhip_op rbmhip_op;
@@ -69,7 +72,8 @@ int rbmhip_rbm_set_params(rbmhip_psi* psi, const double* b, const double* W, con
  * weights = concatenated C_l x size_l complex, biases = concatenated size_l
  * complex, final_weights = size_{L-1} complex.  ref_compat_final = 1 keeps the
  * reference's final-weight derivative log_psi * x_j (PsiDeep.hpp:285-291);
- * 0 uses dlog psi/df_j = x_j. */
+ * 0 uses dlog psi/df_j = x_j.  precision = RBMHIP_BF16 needs layers of at most
+ * 256 units without translational invariance, and weights that fit in LDS. */
 int rbmhip_deep_create(int N, int num_layers, const int32_t* sizes, const int32_t* conn_counts,
                        const uint32_t* conns, const double* weights, const double* biases,
                        const double* final_weights, double prefactor, int translational_invariance,

@@ -182,6 +182,9 @@ struct DevModel {
     DevBuf W, b, a;
     // PsiDeep
     DevBuf conns, weights, biases, final_w, rhs_begin, rhs_entries;
+    // PsiDeep, BF16 MFMA layers
+    uint64_t mfma_version = ~(uint64_t)0;
+    DevBuf mfrag, mvec;
 };
 
 struct rbmhip_psi {
@@ -212,6 +215,7 @@ struct rbmhip_psi {
             kv.second.W.release(); kv.second.b.release(); kv.second.a.release();
             kv.second.conns.release(); kv.second.weights.release(); kv.second.biases.release();
             kv.second.final_w.release(); kv.second.rhs_begin.release(); kv.second.rhs_entries.release();
+            kv.second.mfrag.release(); kv.second.mvec.release();
         }
     }
 };
@@ -251,12 +255,34 @@ static RbmDev<T> rbm_dev(Ctx& c, rbmhip_psi* psi) {
     return md;
 }
 
+// bfloat16 bits of x, round to nearest even (finite inputs)
+static uint16_t bf16_bits(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float bf16_round(double x) {
+    const uint32_t u = (uint32_t)bf16_bits((float)x) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
 template <typename T>
 static DeepDev<T> deep_dev(Ctx& c, rbmhip_psi* psi) {
     DevModel& d = psi->dev[c.device];
     const int L = (int)psi->sizes.size();
     if (d.version != psi->version || d.precision != psi->precision) {
-        upload_cx<T>(c, d.weights, psi->weights);
+        if (psi->precision == RBMHIP_BF16) {
+            // the complex64 kernels (O_k) see the same bfloat16 weights as the MFMA layers
+            std::vector<cd> wr(psi->weights.size());
+            for (size_t i = 0; i < wr.size(); i++)
+                wr[i] = cd(bf16_round(psi->weights[i].real()), bf16_round(psi->weights[i].imag()));
+            upload_cx<T>(c, d.weights, wr);
+        } else {
+            upload_cx<T>(c, d.weights, psi->weights);
+        }
         upload_cx<T>(c, d.biases, psi->biases);
         upload_cx<T>(c, d.final_w, psi->final_weights);
         d.conns.ensure(sizeof(uint32_t) * psi->conns.size());
@@ -296,6 +322,100 @@ static DeepDev<T> deep_dev(Ctx& c, rbmhip_psi* psi) {
     md.final_w = reinterpret_cast<const cx<T>*>(d.final_w.p);
     md.translational_invariance = psi->translational_invariance;
     md.ref_compat_final = psi->ref_compat_final;
+    return md;
+}
+
+// Dense BF16 fragments of a PsiDeep's layers (layout in deep_kernels.hpp).
+static bool deep_mfma_fits(const rbmhip_psi* psi, std::string* why) {
+    if (psi->translational_invariance) { if (why) *why = "translational invariance"; return false; }
+    int prev = psi->N;
+    for (size_t l = 0; l < psi->sizes.size(); l++) {
+        if ((prev + 31) / 32 > DM_MAX_KB || (psi->sizes[l] + 31) / 32 > DM_MAX_KB) {
+            if (why) *why = "a layer wider than 256";
+            return false;
+        }
+        prev = psi->sizes[l];
+    }
+    return true;
+}
+
+static DeepMfmaDev deep_mfma_dev(Ctx& c, rbmhip_psi* psi) {
+    std::string why;
+    if (!deep_mfma_fits(psi, &why))
+        throw Error(RBMHIP_ERR_UNSUPPORTED, "precision=16 (BF16 MFMA) PsiDeep does not support " + why);
+    const int L = (int)psi->sizes.size();
+    DeepMfmaDev md;
+    std::memset(&md, 0, sizeof(md));
+    md.N = psi->N;
+    md.L = L;
+    int fo = 0, vo = 0, prev = psi->N;
+    for (int l = 0; l < L; l++) {
+        md.in_p[l] = (prev + 31) / 32 * 32;
+        md.out_p[l] = l == L - 1 ? (psi->sizes[l] + 15) / 16 * 16 : (psi->sizes[l] + 31) / 32 * 32;
+        md.frag_off[l] = fo;
+        md.vec_off[l] = vo;
+        fo += (md.out_p[l] / 16) * (md.in_p[l] / 32) * 2 * 64;
+        vo += 2 * md.out_p[l];
+        prev = psi->sizes[l];
+    }
+    md.final_off = vo;
+    vo += 2 * md.out_p[L - 1];
+    md.frag_total = fo;
+    md.vec_total = vo;
+    RBM_REQUIRE(deep_mfma_lds_bytes(md) <= c.max_lds,
+                "the bfloat16 layers need " + std::to_string(deep_mfma_lds_bytes(md)) + " bytes of LDS");
+    DevModel& d = psi->dev[c.device];
+    if (d.mfma_version != psi->version || d.mfrag.p == nullptr) {
+        std::vector<uint16_t> frag((size_t)fo * 8, 0);
+        std::vector<float> vec((size_t)vo, 0.0f);
+        int64_t woff = 0, boff = 0;
+        prev = psi->N;
+        for (int l = 0; l < L; l++) {
+            const int size = psi->sizes[l], C = psi->conn_counts[l];
+            // dense W[i][j] (inputs x units) from the connection table
+            std::vector<cd> Wd((size_t)prev * size, cd(0, 0));
+            for (int cc = 0; cc < C; cc++)
+                for (int j = 0; j < size; j++) {
+                    const int64_t slot = woff + (int64_t)cc * size + j;
+                    Wd[(size_t)psi->conns[slot] * size + j] += psi->weights[slot];
+                }
+            const int KB = md.in_p[l] / 32, MT = md.out_p[l] / 16;
+            for (int mt = 0; mt < MT; mt++)
+                for (int kb = 0; kb < KB; kb++)
+                    for (int part = 0; part < 2; part++)
+                        for (int lane = 0; lane < 64; lane++)
+                            for (int v = 0; v < 8; v++) {
+                                const int m = mt * 16 + (lane & 15), g = lane >> 4;
+                                const int k = l == 0 ? kb * 32 + 8 * g + v
+                                                     : kb * 32 + (v < 4 ? 4 * g + v : 16 + 4 * g + (v - 4));
+                                float w = 0.0f;
+                                if (m < size && k < prev) {
+                                    const cd z = Wd[(size_t)k * size + m];
+                                    w = (float)(part == 0 ? z.real() : z.imag());
+                                }
+                                const size_t at = ((size_t)md.frag_off[l] + (size_t)((mt * KB + kb) * 2 + part) * 64 + lane) * 8 + v;
+                                frag[at] = bf16_bits(w);
+                            }
+            for (int j = 0; j < size; j++) {
+                vec[(size_t)md.vec_off[l] + j] = (float)psi->biases[boff + j].real();
+                vec[(size_t)md.vec_off[l] + md.out_p[l] + j] = (float)psi->biases[boff + j].imag();
+            }
+            woff += (int64_t)C * size;
+            boff += size;
+            prev = size;
+        }
+        for (int j = 0; j < psi->sizes[L - 1]; j++) {
+            vec[(size_t)md.final_off + j] = (float)psi->final_weights[j].real();
+            vec[(size_t)md.final_off + md.out_p[L - 1] + j] = (float)psi->final_weights[j].imag();
+        }
+        d.mfrag.ensure(frag.size() * sizeof(uint16_t));
+        RBM_HIPCHK(hipMemcpy(d.mfrag.p, frag.data(), frag.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        d.mvec.ensure(vec.size() * sizeof(float));
+        RBM_HIPCHK(hipMemcpy(d.mvec.p, vec.data(), vec.size() * sizeof(float), hipMemcpyHostToDevice));
+        d.mfma_version = psi->version;
+    }
+    md.frag = reinterpret_cast<const uint4*>(d.mfrag.p);
+    md.vec = reinterpret_cast<const float*>(d.mvec.p);
     return md;
 }
 
@@ -676,6 +796,22 @@ static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const 
         const size_t lds = rbm_w_bytes<T>(psi);
         const bool wlds = lds <= c.max_lds;
         c.launch(label, [&] { return launch_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, wlds, lds, max_blocks, c.stream); });
+    } else if (psi->precision == RBMHIP_BF16) {
+        // log psi and E_loc on the BF16 MFMA layers; O_k rows by the complex64
+        // kernels on the same bfloat16 weights
+        const DeepMfmaDev mm = deep_mfma_dev(c, psi);
+        if (rec.spins || rec.log_psi || rec.e_loc) {
+            Records r{rec.spins, rec.log_psi, rec.e_loc, nullptr};
+            c.launch(label[0] == 'e' ? "deep_mfma_eval" : label, [&] {
+                return launch_deep_mfma_eval(mm, od, op != nullptr, n, configs, index_base, r, c.num_cus, c.stream);
+            });
+        }
+        if (rec.tanh) {
+            const DeepDev<T> md = deep_dev<T>(c, psi);
+            Records r{nullptr, nullptr, nullptr, rec.tanh};
+            const OpDev none = empty_op();
+            c.launch("deep_ok", [&] { return launch_deep_eval<T>(md, none, 0, n, configs, index_base, r, max_blocks, c.stream); });
+        }
     } else {
         const DeepDev<T> md = deep_dev<T>(c, psi);
         c.launch(label[0] == 'e' ? "deep_eval" : label, [&] { return launch_deep_eval<T>(md, od, op != nullptr, n, configs, index_base, rec, max_blocks, c.stream); });
@@ -773,6 +909,11 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
             const size_t lds = rbm_w_bytes<T>(psi);
             const bool wlds = lds + 4096 <= c.max_lds;
             c.launch("mc", [&] { return launch_mc<T>(md, mp, s.spins, counts, wlds, lds, c.stream); });
+            Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
+            eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
+        } else if (psi->precision == RBMHIP_BF16) {
+            const DeepMfmaDev mm = deep_mfma_dev(c, psi);
+            c.launch("deep_mfma_mc", [&] { return launch_deep_mfma_mc(mm, mp, s.spins, counts, c.num_cus, c.stream); });
             Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
             eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
         } else {
@@ -1053,7 +1194,8 @@ int rbmhip_deep_create(int N, int num_layers, const int32_t* sizes, const int32_
         RBM_REQUIRE(N >= 1 && N <= 128, "N must be in [1, 128]");
         RBM_REQUIRE(num_layers >= 1 && num_layers <= DEEP_MAX_LAYERS, "1 to 4 layers are supported");
         RBM_REQUIRE(sizes && conn_counts && conns && weights && biases && final_weights, "null layer tables");
-        RBM_REQUIRE(precision == RBMHIP_FP32 || precision == RBMHIP_FP64, "precision must be 32 or 64");
+        RBM_REQUIRE(precision == RBMHIP_FP32 || precision == RBMHIP_FP64 || precision == RBMHIP_BF16,
+                    "precision must be 32, 64 or 16 (BF16 MFMA layers)");
         auto p = std::make_unique<rbmhip_psi>();
         p->kind = 1; p->N = N; p->precision = precision; p->prefactor = prefactor;
         p->translational_invariance = translational_invariance ? 1 : 0;

@@ -32,13 +32,17 @@ using darr = py::array_t<double, py::array::c_style | py::array::forcecast>;
 using iarr = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
 using uarr = py::array_t<uint32_t, py::array::c_style | py::array::forcecast>;
 
+// Raised for RBMHIP_ERR_UNSUPPORTED and translated to NotImplementedError by
+// the module's exception translator, which runs with the GIL held: check() is
+// often called with the GIL released, where no Python C-API call is allowed.
+struct NotImplemented : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 static void check(int rc) {
     if (rc == RBMHIP_OK) return;
     const std::string msg = rbmhip_last_error();
-    if (rc == RBMHIP_ERR_UNSUPPORTED) {
-        PyErr_SetString(PyExc_NotImplementedError, msg.c_str());
-        throw py::error_already_set();
-    }
+    if (rc == RBMHIP_ERR_UNSUPPORTED) throw NotImplemented(msg);
     if (rc == RBMHIP_ERR_ARG || rc == RBMHIP_ERR_NUMERIC) throw py::value_error(msg);
     throw std::runtime_error(msg);
 }
@@ -321,8 +325,7 @@ static std::pair<double, cd> fluctuation(const P& psi, const Operator& op, E& en
 }
 
 static void not_yet(const char* what) {
-    PyErr_SetString(PyExc_NotImplementedError, (std::string(what) + " is not implemented yet in rbmongpu_amd").c_str());
-    throw py::error_already_set();
+    throw NotImplemented(std::string(what) + " is not implemented yet in rbmongpu_amd");
 }
 
 struct ExpectationValue {
@@ -418,6 +421,13 @@ static carr local_energy_batch(const P& psi, const Operator& o, const warr& spin
 
 PYBIND11_MODULE(_pyRBMonGPU, m) {
     m.doc() = "RBMonGPU API on MI355X (gfx950) HIP kernels";
+    py::register_exception_translator([](std::exception_ptr p) {
+        try {
+            if (p) std::rethrow_exception(p);
+        } catch (const NotImplemented& e) {
+            PyErr_SetString(PyExc_NotImplementedError, e.what());
+        }
+    });
 
     py::class_<Psi>(m, "Psi")
         .def(py::init([](const darr& alpha, const darr& beta, const carr& b, const carr& W, double prefactor,

@@ -101,3 +101,42 @@ def deep_layers(N, M, C, seed=0, scale=0.3):
 def rel_err(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def bf16(x):
+    """Round to the nearest bfloat16 through float32 (ties to even), as float64."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def bf16c(z):
+    z = np.asarray(z)
+    return bf16(z.real) + 1j * bf16(z.imag)
+
+
+def dense_layers(N, conns, weights):
+    """Dense (inputs x units) complex matrices of sparse PsiDeep layers."""
+    out, prev = [], N
+    for c, w in zip(conns, weights):
+        C, size = w.shape
+        D = np.zeros((prev, size), complex)
+        for k in range(C):
+            np.add.at(D, (c[k], np.arange(size)), w[k])
+        out.append(D)
+        prev = size
+    return out
+
+
+def deep_bf16_log_psi(N, biases, conns, weights, final, words):
+    """log psi of the BF16 MFMA forward pass in float64 arithmetic: weights and
+    the inputs of every layer after the first rounded to bfloat16, biases and
+    final weights rounded to float32, my_logcosh from the FP64 oracle."""
+    w = np.asarray(words, np.uint64)
+    x = np.where((w[:, :1] >> np.arange(N, dtype=np.uint64)) & np.uint64(1), 1.0, -1.0).astype(complex)
+    dense = dense_layers(N, conns, weights)
+    for l, (D, b) in enumerate(zip(dense, biases)):
+        a = x @ bf16c(D) + np.asarray(b, np.complex64).astype(complex)
+        y = O.logcosh(a)
+        x = bf16c(y) if l + 1 < len(dense) else y
+    return x @ np.asarray(final, np.complex64).astype(complex)

@@ -1,0 +1,169 @@
+"""GPU checks of PsiDeep with precision=16: every layer as a BF16 MFMA GEMM with
+FP32 accumulation and FP32 activations.
+
+* The forward pass is compared with a float64 evaluation of the same rounded
+  model (bfloat16 weights and hidden-layer inputs, float32 biases and final
+  weights): median |d log psi| < 2e-5, max < 1e-3 (the max covers hidden
+  activations that sit on a bfloat16 rounding boundary, where FP32 and FP64
+  round to neighbouring values).
+* E_loc is compared with the dense Hamiltonian applied to the kernel's own
+  log psi over the whole basis (1e-5 relative): this isolates the local-energy
+  logic (skipped zero terms, flips, coefficients) from the rounding.
+* Sampling: visit frequencies and energy against the exact |psi|^2 of the same
+  model.
+* Against the unrounded FP64 model (the oracle) the error is stated and
+  bounded loosely.
+"""
+import numpy as np
+import pytest
+
+from helpers import O, basis_words, bf16c, deep_bf16_log_psi, deep_layers, dense_hamiltonian, rel_err
+
+import rbmongpu_amd as R
+from rbmongpu_amd import hamiltonians as HAM
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_deep(N, biases, conns, weights, final, precision=16, compat=True):
+    return R.PsiDeep(np.zeros(N), np.zeros(N), list(biases), [np.asarray(c, np.uint32) for c in conns], list(weights),
+                     np.asarray(final, complex), 1.0, False, True, precision=precision, ref_compat_final=compat)
+
+
+def layers(N, M, C, seed, re=0.8, im=0.2, bias=0.5):
+    """Layers whose pre-activations are O(1) with small imaginary parts, away
+    from the poles of my_logcosh near Im z = pi/2 where the output is
+    ill-conditioned and any rounding of the inputs is amplified."""
+    rng = np.random.default_rng(seed)
+    from rbmongpu_amd.new_neural_network import deep_connections
+    conns = deep_connections(N, M, C)
+    biases = [bias * rng.normal(size=m) + 0.2j * bias * rng.normal(size=m) for m in M]
+    weights = [(re * rng.normal(size=(c, m)) + 1j * im * rng.normal(size=(c, m))) / np.sqrt(c) for c, m in zip(C, M)]
+    final = 1.0 + 0.5 * (rng.normal(size=M[-1]) + 1j * rng.normal(size=M[-1]))
+    return biases, conns, weights, final
+
+
+def random_words(N, n, seed):
+    rng = np.random.default_rng(seed)
+    w = np.zeros((n, 2), np.uint64)
+    w[:, 0] = rng.integers(0, 2 ** min(N, 63), size=n, dtype=np.uint64)
+    if N == 64:
+        w[:, 0] |= rng.integers(0, 2, size=n, dtype=np.uint64) << np.uint64(63)
+    return w
+
+
+SHAPES = {
+    # name: N, layer sizes, connections per unit
+    "c4": (40, [160, 80], [40, 160]),      # the deep-tier network, dense
+    "sparse3": (12, [24, 12, 6], [6, 12, 4]),
+    "odd": (7, [13, 5], [7, 13]),          # padding of every dimension
+    "wide": (64, [256, 48], [64, 256]),
+}
+
+
+@pytest.mark.parametrize("shape", list(SHAPES))
+def test_bf16_forward_matches_rounded_model(shape):
+    print(f"[deep bf16 forward] {shape}", flush=True)
+    N, M, C = SHAPES[shape]
+    biases, conns, weights, final = layers(N, M, C, seed=N + len(M))
+    psi = gpu_deep(N, biases, conns, weights, final)
+    words = basis_words(N) if N <= 12 else random_words(N, 4096, seed=N)
+    lp = R.log_psi_batch(psi, words)
+    ref = deep_bf16_log_psi(N, biases, conns, weights, final, words)
+    scale = 1.0 + np.abs(ref).max()
+    d = np.abs(lp - ref)
+    print(f"  kernel vs rounded model: max {d.max():.3e} median {np.median(d):.3e}", flush=True)
+    assert np.median(d) < 1e-5 * scale and d.max() < 1e-3 * scale, (np.median(d), d.max(), scale)
+    # against the unrounded FP64 model
+    exact = O.log_psi(O.Deep(N, biases, conns, weights, final), words)
+    print(f"  bf16 vs fp64: max {np.abs(lp - exact).max():.3e} median {np.median(np.abs(lp - exact)):.3e} "
+          f"(max |log psi| {np.abs(exact).max():.2f})", flush=True)
+    assert np.abs(lp - exact).max() < 0.02 * scale
+
+
+def test_bf16_local_energy_against_dense_hamiltonian():
+    N, M, C = 10, [20, 10], [10, 20]
+    biases, conns, weights, final = layers(N, M, C, seed=3)
+    tab = HAM.j1j2_chain(N)
+    psi = gpu_deep(N, biases, conns, weights, final)
+    words = basis_words(N)
+    lp = R.log_psi_batch(psi, words)
+    H = dense_hamiltonian(N, *tab)
+    psi_v = np.exp(lp - lp.real.max())
+    ref = (H @ psi_v) / psi_v
+    el = R.local_energy_batch(psi, R.Operator(*tab, True), words)
+    assert rel_err(el, ref) < 1e-5
+
+
+def test_bf16_exact_energy_and_gradient():
+    N, M, C = 10, [20, 10], [10, 20]
+    biases, conns, weights, final = layers(N, M, C, seed=4)
+    tab = HAM.j1j2_chain(N)
+    psi = gpu_deep(N, biases, conns, weights, final, compat=False)
+    ex = R.ExactSummation(N, True)
+    psi.normalize(ex)
+    g, E = R.ExpectationValue(True).gradient(psi, R.Operator(*tab, True), ex)
+    # the FP64 oracle on the bfloat16-rounded weights: the remaining difference
+    # is the rounding of the hidden activations
+    rw = [bf16c(w) for w in weights]
+    ref = O.Deep(N, biases, conns, rw, final, prefactor=psi.prefactor, ref_compat_final=False)
+    gr, Er = O.gradient(ref, O.Operator(*tab), O.Exact(N), mode=0)
+    assert abs(E - Er) < 2e-3 * abs(Er), (E, Er)
+    assert rel_err(g, gr) < 2e-2
+
+
+def test_bf16_sampler_visits_psi_squared():
+    N, M, C = 8, [16, 8], [8, 16]
+    biases, conns, weights, final = layers(N, M, C, seed=5, re=0.6, bias=0.3)
+    psi = gpu_deep(N, biases, conns, weights, final)
+    words = basis_words(N)
+    lp = R.log_psi_batch(psi, words)
+    p = np.exp(2 * (lp.real - lp.real.max()))
+    p /= p.sum()
+    chains, per = 1024, 200
+    mc = R.MonteCarloLoop(chains * per, 1, 20, chains, True, seed=9)
+    tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)])
+    R.ExpectationValue(True)(psi, R.Operator(*tab, True), mc)
+    spins, slp, _ = mc.last_samples()
+    counts = np.bincount(spins[:, 0].astype(np.int64), minlength=2 ** N)
+    n = counts.sum()
+    f = counts / n
+    # Sokal-free bound: chains are independent, samples one sweep apart
+    se = np.sqrt(np.maximum(p * (1 - p), 1.0 / n) / n) * 3.0
+    big = p > 1e-3
+    z = np.abs(f - p)[big] / se[big]
+    assert np.mean(z < 4.5) > 0.97, np.sort(z)[-5:]
+    assert rel_err(slp, lp[spins[:, 0].astype(np.int64)]) < 1e-6
+    assert 0.05 < mc.acceptance_rate < 1.0
+
+
+def test_bf16_mc_energy_within_three_standard_errors():
+    N, M, C = 12, [24, 12], [12, 24]
+    biases, conns, weights, final = layers(N, M, C, seed=6, re=0.5, bias=0.3)
+    tab = HAM.j1j2_chain(N)
+    psi = gpu_deep(N, biases, conns, weights, final)
+    ev = R.ExpectationValue(True)
+    ex = R.ExactSummation(N, True)
+    psi.normalize(ex)
+    E_exact = ev(psi, R.Operator(*tab, True), ex).real
+    chains, per = 512, 100
+    mc = R.MonteCarloLoop(chains * per, 1, 30, chains, True, seed=11)
+    E_mc = ev(psi, R.Operator(*tab, True), mc).real
+    _, _, el = mc.last_samples()
+    per_chain = el.real.reshape(per, chains).mean(axis=0)
+    se = per_chain.std(ddof=1) / np.sqrt(chains)
+    assert abs(E_mc - E_exact) < 3 * se, (E_mc, E_exact, se)
+
+
+def test_bf16_rejects_unsupported_models():
+    N, M, C = 8, [256, 256], [8, 256]           # 2 x 256 x 256 bfloat16 weights exceed LDS
+    biases, conns, weights, final = deep_layers(N, M, C, seed=1, scale=0.01)
+    psi = gpu_deep(N, biases, conns, weights, final)
+    with pytest.raises(ValueError):
+        R.log_psi_batch(psi, basis_words(N))
+    N, M, C = 6, [12, 6], [6, 12]
+    biases, conns, weights, final = deep_layers(N, M, C, seed=1, scale=0.1)
+    psi = gpu_deep(N, biases, conns, weights, final)
+    psi.translational_invariance = True
+    with pytest.raises(NotImplementedError):
+        R.log_psi_batch(psi, basis_words(N))
