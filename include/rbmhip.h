@@ -146,9 +146,33 @@ int rbmhip_expectation_list(rbmhip_psi* psi, rbmhip_op* const* ops, int n_ops, r
 /* ExpectationValue::fluctuation: ExpectationValue.cu:130-175 */
 int rbmhip_fluctuation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* fluctuation, double* expectation);
 /* ExpectationValue::gradient: ExpectationValue.cu:177-256; grad has num_params
- * complex values.  The reference's term conj<E_k> equals <E conj(O_k)> for a
- * Hermitian operator; such operators use 2<E conj O> - 2<E> conj<O>. */
+ * complex values.  The reference's term conj<E_k> has the expectation
+ * <E conj(O_k)> for a Hermitian operator; such operators use the centred
+ * 2<E conj O> - 2<E> conj<O>, other operators the reference's E_k estimator. */
 int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad, double* expectation);
+/* The same gradient with a chosen estimator: RBMHIP_GRAD_EK is the reference's
+ * <E conj O_k> + conj<E_k> - 2<E> conj<O_k> with E_k(s) = sum_n c_n psi(s'_n)/psi(s) O_k(s'_n)
+ * (ExpectationValue.cu:177-256, Operator.hpp:145-179), valid for any operator;
+ * RBMHIP_GRAD_HERMITIAN is 2<E conj O_k> - 2<E> conj<O_k> (Hermitian operators
+ * only); RBMHIP_GRAD_AUTO (rbmhip_gradient) picks the Hermitian form when the
+ * operator is Hermitian and E_k otherwise. */
+#define RBMHIP_GRAD_AUTO 0
+#define RBMHIP_GRAD_EK 1
+#define RBMHIP_GRAD_HERMITIAN 2
+int rbmhip_gradient_estimator(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, int estimator, double* grad,
+                              double* expectation);
+/* ExpectationValue::fluctuation_gradient: ExpectationValue.cu:258-332 (the
+ * reference's E_k form); grad has num_params complex values */
+int rbmhip_fluctuation_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad);
+/* ExpectationValue::corrected: ExpectationValue.cu:74-127; value = <E>/<X>
+ * (complex), norm = <X> with X = exp(2 Im E + Re(E^2 - E)).  As in the
+ * reference, both local energies come from op; op2 is accepted and unused. */
+int rbmhip_corrected(rbmhip_psi* psi, rbmhip_op* op, rbmhip_op* op2, rbmhip_ens* ens, double* value, double* norm);
+/* ExpectationValue::difference: ExpectationValue.cu:334-446; on samples of psi,
+ * out[i] = <E'_i> - <r E_i> / <r> with E'_i the local energy of psi_prime and
+ * r = |psi/psi_prime|^2 (2 * n_ops doubles) */
+int rbmhip_difference(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* const* ops, int n_ops, rbmhip_ens* ens,
+                      double* out);
 /* get_S_matrix(Psi, ExactSummation): source/network_functions/S_matrix.cu:10-58
  * (num_params x num_params complex, unweighted and uncentred as in the reference) */
 int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out);

@@ -80,6 +80,9 @@ def _declare(L):
     L.orc_expectation.argtypes = [P, P, P, P]
     L.orc_fluctuation.argtypes = [P, P, P, P, P]
     L.orc_gradient.argtypes = [P, P, P, C.c_int, P, P]
+    L.orc_fluctuation_gradient.argtypes = [P, P, P, P]
+    L.orc_difference.argtypes = [P, P, P, C.c_int, P, P]
+    L.orc_corrected.argtypes = [P, P, P, P, P]
     L.orc_psi_norm.argtypes = [P, P]; L.orc_psi_norm.restype = C.c_double
     L.orc_psi_vector.argtypes = [P, P]
     L.orc_s_matrix.argtypes = [P, P, P]
@@ -306,6 +309,34 @@ def gradient(psi, op, ens, mode=0):
     lib().orc_gradient(*c.args(), mode, g.ctypes.data, E.ctypes.data)
     c.done()
     return g, complex(E[0])
+
+
+def fluctuation_gradient(psi, op, ens):
+    """ExpectationValue::fluctuation_gradient (ExpectationValue.cu:258-332)."""
+    c = _Call(psi, op, ens)
+    g = np.zeros(psi.num_params, dtype=np.complex128)
+    lib().orc_fluctuation_gradient(*c.args(), g.ctypes.data)
+    c.done()
+    return g
+
+
+def difference(psi, psi_prime, ops, ens):
+    """ExpectationValue::difference (ExpectationValue.cu:334-446)."""
+    m, mp, e = psi.struct(), psi_prime.struct(), ens.struct()
+    structs = [o.struct() for o in ops]
+    arr = (C.c_void_p * len(ops))(*[C.cast(C.pointer(x), C.c_void_p) for x in structs])
+    out = np.zeros(len(ops), dtype=np.complex128)
+    lib().orc_difference(C.byref(m), C.byref(mp), arr, len(ops), C.byref(e), out.ctypes.data)
+    return list(out)
+
+
+def corrected(psi, op, ens):
+    """ExpectationValue::corrected (ExpectationValue.cu:74-127): (<E>/<X>, <X>)."""
+    c = _Call(psi, op, ens)
+    v = np.zeros(1, dtype=np.complex128); x = np.zeros(1)
+    lib().orc_corrected(*c.args(), v.ctypes.data, x.ctypes.data)
+    c.done()
+    return complex(v[0]), float(x[0])
 
 
 def psi_norm(psi, ens):

@@ -413,8 +413,17 @@ static cplx local_energy(const orc_model* m, const orc_op* op, const uint64_t* s
 
 /* Operator::foreach_E_k_s_prime: include/operator/Operator.hpp:145-179;
  * accumulates weight * E_s' * O_k(s') into acc (num_params). */
+static void accumulate_E_k2(const orc_model* m, const orc_op* op, const uint64_t* s, cplx log_psi,
+                            const cplx* theta, double weight, cplx* acc, cplx weight2, cplx* acc2, cplx* scratch_ok);
 static void accumulate_E_k(const orc_model* m, const orc_op* op, const uint64_t* s, cplx log_psi,
                            const cplx* theta, double weight, cplx* acc, cplx* scratch_ok) {
+    accumulate_E_k2(m, op, s, log_psi, theta, weight, acc, 0, NULL, scratch_ok);
+}
+
+/* acc += weight E_k(s), acc2 += weight2 E_k(s) (acc2 may be NULL):
+ * Operator.hpp:145-179 */
+static void accumulate_E_k2(const orc_model* m, const orc_op* op, const uint64_t* s, cplx log_psi,
+                            const cplx* theta, double weight, cplx* acc, cplx weight2, cplx* acc2, cplx* scratch_ok) {
     const int64_t P = orc_num_params(m);
     cplx* tp = m->kind == 0 ? (cplx*)malloc(sizeof(cplx) * (size_t)m->M) : NULL;
     for (int n = 0; n < op->S; n++) {
@@ -425,6 +434,8 @@ static void accumulate_E_k(const orc_model* m, const orc_op* op, const uint64_t*
         const cplx e = coef * cexp(lpp - log_psi);
         if (m->kind == 0) rbm_o_k(m, sp, tp, scratch_ok); else deep_o_k(m, sp, scratch_ok);
         for (int64_t k = 2 * (int64_t)m->N; k < P; k++) acc[k] += weight * (e * scratch_ok[k]);
+        if (acc2)
+            for (int64_t k = 2 * (int64_t)m->N; k < P; k++) acc2[k] += weight2 * (e * scratch_ok[k]);
     }
     free(tp);
 }
@@ -650,6 +661,98 @@ void orc_gradient(const orc_model* m, const orc_op* op, orc_ens* e, int mode, cp
     }
     *E_out = E;
     free(c.O); free(c.EO); free(c.Ek); free(c.scratch);
+}
+
+/* ExpectationValue::fluctuation_gradient: ExpectationValue.cu:258-332 */
+typedef struct {
+    const orc_model* m; const orc_op* op; int64_t P;
+    cplx E; double E2; cplx* O; cplx* EO; cplx* Ek; cplx* EEk; cplx* scratch;
+} fg_ctx;
+static void fg_fn(void* vctx, int64_t idx, const uint64_t* s, cplx lp, const cplx* theta, double w) {
+    fg_ctx* c = (fg_ctx*)vctx; (void)idx;
+    const cplx el = local_energy(c->m, c->op, s, lp, theta);
+    c->E += w * el;
+    c->E2 += w * (creal(el) * creal(el) + cimag(el) * cimag(el));
+    if (c->m->kind == 0) rbm_o_k(c->m, s, theta, c->scratch); else deep_o_k(c->m, s, c->scratch);
+    for (int64_t k = 2 * (int64_t)c->m->N; k < c->P; k++) {
+        c->O[k] += w * c->scratch[k];
+        c->EO[k] += w * el * conj(c->scratch[k]);
+    }
+    accumulate_E_k2(c->m, c->op, s, lp, theta, w, c->Ek, w * conj(el), c->EEk, c->scratch);
+}
+
+void orc_fluctuation_gradient(const orc_model* m, const orc_op* op, orc_ens* e, cplx* grad) {
+    const int64_t P = orc_num_params(m);
+    fg_ctx c;
+    c.m = m; c.op = op; c.P = P; c.E = 0; c.E2 = 0;
+    c.O = (cplx*)calloc((size_t)P, sizeof(cplx));
+    c.EO = (cplx*)calloc((size_t)P, sizeof(cplx));
+    c.Ek = (cplx*)calloc((size_t)P, sizeof(cplx));
+    c.EEk = (cplx*)calloc((size_t)P, sizeof(cplx));
+    c.scratch = (cplx*)calloc((size_t)P, sizeof(cplx));
+    ens_foreach(m, e, fg_fn, &c);
+    const double inv = 1.0 / ens_num_steps(e);
+    const cplx E = c.E * inv;
+    const double E2 = c.E2 * inv;
+    const double nE = creal(E) * creal(E) + cimag(E) * cimag(E);
+    const double fl = sqrt(E2 - nE);
+    for (int64_t k = 0; k < P; k++) {
+        const cplx O = c.O[k] * inv, EO = c.EO[k] * inv, Ek = c.Ek[k] * inv, EEk = c.EEk[k] * inv;
+        grad[k] = 1.0 / (2.0 * fl) * (2.0 * conj(EEk - conj(E) * Ek) - 2.0 * (conj(E) * EO) + 2.0 * conj(O) * (2.0 * nE - E2));
+    }
+    free(c.O); free(c.EO); free(c.Ek); free(c.EEk); free(c.scratch);
+}
+
+/* ExpectationValue::difference: ExpectationValue.cu:334-446.  Samples of psi;
+ * a_i = <E'_i> with E'_i the local energy of psi_prime, b_i = <r E_i> with
+ * r = |psi / psi_prime|^2, result_i = a_i - b_i / <r>. */
+void orc_difference(const orc_model* m, const orc_model* mp, const orc_op* const* ops, int n_ops, orc_ens* e, cplx* out) {
+    const int64_t n = ens_num_steps(e);
+    uint64_t* spins = (uint64_t*)malloc(sizeof(uint64_t) * 2 * (size_t)n);
+    cplx* lp = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    orc_records(m, NULL, e, spins, lp, NULL, w);
+    cplx* lpp = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    orc_log_psi(mp, spins, n, lpp);
+    cplx* el = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    cplx* elp = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    double ravg = 0;
+    for (int64_t k = 0; k < n; k++) ravg += w[k] * exp(2.0 * (creal(lp[k]) - creal(lpp[k])));
+    ravg /= (double)n;
+    for (int i = 0; i < n_ops; i++) {
+        orc_local_energy(m, ops[i], spins, n, el);
+        orc_local_energy(mp, ops[i], spins, n, elp);
+        cplx a = 0, b = 0;
+        for (int64_t k = 0; k < n; k++) {
+            const double r = exp(2.0 * (creal(lp[k]) - creal(lpp[k])));
+            a += w[k] * elp[k];
+            b += w[k] * r * el[k];
+        }
+        out[i] = a / (double)n - (b / (double)n) / ravg;
+    }
+    free(spins); free(lp); free(w); free(lpp); free(el); free(elp);
+}
+
+/* ExpectationValue::corrected: ExpectationValue.cu:74-127.  The reference
+ * takes both local energies from operator_ (lines 102-106), so operator2 does
+ * not enter: X = exp(2 Im E + Re(E^2 - E)); returns <E> / <X> and <X>. */
+void orc_corrected(const orc_model* m, const orc_op* op, orc_ens* e, cplx* value, double* norm) {
+    const int64_t n = ens_num_steps(e);
+    uint64_t* spins = (uint64_t*)malloc(sizeof(uint64_t) * 2 * (size_t)n);
+    cplx* el = (cplx*)malloc(sizeof(cplx) * (size_t)n);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    orc_records(m, op, e, spins, NULL, el, w);
+    cplx A = 0;
+    double X = 0;
+    for (int64_t k = 0; k < n; k++) {
+        A += w[k] * el[k];
+        X += w[k] * exp(2.0 * cimag(el[k]) + creal(el[k] * el[k] - el[k]));
+    }
+    A /= (double)n;
+    X /= (double)n;
+    *value = A / X;
+    *norm = X;
+    free(spins); free(el); free(w);
 }
 
 /* psi_norm: source/network_functions/PsiNorm.cu:13-44 */

@@ -827,8 +827,10 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
     Samples s;
     s.n = e->num_steps();
     RBM_REQUIRE(s.n > 0, "the ensemble has no samples");
-    const size_t tsz = psi->kind == 0 ? (size_t)psi->M : (size_t)psi->num_params() - 2 * (size_t)psi->N;
-    if (want_tanh) s.tanh = c.buf("tanh", (size_t)s.n * tsz * sizeof(cx<T>));
+    // RBM: tanh(theta) records (M per sample) for the O_k contraction.  PsiDeep
+    // O_k rows (num_params - 2N per sample) are rebuilt in bounded chunks from
+    // the recorded configurations by contract_params instead.
+    if (want_tanh && psi->kind == 0) s.tanh = c.buf("tanh", (size_t)s.n * (size_t)psi->M * sizeof(cx<T>));
     if (e->kind != 0) {
         RBM_REQUIRE(e->num_chains > 0 && e->num_samples % e->num_chains == 0,
                     "num_samples must be a positive multiple of num_markov_chains (MonteCarloLoop.cu:78)");
@@ -963,10 +965,10 @@ static Scalars reduce_scalars(Ctx& c, int64_t n, const double* sc) {
 // records (chunk partials added in FP64), FP64 VALU on complex128 records.
 template <typename T>
 static std::vector<cd> ok_contract(Ctx& c, int N, int M, int64_t n, const void* tanh, const uint64_t* spins,
-                                   const cx<double>* coef) {
+                                   const cx<double>* coef, bool fp64_sums = false) {
     const int64_t plen = (int64_t)(N + 1) * (M + 1);
     cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("okout", (size_t)plen * sizeof(cx<double>)));
-    if (sizeof(T) == sizeof(float)) {
+    if (sizeof(T) == sizeof(float) && !fp64_sums) {
         int rows_p = 0, cols_p = 0;
         ok_mfma_shape(N, M, &rows_p, &cols_p);
         const int64_t per_chunk = (int64_t)rows_p * cols_p;
@@ -1009,28 +1011,39 @@ static void rbm_params_from_R(const rbmhip_psi* psi, const cd* R, std::vector<cd
 // sum_s coef(s) conj(O_k(s)) in parameter order (entries below 2N are zero).
 template <typename T>
 static std::vector<cd> contract_params(Ctx& c, rbmhip_psi* psi, int64_t n, const void* tanh, const uint64_t* spins,
-                                       const cx<double>* coef) {
+                                       const cx<double>* coef, bool fp64_sums = false) {
     std::vector<cd> one;
     if (psi->kind == 0) {
-        std::vector<cd> R = ok_contract<T>(c, psi->N, psi->M, n, tanh, spins, coef);
+        std::vector<cd> R = ok_contract<T>(c, psi->N, psi->M, n, tanh, spins, coef, fp64_sums);
         rbm_params_from_R(psi, R.data(), one);
         return one;
     }
-    // PsiDeep: O_k rows are materialised per sample (deep kernels) and reduced
-    // column-wise.
+    // PsiDeep: O_k rows of a chunk of the recorded configurations are
+    // materialised by the deep kernels and reduced column-wise; chunks keep the
+    // row buffer at most 4 GiB and their FP64 sums are added on the host.
+    RBM_REQUIRE(spins != nullptr, "PsiDeep O_k needs the recorded configurations");
+    (void)tanh;
     const int64_t K = psi->num_params() - 2 * (int64_t)psi->N;
-    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (K * 16)));
-    nb = std::min<int64_t>(nb, std::max<int64_t>(1, (n + 63) / 64));
-    const int64_t chunk = (n + nb - 1) / nb;
-    cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * K) * sizeof(cx<double>)));
+    const int64_t rows_max = std::max<int64_t>(64, std::min<int64_t>(n, (4ll << 30) / (K * (int64_t)sizeof(cx<T>))));
+    void* rows = c.buf("deep_ok_rows", (size_t)rows_max * K * sizeof(cx<T>));
     cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("okout", (size_t)K * sizeof(cx<double>)));
-    c.launch("ok_reduce", [&] { return launch_rows_reduce<T>(n, K, tanh, coef, 1, chunk, (int)nb, part, c.stream); });
-    c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, K, part, out, c.stream); });
-    std::vector<cd> h((size_t)K);
-    c.d2h(h.data(), out, h.size() * sizeof(cx<double>));
-    c.sync();
+    std::vector<cd> acc((size_t)K, cd(0, 0)), h((size_t)K);
+    for (int64_t s0 = 0; s0 < n; s0 += rows_max) {
+        const int64_t cur = std::min<int64_t>(rows_max, n - s0);
+        Records rec{nullptr, nullptr, nullptr, rows};
+        eval_model<T>(c, psi, nullptr, cur, spins + 2 * s0, 0, rec, "deep_ok");
+        int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (K * 16)));
+        nb = std::min<int64_t>(nb, std::max<int64_t>(1, (cur + 63) / 64));
+        const int64_t chunk = (cur + nb - 1) / nb;
+        cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("okpart", (size_t)(nb * K) * sizeof(cx<double>)));
+        c.launch("ok_reduce", [&] { return launch_rows_reduce<T>(cur, K, rows, coef + s0, 1, chunk, (int)nb, part, c.stream); });
+        c.launch("partials_sum", [&] { return launch_partials_sum((int)nb, K, part, out, c.stream); });
+        c.d2h(h.data(), out, h.size() * sizeof(cx<double>));
+        c.sync();
+        for (int64_t k = 0; k < K; k++) acc[(size_t)k] += h[(size_t)k];
+    }
     one.assign((size_t)psi->num_params(), cd(0, 0));
-    for (int64_t k = 0; k < K; k++) one[(size_t)(2 * psi->N + k)] = h[(size_t)k];
+    for (int64_t k = 0; k < K; k++) one[(size_t)(2 * psi->N + k)] = acc[(size_t)k];
     return one;
 }
 
@@ -1622,13 +1635,63 @@ int rbmhip_fluctuation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* 
     });
 }
 
-int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad, double* expectation) {
+}  // extern "C"
+
+// sum_s w_s [(alpha E_s + beta) conj(O_k(s)) + (gamma E_s + delta) conj(E_k(s))]
+// in parameter order, with the E_k term (Operator.hpp:145-179) summed over the
+// virtual configurations s'_n of every sample in chunks of at most about 1 GiB
+// of records.  Both contractions accumulate in FP64.  The reference's
+// gradient (ExpectationValue.cu:243-253) is (1, -2<E>, 0, 1) / n.
+static cx<double> cxd(cd z) { return mk(z.real(), z.imag()); }
+
+template <typename T>
+static std::vector<cd> gradient_ek_sums(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, const Samples& s,
+                                        cd alpha, cd beta, cd gamma, cd delta) {
+    const int64_t n = s.n;
+    cx<double>* w = reinterpret_cast<cx<double>*>(c.buf("ek_w", (size_t)n * sizeof(cx<double>)));
+    cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)n * sizeof(cx<double>)));
+    CoefArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = n; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs; a.mult = s.mult;
+    a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.el = s.el;
+    a.mode = 4; a.coef = w;
+    c.launch("coef", [&] { return launch_coef(a, c.stream); });
+    a.mode = 5; a.coef = coef; a.alpha = cxd(alpha); a.e_mean = cxd(beta);
+    c.launch("coef", [&] { return launch_coef(a, c.stream); });
+    std::vector<cd> R = contract_params<T>(c, psi, n, s.tanh, s.spins, coef, true);
+    const OpDev od = op_dev(c, op);
+    const int T1 = od.n_terms + 1;
+    const size_t per_v = 4 * sizeof(cx<double>) + (psi->kind == 0 ? (size_t)psi->M * sizeof(cx<T>) : 0);
+    const int64_t cmax = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)((1ull << 30) / (per_v * (size_t)T1))));
+    const int64_t vmax = cmax * T1;
+    uint64_t* vspins = reinterpret_cast<uint64_t*>(c.buf("ek_spins", (size_t)vmax * 2 * sizeof(uint64_t)));
+    cx<double>* vfac = reinterpret_cast<cx<double>*>(c.buf("ek_fac", (size_t)vmax * sizeof(cx<double>)));
+    cx<double>* vlp = reinterpret_cast<cx<double>*>(c.buf("ek_lp", (size_t)vmax * sizeof(cx<double>)));
+    cx<double>* vcoef = reinterpret_cast<cx<double>*>(c.buf("ek_coef", (size_t)vmax * sizeof(cx<double>)));
+    void* vtanh = psi->kind == 0 ? c.buf("ek_tanh", (size_t)vmax * psi->M * sizeof(cx<T>)) : nullptr;
+    for (int64_t i0 = 0; i0 < n; i0 += cmax) {
+        const int64_t cnt = std::min<int64_t>(cmax, n - i0), V = cnt * T1;
+        c.launch("ek_expand", [&] { return launch_ek_expand(od, i0, cnt, s.spins, vspins, vfac, c.stream); });
+        Records rec{nullptr, vlp, nullptr, vtanh};
+        eval_model<T>(c, psi, nullptr, V, vspins, 0, rec, "ek_eval");
+        c.launch("ek_coef", [&] {
+            return launch_ek_coef(i0, cnt, T1, w, s.lp, gamma == cd(0, 0) ? nullptr : s.el, cxd(gamma), cxd(delta), vlp,
+                                  vfac, vcoef, c.stream);
+        });
+        const std::vector<cd> Rv = contract_params<T>(c, psi, V, vtanh, vspins, vcoef, true);
+        for (size_t k = 0; k < R.size(); k++) R[k] += Rv[k];
+    }
+    return R;
+}
+
+static int gradient_impl(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, int estimator, double* grad, double* expectation) {
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(op && grad && expectation, "bad arguments");
-        if (!op->hermitian)
-            throw Error(RBMHIP_ERR_UNSUPPORTED,
-                        "gradient of a non-Hermitian operator (the reference's E_k term) is not implemented yet");
+        RBM_REQUIRE(estimator >= RBMHIP_GRAD_AUTO && estimator <= RBMHIP_GRAD_HERMITIAN, "unknown gradient estimator");
+        if (estimator == RBMHIP_GRAD_HERMITIAN)
+            RBM_REQUIRE(op->hermitian, "the Hermitian gradient estimator needs a Hermitian operator");
+        const bool use_ek = estimator == RBMHIP_GRAD_EK || (estimator == RBMHIP_GRAD_AUTO && !op->hermitian);
         Ctx& c = ctx();
         with_precision(psi->precision, [&](auto tag) {
             using T = decltype(tag);
@@ -1637,6 +1700,17 @@ int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* gra
             const int64_t P = psi->num_params();
             const double n = (double)ens->num_steps();
             const cd E(r.v[1] / n, r.v[2] / n);
+            if (use_ek) {
+                const std::vector<cd> R = gradient_ek_sums<T>(c, psi, op, ens, s, cd(1, 0), -2.0 * E, cd(0, 0), cd(1, 0));
+                for (int64_t k = 0; k < P; k++) {
+                    const cd g = R[(size_t)k] / n;
+                    grad[2 * k] = g.real();
+                    grad[2 * k + 1] = g.imag();
+                }
+                expectation[0] = E.real();
+                expectation[1] = E.imag();
+                return;
+            }
             // ExpectationValue.cu:243-253 with conj<E_k> = <E conj(O_k)> (Hermitian):
             // g_k = 2 <E conj O_k> - 2 <E> conj<O_k> = (2/n) sum_s w (E - <E>) conj(O_k)
             cx<double>* coef = reinterpret_cast<cx<double>*>(c.buf("coef", (size_t)s.n * sizeof(cx<double>)));
@@ -1655,6 +1729,127 @@ int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* gra
             expectation[0] = E.real();
             expectation[1] = E.imag();
         });
+    });
+}
+
+extern "C" {
+
+int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad, double* expectation) {
+    return gradient_impl(psi, op, ens, RBMHIP_GRAD_AUTO, grad, expectation);
+}
+
+int rbmhip_gradient_estimator(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, int estimator, double* grad,
+                              double* expectation) {
+    return gradient_impl(psi, op, ens, estimator, grad, expectation);
+}
+
+int rbmhip_fluctuation_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(op && grad, "bad arguments");
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, op, ens, true, true);
+            Scalars r = energy_scalars<T>(c, psi, s, ens, nullptr);
+            const int64_t P = psi->num_params();
+            const double n = (double)ens->num_steps();
+            const cd E(r.v[1] / n, r.v[2] / n);
+            const double E2 = r.v[3] / n, nE = std::norm(E);
+            const double fl = std::sqrt(E2 - nE);
+            // ExpectationValue.cu:322-330:
+            // [2 conj(<E* E_k> - <E>* <E_k>) - 2 <E>* <E O*> + 2 conj<O> (2|<E>|^2 - <|E|^2>)] / (2 sigma)
+            const std::vector<cd> R = gradient_ek_sums<T>(c, psi, op, ens, s, -2.0 * std::conj(E), cd(2.0 * (2.0 * nE - E2), 0),
+                                                          cd(2, 0), -2.0 * E);
+            for (int64_t k = 0; k < P; k++) {
+                const cd g = R[(size_t)k] / (n * 2.0 * fl);
+                grad[2 * k] = g.real();
+                grad[2 * k + 1] = g.imag();
+            }
+        });
+    });
+}
+
+int rbmhip_corrected(rbmhip_psi* psi, rbmhip_op* op, rbmhip_op* op2, rbmhip_ens* ens, double* value, double* norm) {
+    return guard([&] {
+        check_psi(psi); check_ens(ens);
+        RBM_REQUIRE(op && value && norm, "bad arguments");
+        (void)op2;   // the reference evaluates both local energies with operator_ (ExpectationValue.cu:102-106)
+        Ctx& c = ctx();
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Samples s = sample<T>(c, psi, op, ens, false, false);
+            double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
+            CoefArgs a;
+            std::memset(&a, 0, sizeof(a));
+            a.n = s.n; a.mode = 6; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs; a.mult = s.mult;
+            a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.el = s.el; a.sc = sc;
+            c.launch("coef", [&] { return launch_coef(a, c.stream); });
+            Scalars r = reduce_scalars(c, s.n, sc);
+            const double n = (double)ens->num_steps();
+            const cd A(r.v[0] / n, r.v[1] / n);
+            const double X = r.v[2] / n;
+            const cd v = A / X;
+            value[0] = v.real();
+            value[1] = v.imag();
+            *norm = X;
+        });
+    });
+}
+
+int rbmhip_difference(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* const* ops, int n_ops, rbmhip_ens* ens,
+                      double* out) {
+    return guard([&] {
+        check_psi(psi); check_psi(psi_prime); check_ens(ens);
+        RBM_REQUIRE(ops && n_ops >= 1 && out, "bad arguments");
+        RBM_REQUIRE(psi->N == psi_prime->N, "psi and psi_prime have different N");
+        for (int i = 0; i < n_ops; i++) {
+            RBM_REQUIRE(ops[i] != nullptr, "null operator");
+            check_op_fits(psi, ops[i]);
+        }
+        Ctx& c = ctx();
+        Samples s;
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            s = sample<T>(c, psi, nullptr, ens, true, false);
+        });
+        const uint64_t* configs = s.spins;
+        cx<double>* lp2 = reinterpret_cast<cx<double>*>(c.buf("lp2", (size_t)s.n * sizeof(cx<double>)));
+        cx<double>* el = reinterpret_cast<cx<double>*>(c.buf("diff_el", (size_t)s.n * sizeof(cx<double>)));
+        cx<double>* el2 = reinterpret_cast<cx<double>*>(c.buf("diff_el2", (size_t)s.n * sizeof(cx<double>)));
+        double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
+        with_precision(psi_prime->precision, [&](auto tag) {
+            using T = decltype(tag);
+            Records rec{nullptr, lp2, nullptr, nullptr};
+            eval_model<T>(c, psi_prime, nullptr, s.n, configs, 0, rec);
+        });
+        CoefArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.n = s.n; a.exact = ens->kind == 0; a.num_configs = (double)ens->num_configs; a.mult = s.mult;
+        a.log_prefactor = std::log(psi->prefactor); a.lp = s.lp; a.lp2 = lp2; a.sc = sc;
+        a.mode = 8;
+        c.launch("coef", [&] { return launch_coef(a, c.stream); });
+        const double n = (double)ens->num_steps();
+        const double ravg = reduce_scalars(c, s.n, sc).v[0] / n;
+        for (int i = 0; i < n_ops; i++) {
+            with_precision(psi->precision, [&](auto tag) {
+                using T = decltype(tag);
+                Records rec{nullptr, nullptr, el, nullptr};
+                eval_model<T>(c, psi, ops[i], s.n, configs, 0, rec);
+            });
+            with_precision(psi_prime->precision, [&](auto tag) {
+                using T = decltype(tag);
+                Records rec{nullptr, nullptr, el2, nullptr};
+                eval_model<T>(c, psi_prime, ops[i], s.n, configs, 0, rec);
+            });
+            a.mode = 7; a.el = el; a.el2 = el2;
+            c.launch("coef", [&] { return launch_coef(a, c.stream); });
+            Scalars r = reduce_scalars(c, s.n, sc);
+            const cd av(r.v[0] / n, r.v[1] / n), bv(r.v[2] / n, r.v[3] / n);
+            const cd d = av - bv / ravg;
+            out[2 * i] = d.real();
+            out[2 * i + 1] = d.imag();
+        }
     });
 }
 
@@ -1774,9 +1969,7 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
         with_precision(psi_prime->precision, [&](auto tag) {
             using T = decltype(tag);
             if (want_grad) {
-                const size_t K = psi_prime->kind == 0 ? (size_t)psi_prime->M
-                                                      : (size_t)(psi_prime->num_params() - 2 * psi_prime->N);
-                tanh2 = c.buf("tanh2", (size_t)s.n * K * sizeof(cx<T>));
+                if (psi_prime->kind == 0) tanh2 = c.buf("tanh2", (size_t)s.n * (size_t)psi_prime->M * sizeof(cx<T>));
                 if (ens->kind == 0 && !s.spins) spins2 = reinterpret_cast<uint64_t*>(c.buf("spins", (size_t)s.n * 2 * sizeof(uint64_t)));
             }
             Records rec{spins2, lp2, nullptr, tanh2};

@@ -133,17 +133,6 @@ RBM_DI void mfma_stage(const DeepMfmaDev& md, unsigned char* smem, const uint4**
     *V = v;
 }
 
-// The matrix element of off-diagonal term t at s with per-lane t (plain
-// vector loads; Operator.hpp:45-88 in canonical form).
-RBM_DI cx<double> term_coef_lane(const OpDev& op, int t, uint64_t s0, uint64_t s1) {
-    cx<double> c = mk(0.0, 0.0);
-    for (int e = op.ent_begin[t]; e < op.ent_begin[t + 1]; e++) {
-        const uint64_t z0 = op.ent_zy[2 * e], z1 = op.ent_zy[2 * e + 1];
-        c = c + zy_sign(z0, z1, s0, s1) * op.ent_coef[e];
-    }
-    return c;
-}
-
 // log psi and E_loc of given configurations (configs, or index_base + idx),
 // 16 per wavefront, grid-stride over tiles.  Off-diagonal terms with a zero
 // matrix element are skipped: each column walks its own list of non-zero

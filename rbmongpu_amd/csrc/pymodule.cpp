@@ -301,17 +301,60 @@ static std::vector<cd> expectation_list(const P& psi, const std::vector<Operator
     return r;
 }
 
+static int estimator_code(const std::string& name) {
+    if (name == "auto") return RBMHIP_GRAD_AUTO;
+    if (name == "E_k") return RBMHIP_GRAD_EK;
+    if (name == "hermitian") return RBMHIP_GRAD_HERMITIAN;
+    throw py::value_error("estimator must be 'auto', 'E_k' or 'hermitian'");
+}
+
 template <class P, class E>
-static std::pair<carr, cd> gradient(const P& psi, const Operator& op, E& ens) {
+static std::pair<carr, cd> gradient(const P& psi, const Operator& op, E& ens, const std::string& estimator) {
     const int64_t n = psi.num_params();
     carr g((py::ssize_t)n);
     double E2[2];
     double* gp = reinterpret_cast<double*>(g.mutable_data());
+    const int est = estimator_code(estimator);
     {
         py::gil_scoped_release nogil;
-        check(rbmhip_gradient(psi.h, op.h, ens.h, gp, E2));
+        check(rbmhip_gradient_estimator(psi.h, op.h, ens.h, est, gp, E2));
     }
     return {g, cd(E2[0], E2[1])};
+}
+
+template <class P, class E>
+static carr fluctuation_gradient(const P& psi, const Operator& op, E& ens) {
+    carr g((py::ssize_t)psi.num_params());
+    double* gp = reinterpret_cast<double*>(g.mutable_data());
+    {
+        py::gil_scoped_release nogil;
+        check(rbmhip_fluctuation_gradient(psi.h, op.h, ens.h, gp));
+    }
+    return g;
+}
+
+template <class P, class E>
+static std::pair<cd, double> corrected(const P& psi, const Operator& op, const Operator& op2, E& ens) {
+    double v[2], x = 0;
+    {
+        py::gil_scoped_release nogil;
+        check(rbmhip_corrected(psi.h, op.h, op2.h, ens.h, v, &x));
+    }
+    return {cd(v[0], v[1]), x};
+}
+
+template <class P, class E>
+static std::vector<cd> difference(const P& psi, const P& psi_prime, const std::vector<Operator*>& ops, E& ens) {
+    std::vector<rbmhip_op*> hs;
+    for (auto* o : ops) hs.push_back(o->h);
+    std::vector<double> out(2 * ops.size());
+    {
+        py::gil_scoped_release nogil;
+        check(rbmhip_difference(psi.h, psi_prime.h, hs.data(), (int)hs.size(), ens.h, out.data()));
+    }
+    std::vector<cd> r;
+    for (size_t i = 0; i < ops.size(); i++) r.emplace_back(out[2 * i], out[2 * i + 1]);
+    return r;
 }
 
 template <class P, class E>
@@ -796,11 +839,12 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
     ev.def("__call__", [](ExpectationValue&, const P& psi, const std::vector<Operator*>& ops, E& ens) {          \
         return expectation_list(psi, ops, ens);                                                                 \
     });                                                                                                          \
-    ev.def("gradient", [](ExpectationValue&, const P& psi, const Operator& op, E& ens) { return gradient(psi, op, ens); }); \
+    ev.def("gradient", [](ExpectationValue&, const P& psi, const Operator& op, E& ens, const std::string& est) {    \
+        return gradient(psi, op, ens, est);                                                                      \
+    }, py::arg("psi"), py::arg("operator_"), py::arg("spin_ensemble"), py::kw_only(), py::arg("estimator") = "auto"); \
     ev.def("fluctuation", [](ExpectationValue&, const P& psi, const Operator& op, E& ens) { return fluctuation(psi, op, ens); }); \
-    ev.def("fluctuation_gradient", [](ExpectationValue&, const P&, const Operator&, E&) -> carr {                \
-        not_yet("ExpectationValue.fluctuation_gradient");                                                        \
-        return carr();                                                                                           \
+    ev.def("fluctuation_gradient", [](ExpectationValue&, const P& psi, const Operator& op, E& ens) {              \
+        return fluctuation_gradient(psi, op, ens);                                                               \
     });
     RBM_EV_BIND(Psi, MonteCarloLoop)
     RBM_EV_BIND(Psi, ExactSummation)
@@ -809,6 +853,18 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
     RBM_EV_BIND(Psi, LearnedMonteCarloLoop)
     RBM_EV_BIND(PsiDeep, LearnedMonteCarloLoop)
 #undef RBM_EV_BIND
+    // difference is bound for Psi and corrected for PsiDeep, as in pyRBMonGPU/main.cpp:302-308
+#define RBM_EV_EXTRA(E)                                                                                          \
+    ev.def("difference", [](ExpectationValue&, const Psi& psi, const Psi& psi_prime, const std::vector<Operator*>& ops, E& ens) { \
+        return difference(psi, psi_prime, ops, ens);                                                             \
+    });                                                                                                          \
+    ev.def("corrected", [](ExpectationValue&, const PsiDeep& psi, const Operator& op, const Operator& op2, E& ens) { \
+        return corrected(psi, op, op2, ens);                                                                     \
+    });
+    RBM_EV_EXTRA(MonteCarloLoop)
+    RBM_EV_EXTRA(ExactSummation)
+    RBM_EV_EXTRA(LearnedMonteCarloLoop)
+#undef RBM_EV_EXTRA
 
     auto hs = py::class_<HilbertSpaceDistance>(m, "HilbertSpaceDistance");
     hs.def(py::init([](unsigned N, unsigned num_params, bool gpu) {

@@ -282,4 +282,15 @@ RBM_DI cx<double> term_coef(const OpDev& op, int t, uint64_t s0, uint64_t s1) {
     return c;
 }
 
+// The same matrix element for a term index that differs between lanes (plain
+// vector loads instead of scalar ones).
+RBM_DI cx<double> term_coef_lane(const OpDev& op, int t, uint64_t s0, uint64_t s1) {
+    cx<double> c = mk(0.0, 0.0);
+    for (int e = op.ent_begin[t]; e < op.ent_begin[t + 1]; e++) {
+        const uint64_t z0 = op.ent_zy[2 * e], z1 = op.ent_zy[2 * e + 1];
+        c = c + zy_sign(z0, z1, s0, s1) * op.ent_coef[e];
+    }
+    return c;
+}
+
 }  // namespace rbmhip

@@ -352,6 +352,29 @@ __global__ __launch_bounds__(256) void k_coef(CoefArgs a) {
             a.coef[s] = mk(w, 0.0);
             continue;
         }
+        if (a.mode == 5) {
+            a.coef[s] = w * (a.alpha * E + a.e_mean);
+            continue;
+        }
+        if (a.mode == 6) {
+            // ExpectationValue.cu:100-110: X = exp(2 Im E + Re(E^2 - E))
+            double* sc = a.sc + s * SC_STRIDE;
+            const cx<double> q = E * E - E;
+            sc[0] = w * E.re; sc[1] = w * E.im; sc[2] = w * exp(2.0 * E.im + q.re); sc[3] = 0.0;
+            continue;
+        }
+        if (a.mode == 7 || a.mode == 8) {
+            // ExpectationValue.cu:392-412: r = |psi / psi'|^2 on samples of psi
+            const double r = exp(2.0 * (lp.re - a.lp2[s].re));
+            double* sc = a.sc + s * SC_STRIDE;
+            if (a.mode == 8) {
+                sc[0] = w * r; sc[1] = 0.0; sc[2] = 0.0; sc[3] = 0.0;
+            } else {
+                const cx<double> Ep = a.el2[s];
+                sc[0] = w * Ep.re; sc[1] = w * Ep.im; sc[2] = w * r * E.re; sc[3] = w * r * E.im;
+            }
+            continue;
+        }
         if (a.mode == 0 || a.mode == 2) {
             if (a.mode == 2) {
                 // w (E - <E>): sum_s coef conj(O_k) = n (<E conj O_k> - <E> conj<O_k>)
@@ -382,6 +405,49 @@ __global__ __launch_bounds__(256) void k_coef(CoefArgs a) {
             double* sc = a.sc + s * SC_STRIDE;
             sc[0] = w * omega.re; sc[1] = w * omega.im; sc[2] = w * ratio; sc[3] = w * nrm;
         }
+    }
+}
+
+// E_k of the reference's gradient estimator (Operator.hpp:145-179): every
+// string n maps s to s'_n with matrix element c_n(s), and
+// E_k(s) = sum_n c_n(s) psi(s'_n)/psi(s) O_k(s'_n).  In canonical form the
+// strings are the diagonal part and one entry per flip mask.  Sample i of the
+// chunk [i0, i0 + cnt) expands to T + 1 virtual configurations: slot t < T is
+// flip mask t with its matrix element, slot T is s itself with the diagonal
+// matrix element.  vspins and vfac hold cnt * (T + 1) entries.
+__global__ __launch_bounds__(256) void k_ek_expand(OpDev op, int64_t i0, int64_t cnt, const uint64_t* __restrict__ spins,
+                                                   uint64_t* __restrict__ vspins, cx<double>* __restrict__ vfac) {
+    const int T1 = op.n_terms + 1;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < cnt * T1; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + v / T1;
+        const int t = (int)(v % T1);
+        uint64_t s0 = spins[2 * i], s1 = spins[2 * i + 1];
+        cx<double> f;
+        if (t == op.n_terms) {
+            f = diag_energy(op, s0, s1);
+        } else {
+            f = term_coef_lane(op, t, s0, s1);
+            for (int q = op.flip_begin[t]; q < op.flip_begin[t + 1]; q++) flip_bit(s0, s1, op.flips[q]);
+        }
+        vspins[2 * v] = s0;
+        vspins[2 * v + 1] = s1;
+        vfac[v] = f;
+    }
+}
+
+// Contraction coefficient of virtual configuration v of parent i:
+// w_i (gamma E_i + delta) conj(c_v exp(log psi(s'_v) - log psi(s_i))), so
+// that sum_v coef_v conj(O_k(s'_v)) = sum_i w_i (gamma E_i + delta) conj(E_k(s_i)).
+// w holds the parents' weights (real parts); el may be null when gamma = 0.
+__global__ __launch_bounds__(256) void k_ek_coef(int64_t i0, int64_t cnt, int T1, const cx<double>* __restrict__ w,
+                                                 const cx<double>* __restrict__ lp, const cx<double>* __restrict__ el,
+                                                 cx<double> gamma, cx<double> delta, const cx<double>* __restrict__ vlp,
+                                                 const cx<double>* __restrict__ vfac, cx<double>* __restrict__ vcoef) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < cnt * T1; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + v / T1;
+        const cx<double> e = vfac[v] * cexp(vlp[v] - lp[i]);
+        const cx<double> f = el ? gamma * el[i] + delta : delta;
+        vcoef[v] = w[i].re * (f * conj(e));
     }
 }
 
@@ -799,6 +865,26 @@ hipError_t launch_coef(const CoefArgs& a, hipStream_t stream) {
     if (g > 4096) g = 4096;
     if (g < 1) g = 1;
     hipLaunchKernelGGL(k_coef, dim3((unsigned)g), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ek_expand(const OpDev& op, int64_t i0, int64_t cnt, const uint64_t* spins, uint64_t* vspins,
+                            cx<double>* vfac, hipStream_t stream) {
+    int64_t g = (cnt * (op.n_terms + 1) + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_ek_expand, dim3((unsigned)g), dim3(256), 0, stream, op, i0, cnt, spins, vspins, vfac);
+    return hipGetLastError();
+}
+
+hipError_t launch_ek_coef(int64_t i0, int64_t cnt, int T1, const cx<double>* w, const cx<double>* lp, const cx<double>* el,
+                          cx<double> gamma, cx<double> delta, const cx<double>* vlp, const cx<double>* vfac,
+                          cx<double>* vcoef, hipStream_t stream) {
+    int64_t g = (cnt * T1 + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_ek_coef, dim3((unsigned)g), dim3(256), 0, stream, i0, cnt, T1, w, lp, el, gamma, delta, vlp, vfac,
+                       vcoef);
     return hipGetLastError();
 }
 

@@ -40,7 +40,8 @@ constexpr int OK_ROWS_PER_THREAD = 8;
 
 struct CoefArgs {
     int64_t n;
-    int mode;            // 0 energy, 1 Hilbert-space distance, 2 centred energy, 3 centred distance, 4 weight
+    int mode;            // 0 energy, 1 Hilbert-space distance, 2 centred energy, 3 centred distance, 4 weight,
+                         // 5 affine w (alpha E + e_mean), 6 corrected, 7 difference, 8 probability ratio
     int exact;           // ExactSummation weights
     const double* mult;  // per-configuration multiplicity weights (runs of equal samples), may be null
     int is_unitary;
@@ -49,6 +50,8 @@ struct CoefArgs {
     const cx<double>* lp;
     const cx<double>* el;
     const cx<double>* lp2;
+    const cx<double>* el2;  // modes 7: local energy of the second wavefunction
+    cx<double> alpha;       // mode 5
     cx<double>* coef;    // n (modes 2, 3), may be null
     double* sc;          // SC_STRIDE x n (modes 0, 1)
     cx<double> e_mean;   // mode 2: sum w E / n
@@ -83,6 +86,13 @@ hipError_t launch_mfma_finish(int nchunks, int N, int M, int rows_p, int cols_p,
                               cx<double>* R, hipStream_t stream);
 hipError_t launch_partials_sum(int nblocks, int64_t len, const cx<double>* partials, cx<double>* out, hipStream_t stream);
 hipError_t launch_coef(const CoefArgs& a, hipStream_t stream);
+// E_k estimator: virtual configurations of samples [i0, i0 + cnt) and their
+// contraction coefficients (rbm_kernels.hip)
+hipError_t launch_ek_expand(const OpDev& op, int64_t i0, int64_t cnt, const uint64_t* spins, uint64_t* vspins,
+                            cx<double>* vfac, hipStream_t stream);
+hipError_t launch_ek_coef(int64_t i0, int64_t cnt, int T1, const cx<double>* w, const cx<double>* lp, const cx<double>* el,
+                          cx<double> gamma, cx<double> delta, const cx<double>* vlp, const cx<double>* vfac,
+                          cx<double>* vcoef, hipStream_t stream);
 hipError_t launch_scalar_partials(int64_t n, const double* sc, int nblocks, double* partials, hipStream_t stream);
 hipError_t launch_lse_partials(int64_t n, const cx<double>* lp, double log_prefactor, int nblocks, double* partials,
                                hipStream_t stream);

@@ -184,6 +184,101 @@ def test_gradient_is_derivative_of_energy():
     assert np.all(g0[:2 * N] == 0)
 
 
+def test_fluctuation_gradient_is_derivative_of_sigma():
+    """For Hermitian H and normalised psi the reference's fluctuation gradient
+    equals d sigma/dRe p + i d sigma/dIm p of sigma = sqrt(<H^2> - <H>^2) from
+    dense matrices; PsiDeep included."""
+    N, M = 4, 3
+    b, W, _ = random_rbm(N, M, seed=22, scale=0.3)
+    tab = HAM.tfim_1d(N, 1.0, 0.8)
+    tab = (np.concatenate([tab[0], [0.3]]), np.concatenate([tab[1], [[2, 2, 0]]]),
+           np.concatenate([tab[2], [[0, 1, -1]]]))
+    H = dense_hamiltonian(N, *tab)
+    assert np.allclose(H, H.conj().T)
+    params = np.concatenate([b, W.ravel()])
+
+    def model(p):
+        return O.RBM(p[:M], p[M:].reshape(N, M))
+
+    def sigma(p):
+        v = O.psi_vector(model(p))
+        v = v / np.linalg.norm(v)
+        e = np.vdot(v, H @ v).real
+        return np.sqrt(np.vdot(H @ v, H @ v).real - e * e)
+
+    psi = model(params)
+    psi.prefactor = 1.0 / np.linalg.norm(O.psi_vector(psi))
+    g = O.fluctuation_gradient(psi, O.Operator(*tab), O.Exact(N))
+    f, _ = O.fluctuation(psi, O.Operator(*tab), O.Exact(N))
+    assert abs(f - sigma(params)) < 1e-12
+    h = 1e-6
+    for k in range(len(params)):
+        d = np.zeros_like(params)
+        d[k] = h
+        fd = (sigma(params + d) - sigma(params - d)) / (2 * h)
+        fd += 1j * (sigma(params + 1j * d) - sigma(params - 1j * d)) / (2 * h)
+        assert abs(g[2 * N + k] - fd) < 1e-7
+    assert np.all(g[:2 * N] == 0)
+    # PsiDeep: the same identity on one weight and one bias per layer
+    Nd, Md, Cd = 4, [6, 3], [2, 6]
+    biases, conns, weights, final = deep_layers(Nd, Md, Cd, seed=3, scale=0.3)
+    deep = O.Deep(Nd, biases, conns, weights, final, ref_compat_final=False)
+    deep.prefactor = 1.0 / np.linalg.norm(O.psi_vector(deep))
+    gd = O.fluctuation_gradient(deep, O.Operator(*tab), O.Exact(Nd))
+
+    def sigma_d(wl, dz):
+        w2 = [x.copy() for x in weights]
+        w2[wl][0, 0] += dz
+        v = O.psi_vector(O.Deep(Nd, biases, conns, w2, final))
+        v = v / np.linalg.norm(v)
+        e = np.vdot(v, H @ v).real
+        return np.sqrt(np.vdot(H @ v, H @ v).real - e * e)
+
+    for wl, k in ((0, 2 * Nd + Md[0]), (1, 2 * Nd + Md[0] + Cd[0] * Md[0] + Md[1])):
+        fd = (sigma_d(wl, h) - sigma_d(wl, -h)) / (2 * h) + 1j * (sigma_d(wl, 1j * h) - sigma_d(wl, -1j * h)) / (2 * h)
+        assert abs(gd[k] - fd) < 1e-7
+
+
+def test_difference_invariants():
+    """difference(psi, psi', ops): zero for psi' = psi (any operator, any
+    prefactor of psi'); zero for the identity and any psi' on a normalised psi
+    (a = <w> = 1 and b / <r> = 1); local energies are pinned separately."""
+    N, M = 5, 4
+    b, W, _ = random_rbm(N, M, seed=41, scale=0.3)
+    psi = O.RBM(b, W)
+    psi.prefactor = 1.0 / np.linalg.norm(O.psi_vector(psi))
+    ops = [O.Operator(*HAM.tfim_1d(N)), O.Operator(*random_operator(N, 6, 3, seed=2))]
+    d = O.difference(psi, O.RBM(b, W), ops, O.Exact(N))
+    assert np.max(np.abs(d)) < 1e-12
+    other = O.RBM(b, W, prefactor=3.0)       # the prefactor does not enter log psi'
+    assert np.max(np.abs(O.difference(psi, other, ops, O.Exact(N)))) < 1e-12
+    b2 = b + 0.7 - 0.2j
+    ident = O.Operator(np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    d2 = O.difference(psi, O.RBM(b2, W), [ident], O.Exact(N))
+    assert abs(d2[0]) < 1e-12
+    # a genuinely different psi' gives a non-zero result for a non-trivial operator
+    assert abs(O.difference(psi, O.RBM(b2, W), ops, O.Exact(N))[0]) > 1e-3
+
+
+def test_corrected_identity_and_closed_form():
+    """corrected with the identity: E = 1 so X = 1 and the value is 1; for a
+    diagonal operator E is the classical energy and X = exp(E^2 - E)."""
+    N, M = 4, 3
+    b, W, _ = random_rbm(N, M, seed=43, scale=0.3)
+    psi = O.RBM(b, W)
+    psi.prefactor = 1.0 / np.linalg.norm(O.psi_vector(psi))
+    ident = O.Operator(np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    v, x = O.corrected(psi, ident, O.Exact(N))
+    assert abs(v - 1.0) < 1e-12 and abs(x - 1.0) < 1e-12
+    zz = HAM.table([(0.5, [(3, i), (3, (i + 1) % N)]) for i in range(N)])
+    v, x = O.corrected(psi, O.Operator(*zz), O.Exact(N))
+    p = np.abs(O.psi_vector(psi)) ** 2
+    s = basis_spins(N)
+    E = 0.5 * (s * np.roll(s, -1, axis=1)).sum(axis=1)
+    X = (p * np.exp(E * E - E)).sum()
+    assert abs(x - X) < 1e-12 * X and abs(v - (p * E).sum() / X) < 1e-12
+
+
 def test_s_matrix_outer_products():
     N, M = 3, 4
     b, W, _ = random_rbm(N, M, seed=31)
