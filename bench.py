@@ -25,7 +25,11 @@ same run (plain.value), and the two energies are compared in units of their
 combined standard error.
 
 --config c2 measures the plain GPU configuration (1-D TFIM N = 20, M = 80,
-4096 chains x (200 + 1000 sweeps)) instead.
+4096 chains x (200 + 1000 sweeps)) instead; --config c3 one stochastic-
+reconfiguration step of the 6x6 Heisenberg RBM (M = 144) with its errors
+against an FP64 recomputation; --config c4 the deep-network tier (J1-J2 N = 40,
+PsiDeep 40 -> 160 -> 80 on BF16 MFMA layers, 32768 chains) with the
+BF16 / FP32 / FP64 forward comparison on 10^5 configurations.
 
 roofline prices the dominant kernel of the measured step with algorithmic FLOP
 from rbmongpu_amd/flops.py over its average launch time from HIP events on the
@@ -44,6 +48,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 METRIC = "effective samples/sec incl. E_loc+O_k on 1 GPU: learned vs single-flip proposal"
 
 
@@ -335,13 +340,189 @@ def run_c2(args, R, HAM, F):
     return out
 
 
+def deep_c4_model(seed=1237):
+    """J1-J2 chain N = 40 with a dense 40 -> 160 -> 80 PsiDeep: complex
+    weights with real and imaginary parts N(0, 0.05^2 / fan_in), zero biases,
+    final weights 1 (the summed output)."""
+    from rbmongpu_amd.new_neural_network import deep_connections
+    N, M, C = 40, [160, 80], [40, 160]
+    rng = np.random.default_rng(seed)
+    conns = [np.asarray(c, np.uint32) for c in deep_connections(N, M, C)]
+    weights = [(0.05 / np.sqrt(c)) * (rng.normal(size=(c, m)) + 1j * rng.normal(size=(c, m))) for c, m in zip(C, M)]
+    biases = [np.zeros(m, complex) for m in M]
+    final = np.ones(M[-1], complex)
+    return N, M, C, conns, biases, weights, final
+
+
+def run_c4(args, R, HAM, F):
+    N, M, C, conns, biases, weights, final = deep_c4_model()
+    prec = args.precision if args.precision in (16, 32) else 16
+    CH, THERM = 32768, 50
+    spc = args.samples_per_chain or 10
+
+    def make(p):
+        return R.PsiDeep(np.zeros(N), np.zeros(N), biases, conns, weights, final, 1.0, False, True, precision=p)
+
+    psi = make(prec)
+    tab = HAM.j1j2_chain(N, 1.0, 0.5)
+    op = R.Operator(*tab, True)
+    mc = R.MonteCarloLoop(CH * spc, 1, THERM, CH, True, seed=4)
+    ev = R.ExpectationValue(True)
+    mc_label = "deep_mfma_mc" if prec == 16 else "deep_mc"
+    kernels = (mc_label, "deep_mfma_eval", "deep_ok", "coef", "ok_reduce", "partials_sum", "scalar_reduce")
+    ms, E, kms = time_steps(R, ev, psi, op, mc, args.steps, args.warmup, kernels)
+    _, _, el = mc.last_samples()
+    tau, mean, se = chain_stats(el, spc, CH)
+    ess = CH * spc / (2 * tau) / ms
+    mc_ms, mc_n = kms[mc_label]
+    mc_avg = mc_ms / max(mc_n, 1)
+    flop = F.deep_mc_launch_flops(N, M, CH, (THERM + spc) * N)
+    achieved = flop / (mc_avg / 1e3) / 1e12
+    peak = PEAK_BF16_TFLOPS if prec == 16 else PEAK_FP32_TFLOPS
+    # forward check: BF16 MFMA vs complex64 vs complex128 on 1e5 random configurations
+    rng = np.random.default_rng(99)
+    words = np.zeros((100000, 2), np.uint64)
+    words[:, 0] = rng.integers(0, 2 ** N, size=100000, dtype=np.uint64)
+    lps = {p: R.log_psi_batch(make(p), words) for p in (16, 32, 64)}
+
+    def err(a, b):
+        d = np.abs(lps[a] - lps[b])
+        return {"max": float(d.max()), "median": float(np.median(d)), "rms": float(np.sqrt((d ** 2).mean()))}
+
+    out = {
+        "metric": METRIC + " (deep-network tier, plain single-flip chain)",
+        "value": ess, "unit": "effective samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (MFMA inputs), fp32 accumulation" if prec == 16 else "c64",
+        "data": "synthetic (seeded N(0, 0.05^2/fan_in) PsiDeep weights)",
+        "config": {"workload": "1-D J1-J2 Heisenberg N=40 periodic, J1=1, J2=0.5; PsiDeep dense 40->160->80, "
+                               f"log-cosh, summed output; {CH} chains x ({THERM} therm + {spc} measurement sweeps) "
+                               "single-flip Metropolis with the full forward pass per proposal; gradient step "
+                               "(sampling + E_loc + O_k + FP64 gradient contraction)",
+                   "chains": CH, "samples_per_chain": spc, "thermalization_sweeps": THERM, "N": N,
+                   "layers": M, "precision": prec},
+        "tau_int": tau, "energy": E.real, "energy_se": se, "samples_per_s": CH * spc / ms,
+        "acceptance_rate": mc.acceptance_rate,
+        "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
+        "roofline": {"bound": "mfma" if prec == 16 else "valu", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": "k_deep_mfma_mc"
+                     if prec == 16 else "k_deep_mc", "flop_per_launch": flop, "avg_launch_ms": mc_avg,
+                     "valu_flop_per_launch": CH * (THERM + spc) * N * F.deep_forward_valu_flops(M),
+                     "algorithmic_bytes_per_launch": F.mc_launch_bytes(CH, spc)},
+        "forward_check": {"configurations": 100000, "bf16_vs_fp64": err(16, 64), "fp32_vs_fp64": err(32, 64),
+                          "bf16_vs_fp32": err(16, 32), "max_abs_log_psi": float(np.abs(lps[64]).max())},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        model = O.Deep(N, biases, [np.asarray(c) for c in conns], weights, final)
+        oop = O.Operator(*tab)
+        t0 = time.perf_counter()
+        O.gradient(model, oop, O.MonteCarlo(N, 1, 1, THERM, 1, seed=1), mode=0)
+        t_one = time.perf_counter() - t0
+        spc_cpu = max(1, int(args.cpu_budget / max(t_one, 1e-3)))
+        t0 = time.perf_counter()
+        O.gradient(model, oop, O.MonteCarlo(N, spc_cpu, 1, THERM, 1, seed=2), mode=0)
+        wall = time.perf_counter() - t0
+        cps = spc_cpu / wall
+        out["cpu_baseline"] = {
+            "value": cps / (2.0 * tau), "unit": "effective samples/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (reference algorithm: full forward per proposal and per string, per-string "
+                      f"O_k(s') gradient estimator), 1 chain x ({THERM} thermalisation + {spc_cpu} measured sweeps) "
+                      f"of the same model in {wall:.1f} s on one core; raw {cps:.2f} samples/s divided by "
+                      f"2 tau_int of the GPU run"}
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+def run_c3(args, R, HAM, F):
+    """6x6 Heisenberg, RBM M = 144: one stochastic-reconfiguration step
+    (sampling, E_loc, O_k, gradient, centred S on FP32 MFMA with FP64
+    accumulation, FP64 Cholesky solve of S + 1e-4 I).  Errors of S, the
+    gradient and the update are stated against an FP64 recomputation from the
+    same samples of a 2048-chain ensemble."""
+    N, M, CH, THERM, EPS = 36, 144, 16384, 100, 1e-4
+    spc = args.samples_per_chain or 16
+    b, W = rbm_params(N, M, 1235)
+    psi = R.Psi(np.zeros(N), np.zeros(N), b, W, 1.0, False, True, precision=32)
+    tab = HAM.heisenberg_2d(6, 6)
+    op = R.Operator(*tab, True)
+    mc = R.MonteCarloLoop(CH * spc, 1, THERM, CH, True, seed=5)
+    kernels = ("mc", "eval", "ok_reduce", "s_matrix", "s_gram", "cholesky", "cholesky_solve")
+    for _ in range(args.warmup):
+        R.sr_update(psi, op, mc, EPS)
+    R.synchronize()
+    R.timing_enable(True)
+    R.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        upd, E = R.sr_update(psi, op, mc, EPS)
+    R.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps
+    R.timing_enable(False)
+    kms = {k: R.timing_query(k) for k in kernels}
+    _, _, el = mc.last_samples()
+    tau, mean, se = chain_stats(el, spc, CH)
+    P = M + N * M
+    n = CH * spc
+    gram_ms, gram_n = kms["s_gram"]
+    flop = F.s_matrix_flops(P, n)
+    achieved = flop / (gram_ms / args.steps / 1e3) / 1e12
+    # FP64 recomputation on the samples of one 2048-chain call
+    from oracle import oracle as O
+    chk = R.MonteCarloLoop(2048, 1, THERM, 2048, True, seed=6)
+    chk_s, chk_g = R.MonteCarloLoop(chk), R.MonteCarloLoop(chk)
+    upd_c, E_c = R.sr_update(psi, op, chk, EPS)
+    S_gpu = R.get_S_matrix_centred(psi, chk_s)
+    g_gpu, _ = R.ExpectationValue(True).gradient(psi, op, chk_g)
+    spins, _, _ = chk.last_samples()
+    s = np.where((spins[:, :1] >> np.arange(N, dtype=np.uint64)) & np.uint64(1), 1.0, -1.0)
+    T = O.tanh(b[None, :] + s @ W)
+    Ok = np.concatenate([T, (s[:, :, None] * T[:, None, :]).reshape(len(s), N * M)], axis=1)
+    El = O.local_energy(O.RBM(b, W), O.Operator(*tab), spins)
+    Oc = Ok - Ok.mean(axis=0)
+    S_ref = (Oc.conj().T @ Oc) / len(s)
+    g_ref = (2.0 / len(s)) * ((El - El.mean())[:, None] * Ok.conj()).sum(axis=0)
+    x_ref = np.linalg.solve(S_ref + EPS * np.eye(P), g_ref)
+
+    def rel(a, b_):
+        return float(np.max(np.abs(a - b_)) / np.max(np.abs(b_)))
+
+    resid = np.linalg.norm((S_gpu + EPS * np.eye(P)) @ upd_c[2 * N:] - g_gpu[2 * N:]) / np.linalg.norm(g_gpu[2 * N:])
+    out = {
+        "metric": "stochastic-reconfiguration step (sampling, E_loc, O_k, gradient, S, Cholesky) on 1 GPU",
+        "value": n / ms, "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 kernels, fp32 MFMA S, fp64 accumulation and Cholesky",
+        "data": "synthetic (seeded N(0, 0.01^2) RBM parameters)",
+        "config": {"workload": f"2-D Heisenberg 6x6 periodic (72 bonds, 216 Pauli strings), RBM M=144, {CH} chains x "
+                               f"({THERM} therm + {spc} measurement sweeps), SR step with S ({P}x{P} complex) on "
+                               f"FP32 MFMA and the FP64 Cholesky solve of S + {EPS} I",
+                   "chains": CH, "samples_per_chain": spc, "P": P, "eps": EPS},
+        "tau_int": tau, "energy": E.real, "energy_se": se,
+        "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_gemm (S Gram blocks)",
+                     "flop_per_step": flop, "ms_per_step": gram_ms / args.steps},
+        "fp64_recomputation": {"samples": len(s), "S_rel_err": rel(S_gpu, S_ref),
+                               "gradient_rel_err": rel(g_gpu[2 * N:], g_ref),
+                               "update_rel_err": rel(upd_c[2 * N:], x_ref),
+                               "cholesky_residual": float(resid),
+                               "note": "2048 samples give S of rank <= 2048, so S + eps I has condition ~|S|/eps "
+                                       "and amplifies the FP32 error of S in the update"},
+        "cpu_baseline": None,
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=("c5", "c2"), default="c5")
-    ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
+    ap.add_argument("--config", choices=("c5", "c2", "c3", "c4"), default="c5")
+    ap.add_argument("--precision", type=int, default=None, choices=(16, 32, 64),
+                    help="c2/c5: 32 (default) or 64; c4: 16 (BF16 MFMA, default) or 32")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--samples-per-chain", type=int, default=0)
@@ -359,7 +540,10 @@ def main():
     from rbmongpu_amd import flops as F
     from rbmongpu_amd import hamiltonians as HAM
 
-    out = run_c5(args, R, HAM, F) if args.config == "c5" else run_c2(args, R, HAM, F)
+    if args.precision is None:
+        args.precision = 16 if args.config == "c4" else 32
+    run = {"c5": run_c5, "c2": run_c2, "c3": run_c3, "c4": run_c4}[args.config]
+    out = run(args, R, HAM, F)
     if args.copy_calibration:
         copy_calibration()
     print(json.dumps(out))

@@ -63,6 +63,7 @@ class Ens(C.Structure):
         ("num_thermalization_sweeps", C.c_uint32), ("num_markov_chains", C.c_uint32),
         ("seed", C.c_uint64), ("call_index", C.c_uint32),
         ("acceptances", C.c_uint64), ("rejections", C.c_uint64),
+        ("move", C.c_int32), ("tries", C.c_int32), ("k_up", C.c_int32),
     ]
 
 
@@ -185,13 +186,19 @@ class Exact:
 
 
 class MonteCarlo:
-    """Metropolis single-spin-flip ensemble with the shared Philox stream."""
+    """Metropolis ensemble with the shared Philox stream: single spin flips, or
+    with total_z the exchange move in that S_z sector
+    (MonteCarloLoop.set_total_z_symmetry), or with fast_sweep the multi-flip
+    move (set_fast_sweep)."""
 
-    def __init__(self, N, num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains, seed=0, call_index=0):
+    def __init__(self, N, num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains, seed=0, call_index=0,
+                 total_z=None, fast_sweep=None):
         self.N = N
         self.params = (num_samples, num_sweeps, num_thermalization_sweeps, num_markov_chains)
         self.seed = seed
         self.call_index = call_index
+        self.total_z = total_z
+        self.fast_sweep = fast_sweep
         self.acceptances = 0
         self.rejections = 0
 
@@ -200,6 +207,10 @@ class MonteCarlo:
         e.kind = 1; e.N = self.N
         e.num_samples, e.num_sweeps, e.num_thermalization_sweeps, e.num_markov_chains = self.params
         e.seed = self.seed; e.call_index = self.call_index
+        if self.total_z is not None:
+            e.move = 1; e.k_up = (self.total_z + self.N) // 2
+        elif self.fast_sweep is not None:
+            e.move = 2; e.tries = self.fast_sweep
         return e
 
 

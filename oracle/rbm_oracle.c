@@ -466,6 +466,10 @@ typedef struct {
     uint32_t call_index;
     /* outputs */
     uint64_t acceptances, rejections;
+    /* MonteCarloLoop move (MonteCarloLoop.hpp:79-93, 153-201): 0 single flip,
+     * 1 exchange conserving total S_z with k_up spins up, 2 fast sweep with
+     * `tries` random masks ANDed */
+    int32_t move, tries, k_up;
 } orc_ens;
 
 typedef void (*sample_fn)(void* ctx, int64_t index, const uint64_t* s, cplx log_psi, const cplx* theta, double weight);
@@ -523,15 +527,110 @@ static void initial_spins(const orc_ens* e, uint32_t chain, uint64_t* s) {
     mask_n(s, e->N);
 }
 
+/* Move draws, identical to the library's propose_move / sector_start:
+ * exchange: Philox(c, k, call, 4) = (x, y, z): flip p = x % N, then flip the
+ * (y % cnt)-th (in index order) of the cnt spins equal to the new value of p
+ * (p included); u from z.  Fast sweep: mask = AND of max(tries, 1) N-bit
+ * strings Philox(c, k, call, 0x10 + j); u from Philox(c, k, call, 5).x.
+ * Sector start: flip single bits chosen by Philox(c, j, call, 6).x among the
+ * set (too many up) or clear (too few) bits until k_up spins are up. */
+static int select_bit_c(const uint64_t* w, int k) {
+    int base = 0;
+    uint64_t x = w[0];
+    const int c0 = __builtin_popcountll(w[0]);
+    if (k >= c0) { k -= c0; x = w[1]; base = 64; }
+    for (int i = 0; i < k; i++) x &= x - 1u;
+    return base + __builtin_ctzll(x);
+}
+
+static void full_mask_c(int N, uint64_t* m) {
+    m[0] = ~(uint64_t)0; m[1] = ~(uint64_t)0;
+    mask_n(m, N);
+}
+
+static void sector_start_c(const orc_ens* e, uint32_t chain, int N, uint64_t* s) {
+    const uint32_t key[2] = {(uint32_t)e->seed, (uint32_t)(e->seed >> 32)};
+    uint64_t m[2];
+    full_mask_c(N, m);
+    for (uint32_t j = 0;; j++) {
+        const int up = __builtin_popcountll(s[0]) + __builtin_popcountll(s[1]);
+        if (up == e->k_up) break;
+        const uint32_t ctr[4] = {chain, j, e->call_index, 6u};
+        uint32_t r[4];
+        orc_philox(ctr, key, r);
+        if (up > e->k_up) {
+            flip(s, select_bit_c(s, (int)(r[0] % (uint32_t)up)));
+        } else {
+            const uint64_t c[2] = {~s[0] & m[0], ~s[1] & m[1]};
+            flip(s, select_bit_c(c, (int)(r[0] % (uint32_t)(N - up))));
+        }
+    }
+}
+
+static void propose_move_c(const orc_ens* e, int N, uint32_t chain, uint64_t k, const uint64_t* s, uint64_t* n, double* u) {
+    const uint32_t key[2] = {(uint32_t)e->seed, (uint32_t)(e->seed >> 32)};
+    uint64_t m[2];
+    full_mask_c(N, m);
+    uint32_t r[4];
+    if (e->move == 1) {
+        const uint32_t ctr[4] = {chain, (uint32_t)k, e->call_index, 4u};
+        orc_philox(ctr, key, r);
+        n[0] = s[0]; n[1] = s[1];
+        const int p = (int)(r[0] % (uint32_t)N);
+        flip(n, p);
+        const int v = (int)((p < 64 ? n[0] >> p : n[1] >> (p - 64)) & 1u);
+        const uint64_t c[2] = {v ? n[0] : (~n[0] & m[0]), v ? n[1] : (~n[1] & m[1])};
+        const int cnt = __builtin_popcountll(c[0]) + __builtin_popcountll(c[1]);
+        flip(n, select_bit_c(c, (int)(r[1] % (uint32_t)cnt)));
+        *u = ((double)r[2] + 0.5) * (1.0 / 4294967296.0);
+    } else {
+        uint64_t f[2] = {m[0], m[1]};
+        const int nt = e->tries > 1 ? e->tries : 1;
+        for (int j = 0; j < nt; j++) {
+            const uint32_t ctr[4] = {chain, (uint32_t)k, e->call_index, 0x10u + (uint32_t)j};
+            orc_philox(ctr, key, r);
+            f[0] &= (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+            f[1] &= (uint64_t)r[2] | ((uint64_t)r[3] << 32);
+        }
+        n[0] = s[0] ^ f[0]; n[1] = s[1] ^ f[1];
+        const uint32_t ctr[4] = {chain, (uint32_t)k, e->call_index, 5u};
+        orc_philox(ctr, key, r);
+        *u = ((double)r[0] + 0.5) * (1.0 / 4294967296.0);
+    }
+}
+
 static double model_log_psi_real_state(const orc_model* m, const uint64_t* s, const cplx* theta) {
     if (m->kind == 0) return creal(rbm_log_psi_from_angles(m, s, theta));
     return creal(deep_log_psi(m, s));
 }
 
 /* MonteCarloLoop::mc_update: include/spin_ensembles/MonteCarloLoop.hpp:153-256
- * (single-spin-flip branch; total-Sz and fast-sweep moves are not restated). */
+ * (single flip; the exchange and fast-sweep moves draw as propose_move_c). */
 static void mc_update(const orc_model* m, orc_ens* e, uint32_t chain, uint64_t k,
                       double* log_psi_real, uint64_t* s, cplx* theta) {
+    if (e->move != 0) {
+        /* exchange / fast sweep: the angles of the proposal from scratch */
+        uint64_t n[2];
+        double u;
+        propose_move_c(e, m->N, chain, k, s, n, &u);
+        cplx* tn = NULL;
+        if (m->kind == 0) {
+            tn = (cplx*)malloc(sizeof(cplx) * (size_t)m->M);
+            rbm_angles(m, n, tn);
+        }
+        const double next = model_log_psi_real_state(m, n, tn);
+        const double ratio = exp(2.0 * (next - *log_psi_real));
+        if (ratio > 1.0 || u <= ratio) {
+            *log_psi_real = next;
+            s[0] = n[0]; s[1] = n[1];
+            if (tn) memcpy(theta, tn, sizeof(cplx) * (size_t)m->M);
+            e->acceptances++;
+        } else {
+            e->rejections++;
+        }
+        free(tn);
+        return;
+    }
     uint32_t x; double u;
     proposal_randoms(e, chain, k, &x, &u);
     const int position = (int)(x % (uint32_t)m->N);
@@ -558,12 +657,15 @@ static void mc_foreach(const orc_model* m, orc_ens* e, sample_fn f, void* ctx) {
     for (uint32_t c = 0; c < chains; c++) {
         uint64_t s[2];
         initial_spins(e, c, s);
+        if (e->move == 1) sector_start_c(e, c, m->N, s);
         if (theta) rbm_angles(m, s, theta);
         double lpr = model_log_psi_real_state(m, s, theta);
         uint64_t k = 0;
+        /* one sample per num_sweeps proposals with the fast sweep (MonteCarloLoop.hpp:109-112) */
+        const uint64_t per_sweep = e->move == 2 ? 1u : (uint64_t)m->N;
         for (uint64_t i = 0; i < (uint64_t)e->num_thermalization_sweeps * m->N; i++) mc_update(m, e, c, k++, &lpr, s, theta);
         for (uint32_t step = 0; step < steps; step++) {
-            for (uint64_t i = 0; i < (uint64_t)e->num_sweeps * m->N; i++) mc_update(m, e, c, k++, &lpr, s, theta);
+            for (uint64_t i = 0; i < (uint64_t)e->num_sweeps * per_sweep; i++) mc_update(m, e, c, k++, &lpr, s, theta);
             const cplx lp = m->kind == 0 ? rbm_log_psi_from_angles(m, s, theta) : deep_log_psi(m, s);
             f(ctx, (int64_t)step * chains + c, s, lp, theta, 1.0);
         }

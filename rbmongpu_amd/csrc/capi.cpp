@@ -738,6 +738,10 @@ struct rbmhip_ens {
     uint64_t seed = 0;
     uint32_t call = 0;
     uint64_t acc = 0, rej = 0;
+    // MonteCarloLoop move type: has_sector selects the exchange move in the
+    // sector's total S_z (set_total_z_symmetry); fast_sweep the multi-flip move
+    bool fast_sweep = false;
+    uint32_t fast_tries = 1;
     // records of the last Monte Carlo call
     int rec_device = -1;
     int64_t rec_n = 0;
@@ -850,6 +854,17 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
         mp.seed_lo = (uint32_t)e->seed;
         mp.seed_hi = (uint32_t)(e->seed >> 32);
         mp.call = e->call++;
+        if (e->kind == 1 && e->has_sector) {
+            RBM_REQUIRE(!e->fast_sweep, "set_fast_sweep and set_total_z_symmetry cannot be combined "
+                                        "(the reference's fast sweep has no exchange partner, MonteCarloLoop.hpp:160-201)");
+            RBM_REQUIRE((e->sector + psi->N) % 2 == 0 && std::abs(e->sector) < psi->N,
+                        "the total S_z sector must have the parity of N and |sector| < N");
+            mp.move = MC_EXCHANGE;
+            mp.k_up = (e->sector + psi->N) / 2;
+        } else if (e->kind == 1 && e->fast_sweep) {
+            mp.move = MC_FAST_SWEEP;
+            mp.tries = (int)std::max<uint32_t>(1, e->fast_tries);
+        }
         uint32_t* counts = reinterpret_cast<uint32_t*>(c.buf("counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
         const OpDev od = op ? op_dev(c, op) : empty_op();
         if (e->kind == 2) {
@@ -910,7 +925,10 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
             const RbmDev<T> md = rbm_dev<T>(c, psi);
             const size_t lds = rbm_w_bytes<T>(psi);
             const bool wlds = lds + 4096 <= c.max_lds;
-            c.launch("mc", [&] { return launch_mc<T>(md, mp, s.spins, counts, wlds, lds, c.stream); });
+            if (mp.move == MC_SINGLE_FLIP)
+                c.launch("mc", [&] { return launch_mc<T>(md, mp, s.spins, counts, wlds, lds, c.stream); });
+            else
+                c.launch("mc", [&] { return launch_mc_moves<T>(md, mp, s.spins, counts, c.stream); });
             Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
             eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
         } else if (psi->precision == RBMHIP_BF16) {
@@ -1106,7 +1124,7 @@ static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, co
             return launch_o_realified<T>(s0, cur, curp, N, M, P, ld2, s.tanh, s.spins, w, dob, O, c.stream);
         });
         GemmEpiArgs e{};
-        c.launch("s_matrix", [&] { return launch_gemm(EPI_STORE, ld2, ld2, (int)curp, O, 1, ld2, O, ld2, 1, G, ld2, e, c.stream); });
+        c.launch("s_gram", [&] { return launch_gemm(EPI_STORE, ld2, ld2, (int)curp, O, 1, ld2, O, ld2, 1, G, ld2, e, c.stream); });
         c.launch("s_matrix", [&] { return launch_accum_f64((int64_t)ld2 * ld2, G, Gd, c.stream); });
     }
     cx<double>* S = reinterpret_cast<cx<double>*>(c.buf("s_centred", (size_t)P * P * sizeof(cx<double>)));
@@ -1374,6 +1392,7 @@ int rbmhip_mc_copy(const rbmhip_ens* other, rbmhip_ens** out) {
         e->num_samples = other->num_samples; e->num_sweeps = other->num_sweeps; e->num_therm = other->num_therm;
         e->num_chains = other->num_chains; e->seed = other->seed;
         e->has_sector = other->has_sector; e->sector = other->sector;
+        e->fast_sweep = other->fast_sweep; e->fast_tries = other->fast_tries;
         *out = e.release();
     });
 }
@@ -1381,16 +1400,18 @@ int rbmhip_mc_copy(const rbmhip_ens* other, rbmhip_ens** out) {
 int rbmhip_mc_set_total_z(rbmhip_ens* ens, int sector) {
     return guard([&] {
         check_ens(ens);
-        (void)sector;
-        throw Error(RBMHIP_ERR_UNSUPPORTED, "MonteCarloLoop.set_total_z_symmetry (pair moves) is not implemented yet");
+        RBM_REQUIRE(ens->kind == 1, "set_total_z_symmetry applies to a MonteCarloLoop");
+        ens->has_sector = true;
+        ens->sector = sector;
     });
 }
 
 int rbmhip_mc_set_fast_sweep(rbmhip_ens* ens, uint32_t num_tries) {
     return guard([&] {
         check_ens(ens);
-        (void)num_tries;
-        throw Error(RBMHIP_ERR_UNSUPPORTED, "MonteCarloLoop.set_fast_sweep is not implemented yet");
+        RBM_REQUIRE(ens->kind == 1, "set_fast_sweep applies to a MonteCarloLoop");
+        ens->fast_sweep = true;
+        ens->fast_tries = num_tries;
     });
 }
 

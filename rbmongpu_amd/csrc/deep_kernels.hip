@@ -212,24 +212,30 @@ __global__ __launch_bounds__(64) void k_deep_mc(DeepDev<T> md, OpDev op, int has
     uint64_t s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
     uint64_t s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
     mask_words(s0, s1, N);
+    if (mp.move == MC_EXCHANGE) sector_start(N, mp.k_up, (uint32_t)chain, mp.call, mp.seed_lo, mp.seed_hi, s0, s1);
     T cur = deep_log_psi(md, s0, s1, g, lane).re;
     uint32_t nacc = 0, nrej = 0;
     u32x4 rr = r0;
     uint64_t k = 0;
     auto propose = [&](uint64_t kk) {
-        if ((kk & 1) == 0) rr = philox((uint32_t)chain, (uint32_t)(kk >> 1), mp.call, 0u, mp.seed_lo, mp.seed_hi);
-        const uint32_t x = (kk & 1) ? rr.z : rr.x;
-        const uint32_t y = (kk & 1) ? rr.w : rr.y;
-        const int p = (int)(x % (uint32_t)N);
         uint64_t n0 = s0, n1 = s1;
-        flip_bit(n0, n1, p);
+        uint32_t y;
+        if (mp.move == MC_SINGLE_FLIP) {
+            if ((kk & 1) == 0) rr = philox((uint32_t)chain, (uint32_t)(kk >> 1), mp.call, 0u, mp.seed_lo, mp.seed_hi);
+            const uint32_t x = (kk & 1) ? rr.z : rr.x;
+            y = (kk & 1) ? rr.w : rr.y;
+            flip_bit(n0, n1, (int)(x % (uint32_t)N));
+        } else {
+            propose_move(mp.move, mp.tries, N, (uint32_t)chain, kk, mp.call, mp.seed_lo, mp.seed_hi, s0, s1, n0, n1, y);
+        }
         const T next = deep_log_psi(md, n0, n1, g, lane).re;
         const T ratio = (T)exp(T(2) * (next - cur));
         const T uu = (T)(((double)y + 0.5) * (1.0 / 4294967296.0));
         if ((ratio > T(1)) || (uu <= ratio)) { s0 = n0; s1 = n1; cur = next; nacc++; }
         else nrej++;
     };
-    const uint64_t n_therm = (uint64_t)mp.therm * N, n_sweep = (uint64_t)mp.sweeps * N;
+    const uint64_t n_therm = (uint64_t)mp.therm * N;
+    const uint64_t n_sweep = (uint64_t)mp.sweeps * (mp.move == MC_FAST_SWEEP ? 1 : N);
     for (uint64_t i = 0; i < n_therm; i++) propose(k++);
     for (int step = 0; step < mp.steps_per_chain; step++) {
         for (uint64_t i = 0; i < n_sweep; i++) propose(k++);

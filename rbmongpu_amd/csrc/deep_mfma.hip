@@ -211,10 +211,12 @@ __global__ __launch_bounds__(512) void k_deep_mfma_mc(DeepMfmaDev md, McParams m
         s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
         s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
         mask_words(s0, s1, N);
+        if (mp.move == MC_EXCHANGE) sector_start(N, mp.k_up, (uint32_t)chain, mp.call, mp.seed_lo, mp.seed_hi, s0, s1);
     }
     float cur = mfma_log_psi<KBMAX>(md, F, V, s0, s1, lane).re;
     uint32_t nacc = 0, nrej = 0;
-    const uint64_t n_therm = (uint64_t)mp.therm * N, n_sweep = (uint64_t)mp.sweeps * N;
+    const uint64_t n_therm = (uint64_t)mp.therm * N;
+    const uint64_t n_sweep = (uint64_t)mp.sweeps * (mp.move == MC_FAST_SWEEP ? 1 : N);
     const uint64_t total = n_therm + n_sweep * (uint64_t)mp.steps_per_chain;
     uint64_t next_rec = n_therm + n_sweep;
     int step = 0;
@@ -230,15 +232,21 @@ __global__ __launch_bounds__(512) void k_deep_mfma_mc(DeepMfmaDev md, McParams m
     }
     uint32_t rx = 0, ry = 0, rz = 0, rw = 0;
     for (uint64_t k = 0; k < total; k++) {
-        if ((k & 1) == 0) {
-            const u32x4 r = philox((uint32_t)chain, (uint32_t)(k >> 1), mp.call, 0u, mp.seed_lo, mp.seed_hi);
-            rx = r.x; ry = r.y; rz = r.z; rw = r.w;
-        }
-        const uint32_t x = (k & 1) ? rz : rx;
-        const uint32_t y = (k & 1) ? rw : ry;
-        const int p = (int)(x % (uint32_t)N);
         uint64_t n0 = s0, n1 = s1;
-        flip_bit(n0, n1, p);
+        uint32_t y;
+        if (mp.move == MC_SINGLE_FLIP) {
+            if ((k & 1) == 0) {
+                const u32x4 r = philox((uint32_t)chain, (uint32_t)(k >> 1), mp.call, 0u, mp.seed_lo, mp.seed_hi);
+                rx = r.x; ry = r.y; rz = r.z; rw = r.w;
+            }
+            const uint32_t x = (k & 1) ? rz : rx;
+            y = (k & 1) ? rw : ry;
+            flip_bit(n0, n1, (int)(x % (uint32_t)N));
+        } else if (valid) {
+            propose_move(mp.move, mp.tries, N, (uint32_t)chain, k, mp.call, mp.seed_lo, mp.seed_hi, s0, s1, n0, n1, y);
+        } else {
+            y = 0u;
+        }
         const float next = mfma_log_psi<KBMAX>(md, F, V, n0, n1, lane).re;
         const float ratio = expf(2.0f * (next - cur));
         const float uu = (float)(((double)y + 0.5) * (1.0 / 4294967296.0));

@@ -282,6 +282,78 @@ RBM_DI cx<double> term_coef(const OpDev& op, int t, uint64_t s0, uint64_t s1) {
     return c;
 }
 
+// ---------------------------------------------------------------------------
+// Move types of MonteCarloLoop other than the single flip
+// (include/spin_ensembles/MonteCarloLoop.hpp:79-93, 153-201).  Proposal k of
+// chain c in call `call` draws from Philox(key = seed, ctr = (c, k, call, .)):
+//   exchange (set_total_z_symmetry): block 4 = (x, y, z, .): flip p = x % N, then
+//     flip the (y % cnt)-th (in index order) of the cnt spins equal to the new
+//     value of p (p itself included, which undoes the move); u from z.  The
+//     reference draws the second spin by rejection, uniformly over the same set.
+//   fast sweep (set_fast_sweep): flip mask = AND of max(tries, 1) random N-bit
+//     strings, blocks 0x10 + j; u from block 5, word x.
+// A chain in a sector starts from its random configuration (block 1 of k = 0)
+// with single bits set or cleared until k_up spins are up, bit j chosen by
+// block (c, j, call, 6).
+// ---------------------------------------------------------------------------
+RBM_HDI int select_bit(uint64_t w0, uint64_t w1, int k) {
+    const int c0 = __builtin_popcountll(w0);
+    uint64_t w = w0;
+    int base = 0;
+    if (k >= c0) { k -= c0; w = w1; base = 64; }
+    for (int i = 0; i < k; i++) w &= w - 1u;
+    return base + __builtin_ctzll(w);
+}
+
+RBM_HDI void full_mask(int N, uint64_t& m0, uint64_t& m1) {
+    m0 = ~(uint64_t)0; m1 = ~(uint64_t)0;
+    mask_words(m0, m1, N);
+}
+
+RBM_HDI void sector_start(int N, int k_up, uint32_t chain, uint32_t call, uint32_t sk0, uint32_t sk1,
+                          uint64_t& s0, uint64_t& s1) {
+    uint64_t m0, m1;
+    full_mask(N, m0, m1);
+    for (uint32_t j = 0;; j++) {
+        const int up = __builtin_popcountll(s0) + __builtin_popcountll(s1);
+        if (up == k_up) break;
+        const u32x4 r = philox(chain, j, call, 6u, sk0, sk1);
+        if (up > k_up) {
+            flip_bit(s0, s1, select_bit(s0, s1, (int)(r.x % (uint32_t)up)));
+        } else {
+            flip_bit(s0, s1, select_bit(~s0 & m0, ~s1 & m1, (int)(r.x % (uint32_t)(N - up))));
+        }
+    }
+}
+
+// The proposed configuration (n0, n1) and the uniform's 32-bit word.
+RBM_HDI void propose_move(int move, int tries, int N, uint32_t chain, uint64_t k, uint32_t call, uint32_t sk0,
+                          uint32_t sk1, uint64_t s0, uint64_t s1, uint64_t& n0, uint64_t& n1, uint32_t& uword) {
+    uint64_t m0, m1;
+    full_mask(N, m0, m1);
+    if (move == 1) {
+        const u32x4 r = philox(chain, (uint32_t)k, call, 4u, sk0, sk1);
+        const int p = (int)(r.x % (uint32_t)N);
+        n0 = s0; n1 = s1;
+        flip_bit(n0, n1, p);
+        const bool v = bit_of(n0, n1, p) != 0;
+        const uint64_t c0 = v ? n0 : (~n0 & m0), c1 = v ? n1 : (~n1 & m1);
+        const int cnt = __builtin_popcountll(c0) + __builtin_popcountll(c1);
+        flip_bit(n0, n1, select_bit(c0, c1, (int)(r.y % (uint32_t)cnt)));
+        uword = r.z;
+    } else {
+        uint64_t f0 = m0, f1 = m1;
+        const int nt = tries > 1 ? tries : 1;
+        for (int j = 0; j < nt; j++) {
+            const u32x4 r = philox(chain, (uint32_t)k, call, 0x10u + (uint32_t)j, sk0, sk1);
+            f0 &= (uint64_t)r.x | ((uint64_t)r.y << 32);
+            f1 &= (uint64_t)r.z | ((uint64_t)r.w << 32);
+        }
+        n0 = s0 ^ f0; n1 = s1 ^ f1;
+        uword = philox(chain, (uint32_t)k, call, 5u, sk0, sk1).x;
+    }
+}
+
 // The same matrix element for a term index that differs between lanes (plain
 // vector loads instead of scalar ones).
 RBM_DI cx<double> term_coef_lane(const OpDev& op, int t, uint64_t s0, uint64_t s1) {

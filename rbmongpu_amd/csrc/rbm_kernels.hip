@@ -784,6 +784,82 @@ static int pick_jpl(int M) {
     return -1;
 }
 
+// Metropolis chains with the exchange (total S_z) or fast-sweep move
+// (MonteCarloLoop.hpp:153-256 with the move draws of propose_move).  One
+// 16-lane row per chain as in k_mc_rbm; theta' adds the W rows of every spin
+// that the move flips.  The chain records its configurations only.
+template <typename T, int JPL>
+__global__ __launch_bounds__(256) void k_mc_rbm_moves(RbmDev<T> md, McParams mp, uint64_t* __restrict__ spins_out,
+                                                      uint32_t* __restrict__ counts) {
+    const int N = md.N, ld = md.ld;
+    const int lane = threadIdx.x & 15, row = threadIdx.x >> 4;
+    const int chain_raw = blockIdx.x * 16 + row;
+    const bool valid = chain_raw < mp.chains;
+    const uint32_t chain = (uint32_t)(valid ? chain_raw : mp.chains - 1);
+    const u32x4 r0 = philox(chain, 0u, mp.call, 1u, mp.seed_lo, mp.seed_hi);
+    uint64_t s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
+    uint64_t s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
+    mask_words(s0, s1, N);
+    if (mp.move == MC_EXCHANGE) sector_start(N, mp.k_up, chain, mp.call, mp.seed_lo, mp.seed_hi, s0, s1);
+    cx<T> th[JPL];
+    init_angles<T, JPL>(md, md.W, s0, s1, lane, th);
+    T lane_lc = logcosh_re(th[0]);
+#pragma unroll
+    for (int u = 1; u < JPL; u++) lane_lc += logcosh_re(th[u]);
+    const cx<T>* __restrict__ Wl = md.W + lane;
+    const uint64_t n_therm = (uint64_t)mp.therm * N;
+    const uint64_t n_sweep = (uint64_t)mp.sweeps * (mp.move == MC_FAST_SWEEP ? 1 : N);
+    const uint64_t total = n_therm + (uint64_t)mp.steps_per_chain * n_sweep;
+    uint64_t next_rec = n_therm + n_sweep;
+    int step = 0;
+    uint32_t nacc = 0;
+    auto record = [&]() {
+        if (valid && lane == 0) {
+            const int64_t idx = (int64_t)step * mp.chains + chain;
+            spins_out[2 * idx] = s0;
+            spins_out[2 * idx + 1] = s1;
+        }
+        step++;
+        next_rec += n_sweep;
+    };
+    while (step < mp.steps_per_chain && next_rec == 0) record();
+    for (uint64_t k = 0; k < total; k++) {
+        uint64_t n0, n1;
+        uint32_t uw;
+        propose_move(mp.move, mp.tries, N, chain, k, mp.call, mp.seed_lo, mp.seed_hi, s0, s1, n0, n1, uw);
+        cx<T> thn[JPL];
+#pragma unroll
+        for (int u = 0; u < JPL; u++) thn[u] = th[u];
+        T vis = T(0);
+        uint64_t d0 = s0 ^ n0, d1 = s1 ^ n1;
+        while (d0 | d1) {
+            const int i = d0 ? __builtin_ctzll(d0) : 64 + __builtin_ctzll(d1);
+            if (d0) d0 &= d0 - 1u; else d1 &= d1 - 1u;
+            const T f = T(-2) * spin_of<T>(s0, s1, i);
+#pragma unroll
+            for (int u = 0; u < JPL; u++) thn[u] = thn[u] + f * Wl[i * ld + 16 * u];
+            if (md.a) vis += f * md.a[i].re;
+        }
+        T lnew = logcosh_re(thn[0]);
+#pragma unroll
+        for (int u = 1; u < JPL; u++) lnew += logcosh_re(thn[u]);
+        const T delta = row_sum16(lnew - lane_lc) + vis;
+        const bool accept = unpack_u(pack_u(uw, T(0)), T(0)) <= ratio_of(delta);
+        if (accept) {
+#pragma unroll
+            for (int u = 0; u < JPL; u++) th[u] = thn[u];
+            lane_lc = lnew;
+            s0 = n0; s1 = n1;
+            nacc++;
+        }
+        while (step < mp.steps_per_chain && k + 1 == next_rec) record();
+    }
+    if (lane == 0 && counts && valid) {
+        counts[2 * chain] = nacc;
+        counts[2 * chain + 1] = (uint32_t)(total - nacc);
+    }
+}
+
 int max_hidden_units() { return 16 * 16; }
 int rbm_padded_width(int M) { return 16 * pick_jpl(M); }
 
@@ -1119,6 +1195,24 @@ hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, in
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_mc_moves(const RbmDev<T>& md, const McParams& mp, uint64_t* spins, uint32_t* counts, hipStream_t stream) {
+    const int jpl = pick_jpl(md.M);
+    const dim3 block(256), grid((mp.chains + 15) / 16);
+#define RBM_MOVES_CASE(J)                                                                                       \
+    case J:                                                                                                     \
+        hipLaunchKernelGGL((k_mc_rbm_moves<T, J>), grid, block, 0, stream, md, mp, spins, counts);              \
+        break;
+    switch (jpl) {
+        RBM_JPL_LIST(RBM_MOVES_CASE)
+        default: return hipErrorInvalidValue;
+    }
+#undef RBM_MOVES_CASE
+    return hipGetLastError();
+}
+
+template hipError_t launch_mc_moves<float>(const RbmDev<float>&, const McParams&, uint64_t*, uint32_t*, hipStream_t);
+template hipError_t launch_mc_moves<double>(const RbmDev<double>&, const McParams&, uint64_t*, uint32_t*, hipStream_t);
 template hipError_t launch_mc<float>(const RbmDev<float>&, const McParams&, uint64_t*, uint32_t*, bool, size_t, hipStream_t);
 template hipError_t launch_mc<double>(const RbmDev<double>&, const McParams&, uint64_t*, uint32_t*, bool, size_t, hipStream_t);
 template hipError_t launch_eval<float>(const RbmDev<float>&, const OpDev&, int, int64_t, const uint64_t*, uint64_t, const Records&, bool, size_t, int, hipStream_t);

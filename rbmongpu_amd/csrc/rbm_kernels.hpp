@@ -27,12 +27,18 @@ struct Records {
     void* tanh;           // cx<T> per sample x M
 };
 
+// move: MC_SINGLE_FLIP, MC_EXCHANGE (total S_z conserved; k_up spins up) or
+// MC_FAST_SWEEP (tries random masks ANDed; one proposal per sweep)
+enum { MC_SINGLE_FLIP = 0, MC_EXCHANGE = 1, MC_FAST_SWEEP = 2 };
 struct McParams {
     int chains;
     int steps_per_chain;
     int sweeps;
     int therm;
     uint32_t seed_lo, seed_hi, call;
+    int move = MC_SINGLE_FLIP;
+    int tries = 1;
+    int k_up = 0;
 };
 
 constexpr int SC_STRIDE = 4;
@@ -63,6 +69,9 @@ int max_hidden_units();
 // hidden units rounded up to 16 lanes x the compiled units-per-lane count
 int rbm_padded_width(int M);
 
+// Metropolis chains with the exchange or fast-sweep move (rbm_kernels.hip)
+template <typename T>
+hipError_t launch_mc_moves(const RbmDev<T>& md, const McParams& mp, uint64_t* spins, uint32_t* counts, hipStream_t stream);
 template <typename T>
 hipError_t launch_mc(const RbmDev<T>& md, const McParams& mp, uint64_t* spins, uint32_t* counts, bool wlds,
                      size_t lds_bytes, hipStream_t stream);

@@ -48,3 +48,38 @@ def eval_bytes(M, n, tanh_bytes=8, with_tanh=True):
     """Algorithmic HBM bytes of evaluating n recorded samples: read 16-byte
     configurations, write log psi and E_loc (16 B each) and M my_tanh values."""
     return n * (16 + 32 + (M * tanh_bytes if with_tanh else 0))
+
+
+def deep_forward_mfma_flops(N, sizes):
+    """Matrix-core FLOP of one dense PsiDeep forward pass: 4 in out for the
+    first layer (real spins times complex weights) and 8 in out for every
+    later complex layer (four real products)."""
+    flop, prev = 0, N
+    for l, size in enumerate(sizes):
+        flop += (4 if l == 0 else 8) * prev * size
+        prev = size
+    return flop
+
+
+def deep_forward_valu_flops(sizes):
+    """Vector FLOP of one forward pass: complex my_logcosh per unit and the
+    final weighted sum (6 + 2 per unit of the last layer)."""
+    return LOGCOSH * sum(sizes) + 8 * sizes[-1]
+
+
+def deep_mc_launch_flops(N, sizes, chains, proposals_per_chain):
+    """Matrix-core FLOP of one PsiDeep Metropolis launch: one full forward pass
+    per proposal and one at the start of each chain."""
+    return chains * (proposals_per_chain + 1) * deep_forward_mfma_flops(N, sizes)
+
+
+def s_matrix_flops(P, samples):
+    """FP32 matrix-core FLOP of the centred S-matrix Gram GEMM on real rows of
+    2P values per sample (both triangles)."""
+    return 2.0 * (2 * P) ** 2 * samples
+
+
+def cholesky_flops(n):
+    """Real FLOP of a complex Cholesky factorisation: n^3 / 3 complex
+    multiply-adds of 8 real FLOP."""
+    return 8.0 * n ** 3 / 3.0

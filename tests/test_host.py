@@ -154,8 +154,9 @@ def test_ensembles_host_side():
     assert mc2.num_steps == mc.num_steps
     with pytest.raises(ValueError, match="multiple"):
         R.MonteCarloLoop(100, 1, 1, 7, True)
-    with pytest.raises(NotImplementedError):
-        mc.set_fast_sweep(2)
+    mc.set_fast_sweep(2)              # move settings are host state; copies keep them
+    mc.set_total_z_symmetry(0)
+    R.MonteCarloLoop(mc)
     ex = R.ExactSummation(10, True)
     assert ex.num_steps == 1024
     ex.set_total_z_symmetry(0)

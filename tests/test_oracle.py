@@ -358,6 +358,39 @@ def test_metropolis_stationary_distribution():
     assert mc.acceptances + mc.rejections == chains * (10 + per * 2) * N
 
 
+@pytest.mark.parametrize("move", ["exchange", "fast_sweep"])
+def test_metropolis_moves_stationary_distribution(move):
+    """The exchange move keeps every sample in its total S_z sector and visits
+    the sector's configurations with frequencies |psi|^2 / Z_sector; the fast
+    sweep visits the whole basis with |psi|^2.  Same z-score test as above."""
+    N, M = 6, 4
+    b, W, _ = random_rbm(N, M, seed=63, scale=0.6)
+    psi = O.RBM(b, W)
+    p = np.abs(O.psi_vector(psi)) ** 2
+    chains, per = 256, 200
+    if move == "exchange":
+        sector = 0
+        up = np.array([bin(i).count("1") for i in range(2 ** N)])
+        p = np.where(2 * up - N == sector, p, 0.0)
+        mc = O.MonteCarlo(N, chains * per, 1, 10, chains, seed=8, total_z=sector)
+    else:
+        mc = O.MonteCarlo(N, chains * per, 3, 10, chains, seed=8, fast_sweep=2)
+    p /= p.sum()
+    spins, _, _, _ = O.records(psi, None, mc)
+    idx = spins[:, 0].astype(np.int64).reshape(per, chains)
+    if move == "exchange":
+        assert np.all(p[idx] > 0)               # never leaves the sector
+    freq = np.stack([np.bincount(idx[:, c], minlength=2 ** N) / per for c in range(chains)])
+    mean, se = freq.mean(axis=0), freq.std(axis=0, ddof=1) / np.sqrt(chains)
+    # rare configurations that no chain visited: floor with the binomial error
+    se = np.maximum(se, np.sqrt(p * (1 - p) / (chains * per)))
+    z = np.abs(mean - p) / np.maximum(se, 1e-12)
+    assert z.max() < 4.5, z
+    per_sweep = N if move == "exchange" else 1
+    sweeps = 1 if move == "exchange" else 3
+    assert mc.acceptances + mc.rejections == chains * (10 * N + per * sweeps * per_sweep)
+
+
 # ---------------------------------------------------------------- PsiDeep
 @pytest.mark.parametrize("ti", [False, True])
 def test_deep_log_psi_closed_form(ti):
