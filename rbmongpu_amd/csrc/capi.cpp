@@ -360,8 +360,19 @@ static DeepMfmaDev deep_mfma_dev(Ctx& c, rbmhip_psi* psi) {
     }
     md.final_off = vo;
     vo += 2 * md.out_p[L - 1];
+    md.fwd_total = fo;
+    // backward fragments: W_{l+1} itself, units of layer l x units of layer l+1
+    for (int l = 0; l + 1 < L; l++) {
+        md.bfrag_off[l] = fo;
+        fo += (md.out_p[l] / 16) * ((md.out_p[l + 1] + 31) / 32) * 2 * 64;
+    }
     md.frag_total = fo;
     md.vec_total = vo;
+    int ro = 0;
+    for (int l = 0; l < L; l++) { md.rec_x[l] = ro; ro += md.out_p[l]; }
+    for (int l = 0; l < L; l++) { md.rec_d[l] = ro; ro += md.out_p[l]; }
+    md.rec_lp = ro;
+    md.rec_stride = ro + 1;
     RBM_REQUIRE(deep_mfma_lds_bytes(md) <= c.max_lds,
                 "the bfloat16 layers need " + std::to_string(deep_mfma_lds_bytes(md)) + " bytes of LDS");
     DevModel& d = psi->dev[c.device];
@@ -370,10 +381,12 @@ static DeepMfmaDev deep_mfma_dev(Ctx& c, rbmhip_psi* psi) {
         std::vector<float> vec((size_t)vo, 0.0f);
         int64_t woff = 0, boff = 0;
         prev = psi->N;
+        std::vector<std::vector<cd>> dense((size_t)L);
         for (int l = 0; l < L; l++) {
             const int size = psi->sizes[l], C = psi->conn_counts[l];
             // dense W[i][j] (inputs x units) from the connection table
-            std::vector<cd> Wd((size_t)prev * size, cd(0, 0));
+            std::vector<cd>& Wd = dense[(size_t)l];
+            Wd.assign((size_t)prev * size, cd(0, 0));
             for (int cc = 0; cc < C; cc++)
                 for (int j = 0; j < size; j++) {
                     const int64_t slot = woff + (int64_t)cc * size + j;
@@ -407,6 +420,27 @@ static DeepMfmaDev deep_mfma_dev(Ctx& c, rbmhip_psi* psi) {
         for (int j = 0; j < psi->sizes[L - 1]; j++) {
             vec[(size_t)md.final_off + j] = (float)psi->final_weights[j].real();
             vec[(size_t)md.final_off + md.out_p[L - 1] + j] = (float)psi->final_weights[j].imag();
+        }
+        for (int l = 0; l + 1 < L; l++) {
+            // A[m][k] = W_{l+1}[m][k]: m a unit of layer l, k a unit of layer l+1
+            const std::vector<cd>& Wd = dense[(size_t)l + 1];
+            const int rows = psi->sizes[l], cols = psi->sizes[l + 1];
+            const int MT = md.out_p[l] / 16, KB = (md.out_p[l + 1] + 31) / 32;
+            for (int mt = 0; mt < MT; mt++)
+                for (int kb = 0; kb < KB; kb++)
+                    for (int part = 0; part < 2; part++)
+                        for (int lane = 0; lane < 64; lane++)
+                            for (int v = 0; v < 8; v++) {
+                                const int m = mt * 16 + (lane & 15), g = lane >> 4;
+                                const int k = kb * 32 + (v < 4 ? 4 * g + v : 16 + 4 * g + (v - 4));
+                                float w = 0.0f;
+                                if (m < rows && k < cols) {
+                                    const cd z = Wd[(size_t)m * cols + k];
+                                    w = (float)(part == 0 ? z.real() : z.imag());
+                                }
+                                const size_t at = ((size_t)md.bfrag_off[l] + (size_t)((mt * KB + kb) * 2 + part) * 64 + lane) * 8 + v;
+                                frag[at] = bf16_bits(w);
+                            }
         }
         d.mfrag.ensure(frag.size() * sizeof(uint16_t));
         RBM_HIPCHK(hipMemcpy(d.mfrag.p, frag.data(), frag.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -1013,6 +1047,93 @@ static std::vector<cd> ok_contract(Ctx& c, int N, int M, int64_t n, const void* 
     return h;
 }
 
+// BF16 PsiDeep: sum_s coef(s) conj(O_k(s)) in parameter order from gradient
+// records (x_l, delta_l, log psi per configuration, MFMA backward pass) and one
+// FP32 MFMA GEMM per layer over the samples, G_l = sum_s x_{l-1}(s) (x)
+// conj(coef_s) delta_l(s) on real rows, split over K and accumulated in FP64:
+// the O_k rows (num_params - 2N per sample) are never formed.
+static bool deep_mfma_grad_fits(Ctx& c, rbmhip_psi* psi) {
+    if (!deep_mfma_fits(psi, nullptr)) return false;
+    const DeepMfmaDev md = deep_mfma_dev(c, psi);
+    return deep_mfma_lds_bytes(md, true) <= c.max_lds;
+}
+
+static std::vector<cd> deep_mfma_contract(Ctx& c, rbmhip_psi* psi, int64_t n, const uint64_t* spins, const cx<double>* coef) {
+    RBM_REQUIRE(spins != nullptr, "PsiDeep O_k needs the recorded configurations");
+    const DeepMfmaDev md = deep_mfma_dev(c, psi);
+    const int L = md.L, N = psi->N;
+    const int SPLITS = 32, KQ = 16 * SPLITS;
+    auto pad64 = [](int x) { return (x + 63) / 64 * 64; };
+    std::vector<int> in(L + 1), out(L + 1), ldx(L + 1), ldy(L + 1);
+    int mx = 0, my = 0;
+    for (int l = 0; l <= L; l++) {
+        in[l] = l == 0 ? N : psi->sizes[l - 1];
+        out[l] = l < L ? psi->sizes[l] : 1;
+        ldx[l] = pad64(2 * in[l] + (l < L ? 1 : 0));
+        ldy[l] = pad64(2 * out[l]);
+        mx = std::max(mx, ldx[l]);
+        my = std::max(my, ldy[l]);
+    }
+    const int64_t chunk = std::min<int64_t>(n, 1 << 18);
+    const int64_t rows_max = (chunk + KQ - 1) / KQ * KQ;
+    cx<float>* rec = reinterpret_cast<cx<float>*>(c.buf("dg_records", (size_t)chunk * md.rec_stride * sizeof(cx<float>)));
+    float* X = reinterpret_cast<float*>(c.buf("dg_x", (size_t)rows_max * mx * sizeof(float)));
+    float* Y = reinterpret_cast<float*>(c.buf("dg_y", (size_t)rows_max * my * sizeof(float)));
+    float* G = reinterpret_cast<float*>(c.buf("dg_g", (size_t)mx * my * sizeof(float)));
+    float* part = reinterpret_cast<float*>(c.buf("dg_part", (size_t)SPLITS * mx * my * sizeof(float)));
+    std::vector<size_t> goff(L + 2, 0);
+    for (int l = 0; l <= L; l++) goff[l + 1] = goff[l] + (size_t)ldx[l] * ldy[l];
+    double* Gd = reinterpret_cast<double*>(c.buf("dg_g64", goff[L + 1] * sizeof(double)));
+    c.launch("deep_grad", [&] { return launch_zero((int64_t)(2 * goff[L + 1]), reinterpret_cast<float*>(Gd), c.stream); });
+    for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+        const int64_t cnt = std::min<int64_t>(chunk, n - s0), rows = (cnt + KQ - 1) / KQ * KQ;
+        c.launch("deep_grad", [&] { return launch_deep_mfma_grad(md, cnt, spins + 2 * s0, rec, c.num_cus, c.stream); });
+        for (int l = 0; l <= L; l++) {
+            c.launch("deep_grad_rows", [&] {
+                return launch_deep_grad_rows(md, l, in[l], out[l], 0, cnt, rows, spins + 2 * s0, rec, coef + s0,
+                                             psi->ref_compat_final, X, ldx[l], Y, ldy[l], c.stream);
+            });
+            GemmEpiArgs e{};
+            c.launch("deep_grad_gemm", [&] {
+                return launch_gemm_splitk(EPI_STORE, ldx[l], ldy[l], (int)rows, X, 1, ldx[l], Y, ldy[l], 1, G, ldy[l], e,
+                                          SPLITS, part, c.stream);
+            });
+            c.launch("deep_grad_gemm", [&] { return launch_accum_f64((int64_t)ldx[l] * ldy[l], G, Gd + goff[l], c.stream); });
+        }
+    }
+    std::vector<double> h(goff[L + 1]);
+    c.d2h(h.data(), Gd, h.size() * sizeof(double));
+    c.sync();
+    std::vector<cd> one((size_t)psi->num_params(), cd(0, 0));
+    int64_t p = 2 * (int64_t)N, woff = 0;
+    for (int l = 0; l < L; l++) {
+        const double* g = h.data() + goff[l];
+        const int I = in[l], J = out[l], ly = ldy[l];
+        for (int j = 0; j < J; j++) one[(size_t)(p + j)] = std::conj(cd(g[(size_t)(2 * I) * ly + j], g[(size_t)(2 * I) * ly + J + j]));
+        p += J;
+        const int C = psi->conn_counts[l];
+        for (int cc = 0; cc < C; cc++)
+            for (int j = 0; j < J; j++) {
+                const int i = (int)psi->conns[(size_t)(woff + (int64_t)cc * J + j)];
+                const double rr = g[(size_t)i * ly + j], ii = g[(size_t)(I + i) * ly + J + j];
+                const double ri = g[(size_t)i * ly + J + j], ir = g[(size_t)(I + i) * ly + j];
+                one[(size_t)(p + (int64_t)cc * J + j)] = std::conj(cd(rr - ii, ri + ir));
+            }
+        p += (int64_t)C * J;
+        woff += (int64_t)C * J;
+    }
+    {
+        const double* g = h.data() + goff[L];
+        const int I = in[L], ly = ldy[L];
+        for (int j = 0; j < I; j++) {
+            const double rr = g[(size_t)j * ly], ri = g[(size_t)j * ly + 1];
+            const double ir = g[(size_t)(I + j) * ly], ii = g[(size_t)(I + j) * ly + 1];
+            one[(size_t)(p + j)] = std::conj(cd(rr - ii, ri + ir));
+        }
+    }
+    return one;
+}
+
 // Map the contraction R of one coefficient vector to parameter order
 // [alpha (N), beta (N), b (M), W (N x M), a (N)]:  entries are
 // sum_s coef_s conj(O_k(s)) (Psi.hpp:165-202).
@@ -1036,6 +1157,7 @@ static std::vector<cd> contract_params(Ctx& c, rbmhip_psi* psi, int64_t n, const
         rbm_params_from_R(psi, R.data(), one);
         return one;
     }
+    if (psi->precision == RBMHIP_BF16 && deep_mfma_grad_fits(c, psi)) return deep_mfma_contract(c, psi, n, spins, coef);
     // PsiDeep: O_k rows of a chunk of the recorded configurations are
     // materialised by the deep kernels and reduced column-wise; chunks keep the
     // row buffer at most 4 GiB and their FP64 sums are added on the host.

@@ -41,9 +41,9 @@ RBM_DI uint32_t spin_bf16(uint32_t bits, int q, int i, int N) {
 // Complex log psi (without the prefactor) of the configuration in this lane's
 // column; every lane of the column returns the same bits.  F: weight
 // fragments, V: biases and final weights, both in LDS.
-template <int KBMAX>
+template <int KBMAX, bool REC = false>
 RBM_DI cx<float> mfma_log_psi(const DeepMfmaDev& md, const uint4* __restrict__ F, const float* __restrict__ V,
-                              uint64_t s0, uint64_t s1, int lane) {
+                              uint64_t s0, uint64_t s1, int lane, cx<float>* row = nullptr) {
     const int g = lane >> 4;
     uint32_t xr[KBMAX][4], xi[KBMAX][4];
 #pragma unroll
@@ -91,6 +91,14 @@ RBM_DI cx<float> mfma_log_psi(const DeepMfmaDev& md, const uint4* __restrict__ F
                 cx<float> x[4];
 #pragma unroll
                 for (int v = 0; v < 4; v++) x[v] = logcosh(mk(ar[v], ai[v]));
+                if (REC && row) {
+                    // x_l and, in the delta slots until the backward pass, my_tanh(a_l)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        row[md.rec_x[l] + u0 + v] = x[v];
+                        row[md.rec_d[l] + u0 + v] = dlogcosh(mk(ar[v], ai[v]));
+                    }
+                }
                 if (last) {
                     const float* fr = V + md.final_off;
                     const float* fi = fr + md.out_p[l];
@@ -122,15 +130,101 @@ RBM_DI cx<float> mfma_log_psi(const DeepMfmaDev& md, const uint4* __restrict__ F
     return part;
 }
 
-// Copy the fragments and vectors into LDS (dynamic shared memory).
-RBM_DI void mfma_stage(const DeepMfmaDev& md, unsigned char* smem, const uint4** F, const float** V) {
-    uint4* f = reinterpret_cast<uint4*>(smem);
-    for (int i = threadIdx.x; i < md.frag_total; i += blockDim.x) f[i] = md.frag[i];
-    float* v = reinterpret_cast<float*>(f + md.frag_total);
+// Copy the vectors and the first nfrag fragments into LDS (dynamic shared
+// memory): vectors first, then the forward fragments, then the backward ones.
+RBM_DI void mfma_stage(const DeepMfmaDev& md, unsigned char* smem, int nfrag, const uint4** F, const float** V) {
+    float* v = reinterpret_cast<float*>(smem);
     for (int i = threadIdx.x; i < md.vec_total; i += blockDim.x) v[i] = md.vec[i];
+    uint4* f = reinterpret_cast<uint4*>(v + md.vec_total);
+    for (int i = threadIdx.x; i < nfrag; i += blockDim.x) f[i] = md.frag[i];
     __syncthreads();
     *F = f;
     *V = v;
+}
+
+// Back-propagation (PsiDeep.hpp:214-293) for the column's configuration after
+// mfma_log_psi<KBMAX, true>: delta_{L-1} = final * my_tanh(a_{L-1}), then
+// delta_l = (W_{l+1} delta_{l+1}) * my_tanh(a_l) on the backward fragments;
+// every lane reads back only the entries it wrote in the forward pass and
+// replaces my_tanh(a_l) by delta_l.  delta_{l+1} enters the MFMA as bfloat16.
+template <int KBMAX>
+RBM_DI void mfma_backward(const DeepMfmaDev& md, const uint4* __restrict__ F, const float* __restrict__ V,
+                          cx<float>* row, int lane) {
+    const int g = lane >> 4;
+    uint32_t dr[KBMAX][4], di[KBMAX][4];
+#pragma unroll
+    for (int kb = 0; kb < KBMAX; kb++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) { dr[kb][q] = 0u; di[kb][q] = 0u; }
+    {
+        const int l = md.L - 1, MT = md.out_p[l] >> 4;
+        const float* fr = V + md.final_off;
+        const float* fi = fr + md.out_p[l];
+#pragma unroll
+        for (int mt = 0; mt < 2 * KBMAX; mt++) {
+            if (mt < MT) {
+                const int u0 = mt * 16 + 4 * g;
+                cx<float> d[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    d[v] = mk(fr[u0 + v], fi[u0 + v]) * (row ? row[md.rec_d[l] + u0 + v] : mk(0.0f, 0.0f));
+                    if (row) row[md.rec_d[l] + u0 + v] = d[v];
+                }
+                const int q = (mt & 1) * 2;
+                dr[mt >> 1][q] = pack_bf16(d[0].re, d[1].re);
+                dr[mt >> 1][q + 1] = pack_bf16(d[2].re, d[3].re);
+                di[mt >> 1][q] = pack_bf16(d[0].im, d[1].im);
+                di[mt >> 1][q + 1] = pack_bf16(d[2].im, d[3].im);
+            }
+        }
+    }
+    for (int l = md.L - 2; l >= 0; l--) {
+        const int MT = md.out_p[l] >> 4, KB = (md.out_p[l + 1] + 31) >> 5;
+        const uint4* Fl = F + md.bfrag_off[l];
+        uint32_t nr[KBMAX][4], ni[KBMAX][4];
+#pragma unroll
+        for (int kb = 0; kb < KBMAX; kb++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) { nr[kb][q] = 0u; ni[kb][q] = 0u; }
+#pragma unroll
+        for (int mt = 0; mt < 2 * KBMAX; mt++) {
+            if (mt < MT) {
+                const int u0 = mt * 16 + 4 * g;
+                f32x4 ar = {0.0f, 0.0f, 0.0f, 0.0f}, ai = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kb = 0; kb < KBMAX; kb++) {
+                    if (kb < KB) {
+                        const uint4* fb = Fl + (size_t)((mt * KB + kb) * 2) * 64 + lane;
+                        const bf16x8 Ar = as_bf16x8(fb[0]);
+                        const bf16x8 Ai = as_bf16x8(fb[64]);
+                        const bf16x8 Br = as_bf16x8(make_uint4(dr[kb][0], dr[kb][1], dr[kb][2], dr[kb][3]));
+                        const bf16x8 Bi = as_bf16x8(make_uint4(di[kb][0], di[kb][1], di[kb][2], di[kb][3]));
+                        const bf16x8 Bn = as_bf16x8(make_uint4(di[kb][0] ^ 0x80008000u, di[kb][1] ^ 0x80008000u,
+                                                               di[kb][2] ^ 0x80008000u, di[kb][3] ^ 0x80008000u));
+                        ar = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Br, ar, 0, 0, 0);
+                        ar = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, Bn, ar, 0, 0, 0);
+                        ai = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, Br, ai, 0, 0, 0);
+                        ai = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Bi, ai, 0, 0, 0);
+                    }
+                }
+                cx<float> d[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    d[v] = mk(ar[v], ai[v]) * (row ? row[md.rec_d[l] + u0 + v] : mk(0.0f, 0.0f));
+                    if (row) row[md.rec_d[l] + u0 + v] = d[v];
+                }
+                const int q = (mt & 1) * 2;
+                nr[mt >> 1][q] = pack_bf16(d[0].re, d[1].re);
+                nr[mt >> 1][q + 1] = pack_bf16(d[2].re, d[3].re);
+                ni[mt >> 1][q] = pack_bf16(d[0].im, d[1].im);
+                ni[mt >> 1][q + 1] = pack_bf16(d[2].im, d[3].im);
+            }
+        }
+#pragma unroll
+        for (int kb = 0; kb < KBMAX; kb++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) { dr[kb][q] = nr[kb][q]; di[kb][q] = ni[kb][q]; }
+    }
 }
 
 // log psi and E_loc of given configurations (configs, or index_base + idx),
@@ -144,7 +238,7 @@ __global__ __launch_bounds__(512) void k_deep_mfma_eval(DeepMfmaDev md, OpDev op
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const uint4* F;
     const float* V;
-    mfma_stage(md, smem_raw, &F, &V);
+    mfma_stage(md, smem_raw, md.fwd_total, &F, &V);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
     const int waves = blockDim.x >> 6;
     for (int64_t base = ((int64_t)blockIdx.x * waves + wave) * 16; base < n; base += (int64_t)gridDim.x * waves * 16) {
@@ -199,7 +293,7 @@ __global__ __launch_bounds__(512) void k_deep_mfma_mc(DeepMfmaDev md, McParams m
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const uint4* F;
     const float* V;
-    mfma_stage(md, smem_raw, &F, &V);
+    mfma_stage(md, smem_raw, md.fwd_total, &F, &V);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
     const int waves = blockDim.x >> 6;
     const int chain = (blockIdx.x * waves + wave) * 16 + col;
@@ -265,8 +359,84 @@ __global__ __launch_bounds__(512) void k_deep_mfma_mc(DeepMfmaDev md, McParams m
     if (valid && lane < 16 && counts) { counts[2 * chain] = nacc; counts[2 * chain + 1] = nrej; }
 }
 
-size_t deep_mfma_lds_bytes(const DeepMfmaDev& md) {
-    return (size_t)md.frag_total * 16 + (size_t)md.vec_total * 4;
+// Gradient records of given configurations: the forward pass of log psi with
+// x_l and my_tanh(a_l) written to the configuration's row, then the backward
+// pass (16 configurations per wavefront, grid-stride over tiles).
+template <int KBMAX>
+__global__ __launch_bounds__(512) void k_deep_mfma_grad(DeepMfmaDev md, int64_t n, const uint64_t* __restrict__ configs,
+                                                        cx<float>* __restrict__ records) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const uint4* F;
+    const float* V;
+    mfma_stage(md, smem_raw, md.frag_total, &F, &V);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
+    const int waves = blockDim.x >> 6;
+    for (int64_t base = ((int64_t)blockIdx.x * waves + wave) * 16; base < n; base += (int64_t)gridDim.x * waves * 16) {
+        const int64_t idx = base + col;
+        const bool valid = idx < n;
+        uint64_t s0 = 0, s1 = 0;
+        cx<float>* row = nullptr;
+        if (valid) {
+            s0 = configs[2 * idx];
+            s1 = configs[2 * idx + 1];
+            mask_words(s0, s1, md.N);
+            row = records + idx * md.rec_stride;
+        }
+        const cx<float> lp = mfma_log_psi<KBMAX, true>(md, F, V, s0, s1, lane, row);
+        mfma_backward<KBMAX>(md, F, V, row, lane);
+        if (valid && lane < 16) row[md.rec_lp] = lp;
+    }
+}
+
+// Rows of the contraction GEMM G = X^T Y for one layer (see deep_kernels.hpp).
+__global__ __launch_bounds__(256) void k_deep_grad_rows(DeepMfmaDev md, int layer, int in, int out, int64_t s0, int64_t cnt,
+                                                        int64_t rows, const uint64_t* __restrict__ configs,
+                                                        const cx<float>* __restrict__ records,
+                                                        const cx<double>* __restrict__ coef, int lp_factor,
+                                                        float* __restrict__ X, int ldx, float* __restrict__ Y, int ldy) {
+    const int64_t totx = rows * ldx, toty = rows * ldy;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < totx + toty; t += (int64_t)gridDim.x * blockDim.x) {
+        const bool isx = t < totx;
+        const int64_t tt = isx ? t : t - totx;
+        const int ld = isx ? ldx : ldy;
+        const int64_t r = tt / ld;
+        const int c = (int)(tt - r * ld);
+        float v = 0.0f;
+        if (r < cnt) {
+            const int64_t s = s0 + r;
+            const cx<float>* row = records + s * md.rec_stride;
+            if (isx) {
+                if (layer == 0) {
+                    if (c < in) v = (float)spin_of<float>(configs[2 * s], configs[2 * s + 1], c);
+                    else if (c == 2 * in) v = 1.0f;
+                } else {
+                    const cx<float>* x = row + md.rec_x[layer - 1];
+                    if (c < in) v = x[c].re;
+                    else if (c < 2 * in) v = x[c - in].im;
+                    else if (c == 2 * in && layer < md.L) v = 1.0f;
+                }
+            } else {
+                const cx<double> cf = conj(coef[s]);
+                cx<double> y = mk(0.0, 0.0);
+                int j = -1;
+                if (layer < md.L) {
+                    if (c < 2 * out) {
+                        j = c < out ? c : c - out;
+                        y = cf * to_d(row[md.rec_d[layer] + j]);
+                    }
+                } else if (c < 2) {
+                    y = lp_factor ? cf * to_d(row[md.rec_lp]) : cf;
+                    j = 0;
+                }
+                if (j >= 0) v = (float)((layer < md.L ? c < out : c == 0) ? y.re : y.im);
+            }
+        }
+        (isx ? X : Y)[tt] = v;
+    }
+}
+
+size_t deep_mfma_lds_bytes(const DeepMfmaDev& md, bool grad) {
+    return (size_t)(grad ? md.frag_total : md.fwd_total) * 16 + (size_t)md.vec_total * 4;
 }
 
 int deep_mfma_kbmax(const DeepMfmaDev& md) {
@@ -277,6 +447,14 @@ int deep_mfma_kbmax(const DeepMfmaDev& md) {
     }
     return kb;
 }
+
+// K blocks of the widest layer, rounded up to an instantiated register budget
+#define RBM_DM_DISPATCH(kbm, X) \
+    if ((kbm) <= 2) { X(2) }    \
+    else if ((kbm) <= 4) { X(4) } \
+    else if ((kbm) <= 5) { X(5) } \
+    else if ((kbm) <= 6) { X(6) } \
+    else { X(8) }
 
 template <typename K>
 static hipError_t set_lds(K kernel, size_t lds) {
@@ -293,16 +471,13 @@ hipError_t launch_deep_mfma_eval(const DeepMfmaDev& md, const OpDev& op, int has
     int64_t blocks = (tiles + waves - 1) / waves;
     if (blocks > num_cus) blocks = num_cus;
     if (blocks < 1) blocks = 1;
-    hipError_t e;
-    if (kbm <= 4) {
-        if ((e = set_lds(&k_deep_mfma_eval<4>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((k_deep_mfma_eval<4>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, op, has_op, n,
-                           configs, index_base, rec);
-    } else {
-        if ((e = set_lds(&k_deep_mfma_eval<8>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((k_deep_mfma_eval<8>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, op, has_op, n,
-                           configs, index_base, rec);
-    }
+    hipError_t e = hipSuccess;
+#define RBM_DM_EVAL(KB)                                                                                            \
+    if ((e = set_lds(&k_deep_mfma_eval<KB>, lds)) != hipSuccess) return e;                                        \
+    hipLaunchKernelGGL((k_deep_mfma_eval<KB>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, op, has_op, n, \
+                       configs, index_base, rec);
+    RBM_DM_DISPATCH(kbm, RBM_DM_EVAL)
+#undef RBM_DM_EVAL
     return hipGetLastError();
 }
 
@@ -316,14 +491,41 @@ hipError_t launch_deep_mfma_mc(const DeepMfmaDev& md, const McParams& mp, uint64
     if (waves < 1) waves = 1;
     if (waves > 8) waves = 8;
     const int blocks = (wave_count + waves - 1) / waves;
-    hipError_t e;
-    if (kbm <= 4) {
-        if ((e = set_lds(&k_deep_mfma_mc<4>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((k_deep_mfma_mc<4>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, mp, spins, counts);
-    } else {
-        if ((e = set_lds(&k_deep_mfma_mc<8>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((k_deep_mfma_mc<8>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, mp, spins, counts);
-    }
+    hipError_t e = hipSuccess;
+#define RBM_DM_MC(KB)                                                                                              \
+    if ((e = set_lds(&k_deep_mfma_mc<KB>, lds)) != hipSuccess) return e;                                          \
+    hipLaunchKernelGGL((k_deep_mfma_mc<KB>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, mp, spins, counts);
+    RBM_DM_DISPATCH(kbm, RBM_DM_MC)
+#undef RBM_DM_MC
+    return hipGetLastError();
+}
+
+hipError_t launch_deep_mfma_grad(const DeepMfmaDev& md, int64_t n, const uint64_t* configs, cx<float>* records,
+                                 int num_cus, hipStream_t stream) {
+    const size_t lds = deep_mfma_lds_bytes(md, true);
+    const int kbm = deep_mfma_kbmax(md);
+    const int64_t tiles = (n + 15) / 16;
+    const int waves = 8;
+    int64_t blocks = (tiles + waves - 1) / waves;
+    if (blocks > num_cus) blocks = num_cus;
+    if (blocks < 1) blocks = 1;
+    hipError_t e = hipSuccess;
+#define RBM_DM_GRAD(KB)                                                                                            \
+    if ((e = set_lds(&k_deep_mfma_grad<KB>, lds)) != hipSuccess) return e;                                        \
+    hipLaunchKernelGGL((k_deep_mfma_grad<KB>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, n, configs, records);
+    RBM_DM_DISPATCH(kbm, RBM_DM_GRAD)
+#undef RBM_DM_GRAD
+    return hipGetLastError();
+}
+
+hipError_t launch_deep_grad_rows(const DeepMfmaDev& md, int layer, int in, int out, int64_t s0, int64_t cnt, int64_t rows,
+                                 const uint64_t* configs, const cx<float>* records, const cx<double>* coef,
+                                 int lp_factor, float* X, int ldx, float* Y, int ldy, hipStream_t stream) {
+    const int64_t total = rows * (ldx + ldy);
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_deep_grad_rows, dim3((unsigned)blocks), dim3(256), 0, stream, md, layer, in, out, s0, cnt, rows,
+                       configs, records, coef, lp_factor, X, ldx, Y, ldy);
     return hipGetLastError();
 }
 

@@ -112,6 +112,31 @@ def test_bf16_exact_energy_and_gradient():
     assert rel_err(g, gr) < 2e-2
 
 
+@pytest.mark.parametrize("shape,compat,estimator", [
+    ("two", False, "auto"), ("two", True, "auto"), ("three", False, "E_k"), ("odd", True, "hermitian")])
+def test_bf16_gradient_from_mfma_backward(shape, compat, estimator):
+    """The gradient contraction from MFMA back-propagation records and per-layer
+    GEMMs over the samples against the FP64 oracle on the bfloat16-rounded
+    weights (the remaining difference is the rounding of activations and of
+    delta in the backward products)."""
+    print(f"[bf16 gradient] {shape} compat={compat} {estimator}", flush=True)
+    N, M, C = {"two": (10, [20, 10], [10, 20]), "three": (9, [18, 12, 6], [9, 6, 12]),
+               "odd": (7, [13, 5], [7, 13])}[shape]
+    biases, conns, weights, final = layers(N, M, C, seed=len(M) + N)
+    tab = HAM.j1j2_chain(N)
+    psi = gpu_deep(N, biases, conns, weights, final, compat=compat)
+    ex = R.ExactSummation(N, True)
+    psi.normalize(ex)
+    g, E = R.ExpectationValue(True).gradient(psi, R.Operator(*tab, True), ex, estimator=estimator)
+    rw = [bf16c(w) for w in weights]
+    ref = O.Deep(N, biases, conns, rw, final, prefactor=psi.prefactor, ref_compat_final=compat)
+    gr, Er = O.gradient(ref, O.Operator(*tab), O.Exact(N), mode=0)
+    print(f"  rel err {rel_err(g, gr):.2e}", flush=True)
+    assert abs(E - Er) < 2e-3 * abs(Er)
+    assert rel_err(g, gr) < 2e-2
+    assert np.all(g[:2 * N] == 0)
+
+
 def test_bf16_sampler_visits_psi_squared():
     N, M, C = 8, [16, 8], [8, 16]
     biases, conns, weights, final = layers(N, M, C, seed=5, re=0.6, bias=0.3)
