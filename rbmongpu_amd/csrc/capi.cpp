@@ -1240,12 +1240,14 @@ static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, co
     float* G = reinterpret_cast<float*>(c.buf("s_gram", (size_t)ld2 * ld2 * sizeof(float)));
     double* Gd = reinterpret_cast<double*>(c.buf("s_gram64", (size_t)ld2 * ld2 * sizeof(double)));
     c.launch("s_matrix", [&] { return launch_zero((int64_t)2 * ld2 * ld2, reinterpret_cast<float*>(Gd), c.stream); });
+    c.launch("s_matrix", [&] { return launch_zero((int64_t)ld2 * ld2, G, c.stream); });
     for (int64_t s0 = 0; s0 < s.n; s0 += nb) {
         const int64_t cur = std::min<int64_t>(nb, s.n - s0), curp = (cur + 63) / 64 * 64;
         c.launch("s_matrix", [&] {
             return launch_o_realified<T>(s0, cur, curp, N, M, P, ld2, s.tanh, s.spins, w, dob, O, c.stream);
         });
         GemmEpiArgs e{};
+        e.lower_only = 1;
         c.launch("s_gram", [&] { return launch_gemm(EPI_STORE, ld2, ld2, (int)curp, O, 1, ld2, O, ld2, 1, G, ld2, e, c.stream); });
         c.launch("s_matrix", [&] { return launch_accum_f64((int64_t)ld2 * ld2, G, Gd, c.stream); });
     }

@@ -34,6 +34,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
     __shared__ double red[256];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    if (e.lower_only && n0 > m0) return;   // uniform per workgroup, before any barrier
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     // split K: workgroup z covers [z K / gridDim.z, (z + 1) K / gridDim.z) and
     // writes its partial product at C + z M ldc

@@ -43,6 +43,7 @@ struct GemmEpiArgs {
     int n_live, m_live;  // live columns / rows (N or H, real batch)
     float inv_batch;
     double* partials;    // EPI_LOGITS: one log-q partial per workgroup
+    int lower_only;      // compute only the 64 x 64 tiles with n0 <= m0 (symmetric products)
 };
 
 // C[m][n] = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn];

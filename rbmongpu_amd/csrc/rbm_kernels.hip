@@ -1149,12 +1149,17 @@ __global__ __launch_bounds__(256) void k_accum_f64(int64_t n, const float* __res
 
 __global__ __launch_bounds__(256) void k_s_assemble(int P, int ld2, double inv, double eps, const double* __restrict__ G,
                                                     cx<double>* __restrict__ S) {
+    // G = O^T O is symmetric and only its lower 64 x 64 tiles are computed:
+    // every entry read here has row >= column (G[k][P+l] is read as G[P+l][k]);
+    // the upper triangle of S is the conjugate of the lower one.
     const int64_t total = (int64_t)P * P;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int k = (int)(t / P), l = (int)(t % P);
+        const int r = (int)(t / P), c = (int)(t % P);
+        const int k = r >= c ? r : c, l = r >= c ? c : r;
         const double rr = G[(int64_t)k * ld2 + l], ii = G[(int64_t)(P + k) * ld2 + P + l];
-        const double ri = G[(int64_t)k * ld2 + P + l], ir = G[(int64_t)(P + k) * ld2 + l];
-        S[t] = mk((rr + ii) * inv + (k == l ? eps : 0.0), (ri - ir) * inv);
+        const double ri = G[(int64_t)(P + l) * ld2 + k], ir = G[(int64_t)(P + k) * ld2 + l];
+        const double im = (ri - ir) * inv;
+        S[t] = mk((rr + ii) * inv + (k == l ? eps : 0.0), r >= c ? im : -im);
     }
 }
 

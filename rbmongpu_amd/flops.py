@@ -74,9 +74,10 @@ def deep_mc_launch_flops(N, sizes, chains, proposals_per_chain):
 
 
 def s_matrix_flops(P, samples):
-    """FP32 matrix-core FLOP of the centred S-matrix Gram GEMM on real rows of
-    2P values per sample (both triangles)."""
-    return 2.0 * (2 * P) ** 2 * samples
+    """FP32 matrix-core FLOP of the centred S-matrix Gram product on real rows
+    of 2P values per sample: the symmetric product needs its lower triangle,
+    2 (2P)(2P + 1) / 2 FLOP per sample."""
+    return float(2 * P) * (2 * P + 1) * samples
 
 
 def cholesky_flops(n):
