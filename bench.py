@@ -76,7 +76,8 @@ def kernel_source_sha():
     for the kernels it was taken on."""
     import hashlib
     h = hashlib.sha256()
-    for f in ("rbm_kernels.hip", "rbm_device.hpp", "rbm_kernels.hpp", "made_kernels.hip", "made_kernels.hpp"):
+    for f in ("rbm_kernels.hip", "rbm_device.hpp", "rbm_kernels.hpp", "made_kernels.hip", "made_kernels.hpp",
+              "deep_mfma.hip", "deep_kernels.hpp"):
         h.update(open(os.path.join(ROOT, "rbmongpu_amd", "csrc", f), "rb").read())
     return h.hexdigest()[:16]
 
@@ -405,7 +406,8 @@ def run_c4(args, R, HAM, F):
         "acceptance_rate": mc.acceptance_rate,
         "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
         "roofline": {"bound": "mfma" if prec == 16 else "valu", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": "k_deep_mfma_mc"
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": stored_traffic("k_deep_mfma_mc@c4") if prec == 16 else None, "kernel": "k_deep_mfma_mc"
                      if prec == 16 else "k_deep_mc", "flop_per_launch": flop, "avg_launch_ms": mc_avg,
                      "valu_flop_per_launch": CH * (THERM + spc) * N * F.deep_forward_valu_flops(M),
                      "algorithmic_bytes_per_launch": F.mc_launch_bytes(CH, spc)},

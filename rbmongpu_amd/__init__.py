@@ -34,6 +34,7 @@ except ImportError as exc:  # pragma: no cover - exercised only without a build
 from .json_numpy import NumpyEncoder, NumpyDecoder  # noqa: E402
 from .new_neural_network import new_neural_network, new_deep_neural_network, complex_noise, real_noise  # noqa: E402,F401
 from . import hamiltonians  # noqa: E402,F401
+from . import gradient_descent  # noqa: E402,F401
 
 
 def _normalize_psi(self, exact_summation):
