@@ -130,6 +130,140 @@ RBM_DI cx<float> mfma_log_psi(const DeepMfmaDev& md, const uint4* __restrict__ F
     return part;
 }
 
+// Forward pass of a two-layer network whose padded shape is known at compile
+// time: layer 0 has MT0 output tiles over KB0 K blocks, layer 1 MT1 tiles over
+// KB1 = MT0 / 2 K blocks, and K block k of layer 1 is made of layer-0 tiles
+// 2k and 2k + 1.  The pass is software-pipelined over pairs of layer-0 tiles:
+// step k issues the layer-0 products of pair k + 1 and the layer-1 products of
+// K block k - 1, then evaluates my_logcosh of pair k, so matrix-core work is
+// always in flight under the vector work; a scheduling barrier closes each
+// step so that fragment loads are not all hoisted.  Same arithmetic, and the
+// same order of every accumulation, as mfma_log_psi.
+template <int MT0, int KB0, int MT1>
+RBM_DI cx<float> mfma_log_psi_2l(const DeepMfmaDev& md, const uint4* __restrict__ F, const float* __restrict__ V,
+                                 uint64_t s0, uint64_t s1, int lane) {
+    static_assert(MT0 % 2 == 0, "pairs of layer-0 tiles form layer-1 K blocks");
+    constexpr int NP = MT0 / 2;   // pairs = layer-1 K blocks
+    const int g = lane >> 4;
+    uint32_t xs[KB0][4];
+#pragma unroll
+    for (int kb = 0; kb < KB0; kb++) {
+        const int i0 = kb * 32 + 8 * g;
+        const uint32_t bits = i0 < 64 ? (uint32_t)(s0 >> i0) & 0xFFu : (i0 < 128 ? (uint32_t)(s1 >> (i0 - 64)) & 0xFFu : 0u);
+#pragma unroll
+        for (int q = 0; q < 4; q++) xs[kb][q] = spin_bf16(bits, q, i0 + 2 * q, md.N);
+    }
+    const uint4* F0 = F + md.frag_off[0];
+    const uint4* F1 = F + md.frag_off[1];
+    const float* b0r = V + md.vec_off[0];
+    const float* b0i = b0r + md.out_p[0];
+    const float* b1r = V + md.vec_off[1];
+    const float* b1i = b1r + md.out_p[1];
+    // layer-0 products of tiles 2p, 2p + 1 into (ar, ai)
+    auto layer0_pair = [&](int p, f32x4 (&ar)[2], f32x4 (&ai)[2]) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            ar[i] = *reinterpret_cast<const f32x4*>(b0r + (2 * p + i) * 16 + 4 * g);
+            ai[i] = *reinterpret_cast<const f32x4*>(b0i + (2 * p + i) * 16 + 4 * g);
+        }
+#pragma unroll
+        for (int kb = 0; kb < KB0; kb++) {
+            const bf16x8 B = as_bf16x8(make_uint4(xs[kb][0], xs[kb][1], xs[kb][2], xs[kb][3]));
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const uint4* fb = F0 + (size_t)(((2 * p + i) * KB0 + kb) * 2) * 64 + lane;
+                ar[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(fb[0]), B, ar[i], 0, 0, 0);
+                ai[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(fb[64]), B, ai[i], 0, 0, 0);
+            }
+        }
+    };
+    f32x4 yr[MT1], yi[MT1];   // layer-1 accumulators
+#pragma unroll
+    for (int mt = 0; mt < MT1; mt++) {
+        yr[mt] = *reinterpret_cast<const f32x4*>(b1r + mt * 16 + 4 * g);
+        yi[mt] = *reinterpret_cast<const f32x4*>(b1i + mt * 16 + 4 * g);
+    }
+    // layer-1 products of K block kb with its packed inputs
+    auto layer1_block = [&](int kb, const uint32_t (&xr)[4], const uint32_t (&xi)[4]) {
+        const bf16x8 Br = as_bf16x8(make_uint4(xr[0], xr[1], xr[2], xr[3]));
+        const bf16x8 Bi = as_bf16x8(make_uint4(xi[0], xi[1], xi[2], xi[3]));
+        const bf16x8 Bn = as_bf16x8(make_uint4(xi[0] ^ 0x80008000u, xi[1] ^ 0x80008000u, xi[2] ^ 0x80008000u,
+                                               xi[3] ^ 0x80008000u));
+#pragma unroll
+        for (int mt = 0; mt < MT1; mt++) {
+            const uint4* fb = F1 + (size_t)((mt * NP + kb) * 2) * 64 + lane;
+            const bf16x8 Ar = as_bf16x8(fb[0]);
+            const bf16x8 Ai = as_bf16x8(fb[64]);
+            yr[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Br, yr[mt], 0, 0, 0);
+            yi[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, Br, yi[mt], 0, 0, 0);
+            yr[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ai, Bn, yr[mt], 0, 0, 0);
+            yi[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ar, Bi, yi[mt], 0, 0, 0);
+        }
+    };
+    f32x4 cr[2], ci[2];
+    uint32_t pr[4], pi[4];   // packed activations of the previous pair (K block p - 1)
+    layer0_pair(0, cr, ci);
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        f32x4 nr[2], ni[2];
+        if (p + 1 < NP) layer0_pair(p + 1, nr, ni);
+        if (p > 0) layer1_block(p - 1, pr, pi);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            cx<float> x[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) x[v] = logcosh(mk(cr[i][v], ci[i][v]));
+            pr[2 * i] = pack_bf16(x[0].re, x[1].re);
+            pr[2 * i + 1] = pack_bf16(x[2].re, x[3].re);
+            pi[2 * i] = pack_bf16(x[0].im, x[1].im);
+            pi[2 * i + 1] = pack_bf16(x[2].im, x[3].im);
+        }
+        if (p + 1 < NP) {
+#pragma unroll
+            for (int i = 0; i < 2; i++) { cr[i] = nr[i]; ci[i] = ni[i]; }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    layer1_block(NP - 1, pr, pi);
+    cx<float> part = mk(0.0f, 0.0f);
+    const float* fr = V + md.final_off;
+    const float* fi = fr + md.out_p[1];
+#pragma unroll
+    for (int mt = 0; mt < MT1; mt++) {
+        const int u0 = mt * 16 + 4 * g;
+        const f32x4 wr = *reinterpret_cast<const f32x4*>(fr + u0);
+        const f32x4 wi = *reinterpret_cast<const f32x4*>(fi + u0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) part = part + logcosh(mk(yr[mt][v], yi[mt][v])) * mk(wr[v], wi[v]);
+    }
+    part.re += __shfl_xor(part.re, 16);
+    part.im += __shfl_xor(part.im, 16);
+    part.re += __shfl_xor(part.re, 32);
+    part.im += __shfl_xor(part.im, 32);
+    return part;
+}
+
+// The forward pass a kernel instantiation uses: any shape within KBMAX blocks,
+// or one compile-time two-layer shape.
+template <int KBMAX>
+struct FwdAny {
+    RBM_DI static cx<float> run(const DeepMfmaDev& md, const uint4* F, const float* V, uint64_t s0, uint64_t s1, int lane) {
+        return mfma_log_psi<KBMAX>(md, F, V, s0, s1, lane);
+    }
+};
+template <int MT0, int KB0, int MT1>
+struct Fwd2 {
+    RBM_DI static cx<float> run(const DeepMfmaDev& md, const uint4* F, const float* V, uint64_t s0, uint64_t s1, int lane) {
+        return mfma_log_psi_2l<MT0, KB0, MT1>(md, F, V, s0, s1, lane);
+    }
+};
+// the deep-network tier, 40 -> 160 -> 80: in_p 64, 160 and out_p 160, 80
+using FwdC4 = Fwd2<10, 2, 5>;
+
+static inline bool is_c4_shape(const DeepMfmaDev& md) {
+    return md.L == 2 && md.in_p[0] == 64 && md.out_p[0] == 160 && md.in_p[1] == 160 && md.out_p[1] == 80;
+}
+
 // Copy the vectors and the first nfrag fragments into LDS (dynamic shared
 // memory): vectors first, then the forward fragments, then the backward ones.
 RBM_DI void mfma_stage(const DeepMfmaDev& md, unsigned char* smem, int nfrag, const uint4** F, const float** V) {
@@ -231,7 +365,7 @@ RBM_DI void mfma_backward(const DeepMfmaDev& md, const uint4* __restrict__ F, co
 // 16 per wavefront, grid-stride over tiles.  Off-diagonal terms with a zero
 // matrix element are skipped: each column walks its own list of non-zero
 // terms and the wavefront runs as many forward passes as its longest list.
-template <int KBMAX>
+template <class FWD>
 __global__ __launch_bounds__(512) void k_deep_mfma_eval(DeepMfmaDev md, OpDev op, int has_op, int64_t n,
                                                          const uint64_t* __restrict__ configs, uint64_t index_base,
                                                          Records rec) {
@@ -250,7 +384,7 @@ __global__ __launch_bounds__(512) void k_deep_mfma_eval(DeepMfmaDev md, OpDev op
             else { s0 = index_base + (uint64_t)idx; s1 = 0; }
             mask_words(s0, s1, md.N);
         }
-        const cx<float> lp = mfma_log_psi<KBMAX>(md, F, V, s0, s1, lane);
+        const cx<float> lp = FWD::run(md, F, V, s0, s1, lane);
         cx<double> E = mk(0.0, 0.0);
         if (has_op) {
             E = diag_energy(op, s0, s1);
@@ -267,7 +401,7 @@ __global__ __launch_bounds__(512) void k_deep_mfma_eval(DeepMfmaDev md, OpDev op
                 uint64_t p0 = s0, p1 = s1;
                 if (act)
                     for (int q = op.flip_begin[t]; q < op.flip_begin[t + 1]; q++) flip_bit(p0, p1, op.flips[q]);
-                const cx<float> lpp = mfma_log_psi<KBMAX>(md, F, V, p0, p1, lane);
+                const cx<float> lpp = FWD::run(md, F, V, p0, p1, lane);
                 if (act) {
                     E = E + cf * cexp_wide(lpp - lp);
                     t++;
@@ -287,7 +421,7 @@ __global__ __launch_bounds__(512) void k_deep_mfma_eval(DeepMfmaDev md, OpDev op
 // and the proposal stream of the other PsiDeep sampler: Philox block
 // (chain, k/2, call, 0) gives site and uniform of proposals k and k+1.  Only
 // the configurations are recorded; the evaluation kernel measures them.
-template <int KBMAX>
+template <class FWD>
 __global__ __launch_bounds__(512) void k_deep_mfma_mc(DeepMfmaDev md, McParams mp, uint64_t* __restrict__ out_spins,
                                                        uint32_t* __restrict__ counts) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -307,7 +441,7 @@ __global__ __launch_bounds__(512) void k_deep_mfma_mc(DeepMfmaDev md, McParams m
         mask_words(s0, s1, N);
         if (mp.move == MC_EXCHANGE) sector_start(N, mp.k_up, (uint32_t)chain, mp.call, mp.seed_lo, mp.seed_hi, s0, s1);
     }
-    float cur = mfma_log_psi<KBMAX>(md, F, V, s0, s1, lane).re;
+    float cur = FWD::run(md, F, V, s0, s1, lane).re;
     uint32_t nacc = 0, nrej = 0;
     const uint64_t n_therm = (uint64_t)mp.therm * N;
     const uint64_t n_sweep = (uint64_t)mp.sweeps * (mp.move == MC_FAST_SWEEP ? 1 : N);
@@ -341,7 +475,7 @@ __global__ __launch_bounds__(512) void k_deep_mfma_mc(DeepMfmaDev md, McParams m
         } else {
             y = 0u;
         }
-        const float next = mfma_log_psi<KBMAX>(md, F, V, n0, n1, lane).re;
+        const float next = FWD::run(md, F, V, n0, n1, lane).re;
         const float ratio = expf(2.0f * (next - cur));
         const float uu = (float)(((double)y + 0.5) * (1.0 / 4294967296.0));
         if ((ratio > 1.0f) || (uu <= ratio)) { s0 = n0; s1 = n1; cur = next; nacc++; }
@@ -472,12 +606,18 @@ hipError_t launch_deep_mfma_eval(const DeepMfmaDev& md, const OpDev& op, int has
     if (blocks > num_cus) blocks = num_cus;
     if (blocks < 1) blocks = 1;
     hipError_t e = hipSuccess;
-#define RBM_DM_EVAL(KB)                                                                                            \
-    if ((e = set_lds(&k_deep_mfma_eval<KB>, lds)) != hipSuccess) return e;                                        \
-    hipLaunchKernelGGL((k_deep_mfma_eval<KB>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, op, has_op, n, \
+#define RBM_DM_EVAL_F(FW)                                                                                          \
+    if ((e = set_lds(&k_deep_mfma_eval<FW>, lds)) != hipSuccess) return e;                                        \
+    hipLaunchKernelGGL((k_deep_mfma_eval<FW>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, op, has_op, n, \
                        configs, index_base, rec);
-    RBM_DM_DISPATCH(kbm, RBM_DM_EVAL)
+#define RBM_DM_EVAL(KB) RBM_DM_EVAL_F(FwdAny<KB>)
+    if (is_c4_shape(md)) {
+        RBM_DM_EVAL_F(FwdC4)
+    } else {
+        RBM_DM_DISPATCH(kbm, RBM_DM_EVAL)
+    }
 #undef RBM_DM_EVAL
+#undef RBM_DM_EVAL_F
     return hipGetLastError();
 }
 
@@ -492,11 +632,17 @@ hipError_t launch_deep_mfma_mc(const DeepMfmaDev& md, const McParams& mp, uint64
     if (waves > 8) waves = 8;
     const int blocks = (wave_count + waves - 1) / waves;
     hipError_t e = hipSuccess;
-#define RBM_DM_MC(KB)                                                                                              \
-    if ((e = set_lds(&k_deep_mfma_mc<KB>, lds)) != hipSuccess) return e;                                          \
-    hipLaunchKernelGGL((k_deep_mfma_mc<KB>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, mp, spins, counts);
-    RBM_DM_DISPATCH(kbm, RBM_DM_MC)
+#define RBM_DM_MC_F(FW)                                                                                            \
+    if ((e = set_lds(&k_deep_mfma_mc<FW>, lds)) != hipSuccess) return e;                                          \
+    hipLaunchKernelGGL((k_deep_mfma_mc<FW>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, mp, spins, counts);
+#define RBM_DM_MC(KB) RBM_DM_MC_F(FwdAny<KB>)
+    if (is_c4_shape(md)) {
+        RBM_DM_MC_F(FwdC4)
+    } else {
+        RBM_DM_DISPATCH(kbm, RBM_DM_MC)
+    }
 #undef RBM_DM_MC
+#undef RBM_DM_MC_F
     return hipGetLastError();
 }
 

@@ -180,6 +180,30 @@ def test_bf16_mc_energy_within_three_standard_errors():
     assert abs(E_mc - E_exact) < 3 * se, (E_mc, E_exact, se)
 
 
+def test_bf16_deep_tier_shape_sampler_agrees_with_fp32():
+    """The 40 -> 160 -> 80 network runs a compile-time-shaped, software-pipelined
+    forward pass: its chain's energy agrees with the complex64 kernels' chain
+    on the same model within 4 combined standard errors (the models differ by
+    the bfloat16 rounding, well below the statistical error here)."""
+    N, M, C = 40, [160, 80], [40, 160]
+    biases, conns, weights, final = layers(N, M, C, seed=44, re=0.25, im=0.1, bias=0.2)
+    tab = HAM.j1j2_chain(N)
+    ev = R.ExpectationValue(True)
+    res = []
+    for prec in (16, 32):
+        psi = gpu_deep(N, biases, conns, weights, final, precision=prec)
+        chains, per = 2048, 20
+        mc = R.MonteCarloLoop(chains * per, 2, 50, chains, True, seed=5 + prec)
+        E = ev(psi, R.Operator(*tab, True), mc).real
+        _, _, el = mc.last_samples()
+        per_chain = el.real.reshape(per, chains).mean(axis=0)
+        res.append((E, per_chain.std(ddof=1) / np.sqrt(chains), mc.acceptance_rate))
+    (e16, s16, a16), (e32, s32, a32) = res
+    print(f"  bf16 {e16:.5f} +- {s16:.5f} (acc {a16:.3f}), fp32 {e32:.5f} +- {s32:.5f} (acc {a32:.3f})", flush=True)
+    assert abs(e16 - e32) < 4 * np.hypot(s16, s32)
+    assert abs(a16 - a32) < 0.02
+
+
 def test_bf16_rejects_unsupported_models():
     N, M, C = 8, [256, 256], [8, 256]           # 2 x 256 x 256 bfloat16 weights exceed LDS
     biases, conns, weights, final = deep_layers(N, M, C, seed=1, scale=0.01)
