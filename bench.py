@@ -534,7 +534,7 @@ def main():
                     help="also run a 1 GiB device copy (reference kernel for the FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        sys.stderr.write("bench.py measures one GPU: the chains are independent, and this build runs on a single "
+        sys.stderr.write("bench.py measures one GPU: the chains are independent, and the library runs on a single "
                          "MI355X (sharding across GPUs is a design note, not run)\n")
         sys.exit(2)
 

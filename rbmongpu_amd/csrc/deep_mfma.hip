@@ -258,9 +258,9 @@ struct Fwd2 {
     }
 };
 // the deep-network tier, 40 -> 160 -> 80: in_p 64, 160 and out_p 160, 80
-using FwdC4 = Fwd2<10, 2, 5>;
+using FwdDense40x160x80 = Fwd2<10, 2, 5>;
 
-static inline bool is_c4_shape(const DeepMfmaDev& md) {
+static inline bool is_shape_40x160x80(const DeepMfmaDev& md) {
     return md.L == 2 && md.in_p[0] == 64 && md.out_p[0] == 160 && md.in_p[1] == 160 && md.out_p[1] == 80;
 }
 
@@ -611,8 +611,8 @@ hipError_t launch_deep_mfma_eval(const DeepMfmaDev& md, const OpDev& op, int has
     hipLaunchKernelGGL((k_deep_mfma_eval<FW>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, op, has_op, n, \
                        configs, index_base, rec);
 #define RBM_DM_EVAL(KB) RBM_DM_EVAL_F(FwdAny<KB>)
-    if (is_c4_shape(md)) {
-        RBM_DM_EVAL_F(FwdC4)
+    if (is_shape_40x160x80(md)) {
+        RBM_DM_EVAL_F(FwdDense40x160x80)
     } else {
         RBM_DM_DISPATCH(kbm, RBM_DM_EVAL)
     }
@@ -636,8 +636,8 @@ hipError_t launch_deep_mfma_mc(const DeepMfmaDev& md, const McParams& mp, uint64
     if ((e = set_lds(&k_deep_mfma_mc<FW>, lds)) != hipSuccess) return e;                                          \
     hipLaunchKernelGGL((k_deep_mfma_mc<FW>), dim3((unsigned)blocks), dim3(64 * waves), lds, stream, md, mp, spins, counts);
 #define RBM_DM_MC(KB) RBM_DM_MC_F(FwdAny<KB>)
-    if (is_c4_shape(md)) {
-        RBM_DM_MC_F(FwdC4)
+    if (is_shape_40x160x80(md)) {
+        RBM_DM_MC_F(FwdDense40x160x80)
     } else {
         RBM_DM_DISPATCH(kbm, RBM_DM_MC)
     }

@@ -3,7 +3,7 @@
 // order follow the reference binding; arrays are numpy arrays.
 //
 // Differences, all deliberate:
-//   * gpu=False (the reference's CPU path) is rejected: this build computes on
+//   * gpu=False (the reference's CPU path) is rejected: the library computes on
 //     the GPU only;
 //   * keyword-only extensions: Psi(..., a=None, precision=32) for a visible
 //     bias and complex64/complex128 kernels, MonteCarloLoop(..., seed=0);

@@ -54,7 +54,7 @@ def random_words(N, n, seed):
 
 SHAPES = {
     # name: N, layer sizes, connections per unit
-    "c4": (40, [160, 80], [40, 160]),      # the deep-tier network, dense
+    "dense40-160-80": (40, [160, 80], [40, 160]),
     "sparse3": (12, [24, 12, 6], [6, 12, 4]),
     "odd": (7, [13, 5], [7, 13]),          # padding of every dimension
     "wide": (64, [256, 48], [64, 256]),

@@ -272,7 +272,7 @@ def _chain_mean_se(x, chains):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_mc_energy_within_three_standard_errors(precision):
-    """c2-shaped check at reduced size: sampled <H> vs ExactSummation."""
+    """1-D TFIM at reduced size (N = 12, M = 24): sampled <H> vs ExactSummation."""
     N, M = 12, 24
     b, W, _ = random_rbm(N, M, seed=51, scale=0.1)
     tab = HAM.tfim_1d(N, 1.0, 1.0)
@@ -308,13 +308,13 @@ def test_mc_gradient_matches_exact_statistically():
     assert np.median(z) < 1.5 and np.mean(z < 4) > 0.95
 
 
-# ------------------------------------------------------------------ full size (c2)
+# ------------------------------------------------------------------ full size: 1-D TFIM N = 20, M = 80
 def test_c2_full_size_properties():
     """N=20, M=80 TFIM with N(0, 0.01^2) parameters: ExactSummation over 2^20
     states in complex64 agrees with complex128 to 1e-5; the 4096-chain
     sampler agrees with the exact energy within 3 standard errors; the
     distance to the W + N(0, 1e-3^2) copy agrees between precisions."""
-    print("[c2 full size]", flush=True)
+    print("[TFIM N=20 M=80 full size]", flush=True)
     rng = np.random.default_rng(1236)
     N, M = 20, 80
     cn = lambda *s: 0.01 * (rng.normal(size=s) + 1j * rng.normal(size=s))  # noqa: E731

@@ -32,7 +32,7 @@ def test_logcosh_approximates_log_cosh_on_real_axis():
     assert np.abs(O.logcosh(x).imag).max() == 0.0
     f0, f1 = O.logcosh(0.0)[0].real, O.logcosh(0.01)[0].real
     assert abs((f1 - f0) / 0.01 ** 2 - 0.5) < 0.05   # log cosh x = x^2 / 2 + O(x^4)
-    assert abs(O.logcosh(0.0)[0].real - (-1.306188e-7)) < 1e-12  # value at 0 quoted in the survey
+    assert abs(O.logcosh(0.0)[0].real - (-1.306188e-7)) < 1e-12  # my_logcosh(0) of the reference approximant
 
 
 def test_logcosh_is_even():
