@@ -216,3 +216,37 @@ def test_bf16_rejects_unsupported_models():
     psi.translational_invariance = True
     with pytest.raises(NotImplementedError):
         R.log_psi_batch(psi, basis_words(N))
+
+
+def test_bf16_hs_distance_and_learned_ensemble():
+    """The BF16 PsiDeep as psi' of the Hilbert-space distance (its gradient goes
+    through the MFMA back-propagation) against the FP64 oracle on the rounded
+    weights, and as the wavefunction of a learned-proposal ensemble against
+    ExactSummation."""
+    N, M, C = 8, [16, 8], [8, 16]
+    b1, c1, w1, f1 = layers(N, M, C, seed=31)
+    b2, _, w2, f2 = layers(N, M, C, seed=32)
+    ident = (np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    psi = gpu_deep(N, b1, c1, w1, f1, precision=16, compat=False)
+    psi2 = gpu_deep(N, b2, c1, w2, f2, precision=16, compat=False)
+    hs = R.HilbertSpaceDistance(N, psi2.num_params, True)
+    g, d = hs.gradient(psi, psi2, R.Operator(*ident, True), True, R.ExactSummation(N, True), 1.0)
+    r1 = O.Deep(N, b1, c1, [bf16c(w) for w in w1], f1, ref_compat_final=False)
+    r2 = O.Deep(N, b2, c1, [bf16c(w) for w in w2], f2, ref_compat_final=False)
+    gr, dr = O.hs_distance(r1, r2, O.Operator(*ident), True, O.Exact(N), want_grad=True, nu=1.0)
+    assert abs(d - dr) < 2e-3 * dr
+    assert rel_err(g, gr) < 3e-2
+    # learned-proposal ensemble on the BF16 model
+    tab = HAM.j1j2_chain(N)
+    ex = R.ExactSummation(N, True)
+    psi.normalize(ex)
+    ev = R.ExpectationValue(True)
+    E_ex = ev(psi, R.Operator(*tab, True), ex).real
+    chains, per = 512, 60
+    md = R.MADE(N, 32, seed=2)
+    ens = R.LearnedMonteCarloLoop(md, chains * per, 5, chains, True, seed=7)
+    E = ev(psi, R.Operator(*tab, True), ens).real
+    _, _, el = ens.last_samples()
+    per_chain = el.real.reshape(per, chains).mean(axis=0)
+    se = per_chain.std(ddof=1) / np.sqrt(chains)
+    assert abs(E - E_ex) < 4 * se, (E, E_ex, se)
