@@ -1233,7 +1233,7 @@ static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, co
     }
     cx<float>* dob = reinterpret_cast<cx<float>*>(c.buf("s_obar", (size_t)P * sizeof(cx<float>)));
     c.h2d(dob, obar.data(), obar.size() * sizeof(cx<float>));
-    const int ld2 = made_pad(2 * P);
+    const int ld2 = (2 * P + 127) / 128 * 128;   // k_gram_lower works on 128 x 128 tiles
     int64_t nb = ((int64_t)(512ll << 20) / ((int64_t)ld2 * 4)) / 64 * 64;
     nb = std::max<int64_t>(64, std::min<int64_t>(nb, (s.n + 63) / 64 * 64));
     float* O = reinterpret_cast<float*>(c.buf("s_rows", (size_t)nb * ld2 * sizeof(float)));
@@ -1246,9 +1246,7 @@ static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, co
         c.launch("s_matrix", [&] {
             return launch_o_realified<T>(s0, cur, curp, N, M, P, ld2, s.tanh, s.spins, w, dob, O, c.stream);
         });
-        GemmEpiArgs e{};
-        e.lower_only = 1;
-        c.launch("s_gram", [&] { return launch_gemm(EPI_STORE, ld2, ld2, (int)curp, O, 1, ld2, O, ld2, 1, G, ld2, e, c.stream); });
+        c.launch("s_gram", [&] { return launch_gram_lower(ld2, (int)curp, O, G, c.stream); });
         c.launch("s_matrix", [&] { return launch_accum_f64((int64_t)ld2 * ld2, G, Gd, c.stream); });
     }
     cx<double>* S = reinterpret_cast<cx<double>*>(c.buf("s_centred", (size_t)P * P * sizeof(cx<double>)));

@@ -192,6 +192,86 @@ __global__ __launch_bounds__(1024) void k_chol_solve(int n, const cx<double>* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Gram product C = A^T A, lower tiles only (the symmetric S-matrix blocks).
+// A 256-thread workgroup computes a 128 x 128 tile as 2 x 2 waves of 64 x 64,
+// each 2 x 2 v_mfma_f32_32x32x2f32 tiles (lane l supplies A[l % 32][l / 32]
+// and B[l / 32][l % 32]; accumulator v of lane l is row
+// 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).  K advances 16 rows per LDS
+// stage; the next stage's rows are loaded into registers while the current
+// one is multiplied.  Workgroup b covers lower tile (bm, bn), b = bm (bm + 1) / 2 + bn.
+// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int GR_T = 128, GR_K = 16, GR_LD = GR_T + 4;
+
+__global__ __launch_bounds__(256) void k_gram_lower(int ld, int K, const float* __restrict__ A, float* __restrict__ C) {
+    __shared__ float As[GR_K][GR_LD];
+    __shared__ float Bs[GR_K][GR_LD];
+    const int t = blockIdx.x;
+    int bm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bm + 1) * (bm + 2) / 2 <= t) bm++;
+    while (bm * (bm + 1) / 2 > t) bm--;
+    const int bn = t - bm * (bm + 1) / 2;
+    const int m0 = bm * GR_T, n0 = bn * GR_T;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    // staging: 16 rows x 128 columns per operand = 512 float4; thread tid loads
+    // float4 q = tid and tid + 256 of each (row q / 32, column 4 (q % 32))
+    const int q0 = tid, q1 = tid + 256;
+    const int r0 = q0 >> 5, c0 = (q0 & 31) * 4, r1 = q1 >> 5, c1 = (q1 & 31) * 4;
+    float4 ra0, ra1, rb0, rb1;
+#define GR_FETCH(k0)                                                                        \
+    ra0 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r0) * ld + m0 + c0);         \
+    ra1 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r1) * ld + m0 + c1);         \
+    rb0 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r0) * ld + n0 + c0);         \
+    rb1 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r1) * ld + n0 + c1);
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x16{};
+    GR_FETCH(0)
+    for (int k0 = 0; k0 < K; k0 += GR_K) {
+        __syncthreads();
+        *reinterpret_cast<float4*>(&As[r0][c0]) = ra0;
+        *reinterpret_cast<float4*>(&As[r1][c1]) = ra1;
+        *reinterpret_cast<float4*>(&Bs[r0][c0]) = rb0;
+        *reinterpret_cast<float4*>(&Bs[r1][c1]) = rb1;
+        __syncthreads();
+        if (k0 + GR_K < K) { GR_FETCH(k0 + GR_K) }
+#pragma unroll
+        for (int kk = 0; kk < GR_K; kk += 2) {
+            float a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = As[kk + (lane >> 5)][wm + 32 * i + (lane & 31)];
+#pragma unroll
+            for (int j = 0; j < 2; j++) b[j] = Bs[kk + (lane >> 5)][wn + 32 * j + (lane & 31)];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int row = m0 + wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int col = n0 + wn + 32 * j + (lane & 31);
+                C[(int64_t)row * ld + col] = acc[i][j][v];
+            }
+#undef GR_FETCH
+}
+
+hipError_t launch_gram_lower(int ld, int K, const float* A, float* C, hipStream_t stream) {
+    if (ld <= 0 || ld % GR_T || K <= 0 || K % GR_K) return hipErrorInvalidValue;
+    const int T = ld / GR_T;
+    hipLaunchKernelGGL(k_gram_lower, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, stream, ld, K, A, C);
+    return hipGetLastError();
+}
+
 hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream) {
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = n - k0 < NB ? n - k0 : NB;

@@ -13,5 +13,9 @@ namespace rbmhip {
 hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream);
 // x <- (L L^H)^{-1} x with the factor from launch_cholesky
 hipError_t launch_cholesky_solve(int n, const cx<double>* L, cx<double>* x, hipStream_t stream);
+// C = A^T A on the lower 128 x 128 tiles of C (ld x ld), A row-major K x ld
+// (FP32, v_mfma_f32_32x32x2f32); ld a multiple of 128, K of 16.  Tiles above
+// the diagonal are not written.
+hipError_t launch_gram_lower(int ld, int K, const float* A, float* C, hipStream_t stream);
 
 }  // namespace rbmhip
