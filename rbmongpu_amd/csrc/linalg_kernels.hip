@@ -8,7 +8,7 @@
 //   k_potrf_diag   factor the diagonal block in LDS (one 1024-thread workgroup)
 //   k_trsm_panel   rows below: X L_kk^H = A_ik (one thread per row)
 //   k_herk_update  trailing lower triangle: A_il -= X_i X_l^H (32 x 32 tiles)
-// then forward and backward substitution in one workgroup.
+// then blocked forward and backward substitution (k_trsv_block, k_trsv_update).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -130,68 +130,6 @@ __global__ __launch_bounds__(256) void k_herk_update(int n, int k0, cx<double>* 
     }
 }
 
-// Solve L y = b then L^H x = y in one workgroup (x may alias b).
-__global__ __launch_bounds__(1024) void k_chol_solve(int n, const cx<double>* __restrict__ A, cx<double>* __restrict__ x) {
-    __shared__ cx<double> blk[NB];
-    // forward: y_i = (b_i - sum_{m<i} L_im y_m) / L_ii
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        const int nb = n - k0 < NB ? n - k0 : NB;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int j = 0; j < nb; j++) {
-                cx<double> a = x[k0 + j];
-                for (int m = 0; m < j; m++) {
-                    const cx<double> L = A[(int64_t)(k0 + j) * n + k0 + m];
-                    const cx<double> y = blk[m];
-                    a = mk(a.re - (L.re * y.re - L.im * y.im), a.im - (L.re * y.im + L.im * y.re));
-                }
-                const double r = 1.0 / A[(int64_t)(k0 + j) * n + k0 + j].re;
-                blk[j] = mk(a.re * r, a.im * r);
-                x[k0 + j] = blk[j];
-            }
-        }
-        __syncthreads();
-        for (int i = k0 + nb + threadIdx.x; i < n; i += blockDim.x) {
-            cx<double> a = x[i];
-            for (int m = 0; m < nb; m++) {
-                const cx<double> L = A[(int64_t)i * n + k0 + m];
-                const cx<double> y = blk[m];
-                a = mk(a.re - (L.re * y.re - L.im * y.im), a.im - (L.re * y.im + L.im * y.re));
-            }
-            x[i] = a;
-        }
-    }
-    // backward: x_i = (y_i - sum_{m>i} conj(L_mi) x_m) / L_ii
-    const int nblk = (n + NB - 1) / NB;
-    for (int kb = nblk - 1; kb >= 0; kb--) {
-        const int k0 = kb * NB;
-        const int nb = n - k0 < NB ? n - k0 : NB;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int j = nb - 1; j >= 0; j--) {
-                cx<double> a = x[k0 + j];
-                for (int m = j + 1; m < nb; m++) {
-                    const cx<double> u = cmul_conj(blk[m], A[(int64_t)(k0 + m) * n + k0 + j]);
-                    a = mk(a.re - u.re, a.im - u.im);
-                }
-                const double r = 1.0 / A[(int64_t)(k0 + j) * n + k0 + j].re;
-                blk[j] = mk(a.re * r, a.im * r);
-                x[k0 + j] = blk[j];
-            }
-        }
-        __syncthreads();
-        // rows above the block: x_i -= sum_{m in block} conj(L_mi) x_m
-        for (int i = threadIdx.x; i < k0; i += blockDim.x) {
-            cx<double> a = x[i];
-            for (int m = 0; m < nb; m++) {
-                const cx<double> u = cmul_conj(blk[m], A[(int64_t)(k0 + m) * n + i]);
-                a = mk(a.re - u.re, a.im - u.im);
-            }
-            x[i] = a;
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Gram product C = A^T A, lower tiles only (the symmetric S-matrix blocks).
 // A 256-thread workgroup computes a 128 x 128 tile as 2 x 2 waves of 64 x 64,
@@ -285,8 +223,93 @@ hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream) 
     return hipGetLastError();
 }
 
+// Blocked substitution over all CUs.  Per diagonal block k0 (NB rows):
+//   k_trsv_block   one wavefront solves the block with its 32 x 32 factor
+//                  (forward: y_b = L_bb^{-1} x_b; backward: x_b = L_bb^{-H} y_b)
+//   k_trsv_update  the rows outside the block subtract the block's products,
+//                  one wavefront per row pair with coalesced row reads
+//                  (forward: rows below, x_i -= L_ib y_b; backward: rows above,
+//                  x_i -= sum_m conj(L_mi) x_m over the block's rows m).
+__global__ __launch_bounds__(64) void k_trsv_block(int n, int k0, int backward, const cx<double>* __restrict__ A,
+                                                   cx<double>* __restrict__ x) {
+    __shared__ cx<double> Lb[NB][NB + 1];
+    __shared__ cx<double> xb[NB];
+    const int nb = n - k0 < NB ? n - k0 : NB;
+    for (int t = threadIdx.x; t < NB * NB; t += 64) {
+        const int i = t / NB, l = t % NB;
+        Lb[i][l] = (i < nb && l < nb && l <= i) ? A[(int64_t)(k0 + i) * n + k0 + l] : mk(0.0, 0.0);
+    }
+    if (threadIdx.x < NB) xb[threadIdx.x] = threadIdx.x < nb ? x[k0 + threadIdx.x] : mk(0.0, 0.0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!backward) {
+            for (int j = 0; j < nb; j++) {
+                cx<double> a = xb[j];
+                for (int m = 0; m < j; m++) {
+                    const cx<double> L = Lb[j][m], y = xb[m];
+                    a = mk(a.re - (L.re * y.re - L.im * y.im), a.im - (L.re * y.im + L.im * y.re));
+                }
+                const double r = 1.0 / Lb[j][j].re;
+                xb[j] = mk(a.re * r, a.im * r);
+            }
+        } else {
+            for (int j = nb - 1; j >= 0; j--) {
+                cx<double> a = xb[j];
+                for (int m = j + 1; m < nb; m++) {
+                    const cx<double> u = cmul_conj(xb[m], Lb[m][j]);
+                    a = mk(a.re - u.re, a.im - u.im);
+                }
+                const double r = 1.0 / Lb[j][j].re;
+                xb[j] = mk(a.re * r, a.im * r);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nb) x[k0 + threadIdx.x] = xb[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_trsv_update(int n, int k0, int backward, const cx<double>* __restrict__ A,
+                                                     cx<double>* __restrict__ x) {
+    __shared__ cx<double> xb[NB];
+    const int nb = n - k0 < NB ? n - k0 : NB;
+    if (threadIdx.x < NB) xb[threadIdx.x] = threadIdx.x < nb ? x[k0 + threadIdx.x] : mk(0.0, 0.0);
+    __syncthreads();
+    const int lane = threadIdx.x & 31, half = threadIdx.x >> 5;   // 8 rows per workgroup, 32 lanes per row
+    const int64_t row = (int64_t)blockIdx.x * 8 + half;
+    // forward: rows k0 + nb .. n-1; backward: rows 0 .. k0-1
+    const int64_t i = backward ? row : k0 + nb + row;
+    const bool live = backward ? row < k0 : i < n;
+    cx<double> u = mk(0.0, 0.0);
+    if (live && lane < nb) {
+        if (!backward) {
+            const cx<double> L = A[i * n + k0 + lane], y = xb[lane];
+            u = mk(L.re * y.re - L.im * y.im, L.re * y.im + L.im * y.re);
+        } else {
+            u = cmul_conj(xb[lane], A[(int64_t)(k0 + lane) * n + i]);
+        }
+    }
+    // sum over the 32 lanes of the row (lanes 0-31 and 32-63 of a wavefront hold two rows)
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+        u.re += __shfl_xor(u.re, off, 32);
+        u.im += __shfl_xor(u.im, off, 32);
+    }
+    if (live && lane == 0) x[i] = mk(x[i].re - u.re, x[i].im - u.im);
+}
+
 hipError_t launch_cholesky_solve(int n, const cx<double>* L, cx<double>* x, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, stream, n, L, x);
+    const int nblk = (n + NB - 1) / NB;
+    for (int kb = 0; kb < nblk; kb++) {
+        const int k0 = kb * NB, nb = n - k0 < NB ? n - k0 : NB;
+        hipLaunchKernelGGL(k_trsv_block, dim3(1), dim3(64), 0, stream, n, k0, 0, L, x);
+        const int rest = n - k0 - nb;
+        if (rest > 0) hipLaunchKernelGGL(k_trsv_update, dim3((unsigned)((rest + 7) / 8)), dim3(256), 0, stream, n, k0, 0, L, x);
+    }
+    for (int kb = nblk - 1; kb >= 0; kb--) {
+        const int k0 = kb * NB;
+        hipLaunchKernelGGL(k_trsv_block, dim3(1), dim3(64), 0, stream, n, k0, 1, L, x);
+        if (k0 > 0) hipLaunchKernelGGL(k_trsv_update, dim3((unsigned)((k0 + 7) / 8)), dim3(256), 0, stream, n, k0, 1, L, x);
+    }
     return hipGetLastError();
 }
 
