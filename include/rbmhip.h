@@ -197,6 +197,15 @@ int rbmhip_sr_update(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double eps
  * of psi_prime) is written when want_grad != 0. */
 int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
                        int want_grad, double nu, double* distance, double* grad);
+/* The same with free_quantum_axis = true on psi (HilbertSpaceDistance.cu:47-80,
+ * 109-122; update_quaxis 164-180): log psi' gains sum_i [(alpha'_i - alpha_i)
+ * s_i r_i + i (beta'_i - beta_i)(r_i cos(alpha'_i + alpha_i) - s_i sin(...))]
+ * with r_i = psi'(s with spin i flipped) / psi'(s), and grad entries i and N + i
+ * are the alpha / beta derivatives.  alpha, beta are psi's axis angles,
+ * alpha_prime, beta_prime psi_prime's (N each). */
+int rbmhip_hs_distance_axis(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
+                            int want_grad, double nu, const double* alpha, const double* beta, const double* alpha_prime,
+                            const double* beta_prime, double* distance, double* grad);
 
 /* ---- learned proposal: masked autoregressive network (MADE) -------------- */
 /* No counterpart in the reference.  A MADE (Germain et al. 2015) over x_i = 1

@@ -89,6 +89,8 @@ def _declare(L):
     L.orc_s_matrix.argtypes = [P, P, P]
     L.orc_hs_distance.argtypes = [P, P, P, C.c_int, P, C.c_int, C.c_double, P]
     L.orc_hs_distance.restype = C.c_double
+    L.orc_hs_distance_axis.argtypes = [P, P, P, C.c_int, P, C.c_int, C.c_double, P, P, P, P, P]
+    L.orc_hs_distance_axis.restype = C.c_double
 
 
 def _ptr(a):
@@ -372,12 +374,19 @@ def s_matrix(psi, ens):
     return S
 
 
-def hs_distance(psi, psi_prime, op, is_unitary, ens, want_grad=False, nu=1.0):
+def hs_distance(psi, psi_prime, op, is_unitary, ens, want_grad=False, nu=1.0, axis=None):
+    """axis = (alpha, beta, alpha_prime, beta_prime) switches on the free
+    quantum axis of the first wave function."""
     c = _Call(psi, op, ens)
     mp = psi_prime.struct()
     g = np.zeros(psi_prime.num_params, dtype=np.complex128)
     m, o, e = c.args()
-    d = lib().orc_hs_distance(m, C.byref(mp), o, int(bool(is_unitary)), e, int(bool(want_grad)),
-                              float(np.float32(nu)), g.ctypes.data)
+    if axis is None:
+        d = lib().orc_hs_distance(m, C.byref(mp), o, int(bool(is_unitary)), e, int(bool(want_grad)),
+                                  float(np.float32(nu)), g.ctypes.data)
+    else:
+        ax = [np.ascontiguousarray(a, dtype=np.float64) for a in axis]
+        d = lib().orc_hs_distance_axis(m, C.byref(mp), o, int(bool(is_unitary)), e, int(bool(want_grad)),
+                                       float(np.float32(nu)), g.ctypes.data, *[a.ctypes.data for a in ax])
     c.done()
     return (g, d) if want_grad else d

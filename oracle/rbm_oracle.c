@@ -900,17 +900,35 @@ void orc_s_matrix(const orc_model* m, orc_ens* e, cplx* S) {
     free(c.ok);
 }
 
-/* HilbertSpaceDistance::compute_averages (free_quantum_axis = false):
- * source/network_functions/HilbertSpaceDistance.cu:14-136; distance 193-221;
- * gradient 224-269. */
+/* HilbertSpaceDistance::compute_averages: source/network_functions/
+ * HilbertSpaceDistance.cu:14-136; distance 193-221; gradient 224-269.  With
+ * free_quantum_axis (da != NULL) log psi' gets the first-order axis term
+ * sum_i [da_i s_i r_i + i db_i (r_i cos_i - s_i sin_i)] with the single-flip
+ * ratios r_i = psi'(s^i) / psi'(s), da = alpha' - alpha, db = beta' - beta,
+ * cos_i / sin_i of alpha' + alpha (lines 47-80, 164-180), and the gradient
+ * entries of alpha_i (index i) and beta_i (index N + i) are s_i r_i and
+ * i (r_i cos_i - s_i sin_i) (lines 109-122). */
 typedef struct {
     const orc_model* psi; const orc_model* psi_prime; const orc_op* op; int is_unitary; int want_grad; int64_t P;
     cplx omega_avg; double ratio_avg; double norm_avg; cplx* omega_O; cplx* ratio_O; cplx* ok;
+    const double *da, *db, *ca, *sa;
+    cplx* r;
 } hs_ctx;
 static void hs_fn(void* vctx, int64_t idx, const uint64_t* s, cplx lp, const cplx* theta, double w) {
     hs_ctx* c = (hs_ctx*)vctx; (void)idx;
     const cplx el = local_energy(c->psi, c->op, s, lp, theta);
-    const cplx lpp = model_log_psi(c->psi_prime, s);
+    cplx lpp = model_log_psi(c->psi_prime, s);
+    const int N = c->psi_prime->N;
+    if (c->da) {
+        const cplx lpp0 = lpp;
+        for (int i = 0; i < N; i++) {
+            uint64_t si[2] = {s[0], s[1]};
+            flip(si, i);
+            c->r[i] = cexp(model_log_psi(c->psi_prime, si) - lpp0);
+            const double sp = spin(s, i);
+            lpp += c->da[i] * sp * c->r[i] + c->db[i] * I * (c->r[i] * c->ca[i] - sp * c->sa[i]);
+        }
+    }
     cplx omega;
     if (c->is_unitary) {
         omega = cexp(conj(lpp - lp)) * el;
@@ -928,15 +946,49 @@ static void hs_fn(void* vctx, int64_t idx, const uint64_t* s, cplx lp, const cpl
             c->omega_O[k] += w * omega * conj(c->ok[k]);
             c->ratio_O[k] += w * ratio * conj(c->ok[k]);
         }
+        if (c->da) {
+            for (int i = 0; i < N; i++) {
+                const double sp = spin(s, i);
+                const cplx oa = sp * c->r[i];
+                const cplx ob = I * (c->r[i] * c->ca[i] - sp * c->sa[i]);
+                c->omega_O[i] += w * omega * conj(oa);
+                c->ratio_O[i] += w * ratio * conj(oa);
+                c->omega_O[N + i] += w * omega * conj(ob);
+                c->ratio_O[N + i] += w * ratio * conj(ob);
+            }
+        }
     }
 }
 
+double orc_hs_distance_axis(const orc_model* psi, const orc_model* psi_prime, const orc_op* op, int is_unitary,
+                            orc_ens* e, int want_grad, double nu, cplx* grad, const double* alpha, const double* beta,
+                            const double* alpha_prime, const double* beta_prime);
+
 double orc_hs_distance(const orc_model* psi, const orc_model* psi_prime, const orc_op* op, int is_unitary,
                        orc_ens* e, int want_grad, double nu, cplx* grad) {
+    return orc_hs_distance_axis(psi, psi_prime, op, is_unitary, e, want_grad, nu, grad, NULL, NULL, NULL, NULL);
+}
+
+double orc_hs_distance_axis(const orc_model* psi, const orc_model* psi_prime, const orc_op* op, int is_unitary,
+                            orc_ens* e, int want_grad, double nu, cplx* grad, const double* alpha, const double* beta,
+                            const double* alpha_prime, const double* beta_prime) {
     const int64_t P = orc_num_params(psi_prime);
+    const int N = psi_prime->N;
     hs_ctx c;
     memset(&c, 0, sizeof(c));
     c.psi = psi; c.psi_prime = psi_prime; c.op = op; c.is_unitary = is_unitary; c.want_grad = want_grad; c.P = P;
+    double* axis = NULL;
+    if (alpha) {
+        axis = (double*)malloc(sizeof(double) * 4 * (size_t)N);
+        for (int i = 0; i < N; i++) {
+            axis[i] = alpha_prime[i] - alpha[i];
+            axis[N + i] = beta_prime[i] - beta[i];
+            axis[2 * N + i] = cos(alpha_prime[i] + alpha[i]);
+            axis[3 * N + i] = sin(alpha_prime[i] + alpha[i]);
+        }
+        c.da = axis; c.db = axis + N; c.ca = axis + 2 * N; c.sa = axis + 3 * N;
+        c.r = (cplx*)calloc((size_t)N, sizeof(cplx));
+    }
     c.omega_O = (cplx*)calloc((size_t)P, sizeof(cplx));
     c.ratio_O = (cplx*)calloc((size_t)P, sizeof(cplx));
     c.ok = (cplx*)calloc((size_t)P, sizeof(cplx));
@@ -956,6 +1008,6 @@ double orc_hs_distance(const orc_model* psi, const orc_model* psi_prime, const o
             grad[k] = (-(ukp * v - u * vkp) / (v * v)) / pref;
         }
     }
-    free(c.omega_O); free(c.ratio_O); free(c.ok);
+    free(c.omega_O); free(c.ratio_O); free(c.ok); free(axis); free(c.r);
     return distance;
 }

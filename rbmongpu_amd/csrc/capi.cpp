@@ -2092,8 +2092,11 @@ int rbmhip_sr_update(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double eps
     });
 }
 
-int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
-                       int want_grad, double nu, double* distance, double* grad) {
+// HilbertSpaceDistance::distance / gradient; axis (4 N host values: alpha' -
+// alpha, beta' - beta, cos and sin of alpha' + alpha) selects the free quantum
+// axis form, HilbertSpaceDistance.cu:47-80 and 109-122.
+static int hs_distance_impl(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
+                            int want_grad, double nu, const std::vector<double>* axis, double* distance, double* grad) {
     return guard([&] {
         check_psi(psi); check_psi(psi_prime); check_ens(ens);
         RBM_REQUIRE(op && distance && (!want_grad || grad), "bad arguments");
@@ -2119,6 +2122,31 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
             eval_model<T>(c, psi_prime, nullptr, s.n, configs, 0, rec);
         });
         const uint64_t* spins_for_grad = s.spins ? s.spins : spins2;
+        // free quantum axis: log psi' on the N single-flip neighbours of every
+        // sample (chunks of at most 2^26 configurations), ratios r_i and the
+        // first-order axis term added to log psi'
+        const int N = psi->N;
+        double* axis_dev = nullptr;
+        cx<double>* flip_r = nullptr;
+        if (axis) {
+            RBM_REQUIRE((double)s.n * N * sizeof(cx<double>) <= (double)(32ll << 30),
+                        "free_quantum_axis keeps N flip ratios per sample; this ensemble needs more than 32 GiB for them");
+            axis_dev = reinterpret_cast<double*>(c.buf("axis", axis->size() * sizeof(double)));
+            c.h2d(axis_dev, axis->data(), axis->size() * sizeof(double));
+            flip_r = reinterpret_cast<cx<double>*>(c.buf("axis_ratio", (size_t)s.n * N * sizeof(cx<double>)));
+            const int per = (int)std::max<int64_t>(1, std::min<int64_t>(N, (1ll << 26) / s.n));
+            uint64_t* fc = reinterpret_cast<uint64_t*>(c.buf("axis_configs", (size_t)s.n * per * 2 * sizeof(uint64_t)));
+            for (int i0 = 0; i0 < N; i0 += per) {
+                const int cnt = std::min(per, N - i0);
+                c.launch("axis_flip", [&] { return launch_flip_configs(s.n, configs, 0, i0, cnt, fc, c.stream); });
+                with_precision(psi_prime->precision, [&](auto tag) {
+                    using T = decltype(tag);
+                    Records rec{nullptr, flip_r + (size_t)i0 * s.n, nullptr, nullptr};
+                    eval_model<T>(c, psi_prime, nullptr, s.n * cnt, fc, 0, rec, "axis_eval");
+                });
+            }
+            c.launch("axis_log_psi", [&] { return launch_axis_log_psi(s.n, N, configs, 0, axis_dev, lp2, flip_r, c.stream); });
+        }
         double* sc = reinterpret_cast<double*>(c.buf("sc", (size_t)s.n * SC_STRIDE * sizeof(double)));
         CoefArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -2148,6 +2176,19 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
                 using T = decltype(tag);
                 R = contract_params<T>(c, psi_prime, s.n, tanh2, spins_for_grad, coef);
             });
+            if (axis) {
+                const int slices = (int)std::max<int64_t>(1, std::min<int64_t>(256, s.n / 4096));
+                cx<double>* part = reinterpret_cast<cx<double>*>(c.buf("axis_part", (size_t)slices * 2 * N * sizeof(cx<double>)));
+                cx<double>* out = reinterpret_cast<cx<double>*>(c.buf("axis_out", (size_t)2 * N * sizeof(cx<double>)));
+                c.launch("axis_contract", [&] {
+                    return launch_axis_contract(s.n, N, configs, 0, axis_dev, flip_r, coef, slices, part, c.stream);
+                });
+                c.launch("partials_sum", [&] { return launch_partials_sum(slices, 2 * N, part, out, c.stream); });
+                std::vector<cd> h((size_t)2 * N);
+                c.d2h(h.data(), out, h.size() * sizeof(cx<double>));
+                c.sync();
+                for (int k = 0; k < 2 * N; k++) R[(size_t)k] = h[(size_t)k];
+            }
             const int64_t P = psi_prime->num_params();
             const double pref = std::pow(d, (double)(float)nu);
             const cd fac = -std::conj(omega) * nrm / (n * v * v * pref);
@@ -2158,6 +2199,30 @@ int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, in
             }
         }
     });
+}
+
+int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
+                       int want_grad, double nu, double* distance, double* grad) {
+    return hs_distance_impl(psi, psi_prime, op, is_unitary, ens, want_grad, nu, nullptr, distance, grad);
+}
+
+int rbmhip_hs_distance_axis(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
+                            int want_grad, double nu, const double* alpha, const double* beta, const double* alpha_prime,
+                            const double* beta_prime, double* distance, double* grad) {
+    int rc = guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(alpha && beta && alpha_prime && beta_prime, "null quantum-axis angles");
+    });
+    if (rc != 0) return rc;
+    const int N = psi->N;
+    std::vector<double> axis((size_t)4 * N);
+    for (int i = 0; i < N; i++) {
+        axis[(size_t)i] = alpha_prime[i] - alpha[i];
+        axis[(size_t)(N + i)] = beta_prime[i] - beta[i];
+        axis[(size_t)(2 * N + i)] = std::cos(alpha_prime[i] + alpha[i]);
+        axis[(size_t)(3 * N + i)] = std::sin(alpha_prime[i] + alpha[i]);
+    }
+    return hs_distance_impl(psi, psi_prime, op, is_unitary, ens, want_grad, nu, &axis, distance, grad);
 }
 
 // ---- timing -------------------------------------------------------------------------

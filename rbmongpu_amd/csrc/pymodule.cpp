@@ -8,7 +8,7 @@
 //   * keyword-only extensions: Psi(..., a=None, precision=32) for a visible
 //     bias and complex64/complex128 kernels, MonteCarloLoop(..., seed=0);
 //   * Psi keeps beta as given (the reference's PsiBase copies alpha into beta,
-//     include/quantum_state/PsiBase.hpp:140-153);
+//     include/quantum_state/PsiBase.hpp:176, 190);
 //   * psi_O_k_vector returns the M + N*M derivatives of (b, W) (the reference
 //     copies num_params values into an O_k_length array,
 //     source/network_functions/PsiOkVector.cu:12-42).
@@ -367,10 +367,6 @@ static std::pair<double, cd> fluctuation(const P& psi, const Operator& op, E& en
     return {f, cd(E2[0], E2[1])};
 }
 
-static void not_yet(const char* what) {
-    throw NotImplemented(std::string(what) + " is not implemented yet in rbmongpu_amd");
-}
-
 struct ExpectationValue {
     bool gpu = true;
 };
@@ -380,28 +376,34 @@ struct HilbertSpaceDistance {
     bool gpu = true;
 };
 
+// psi.free_quantum_axis selects the form with the axis angles of both
+// wavefunctions (HilbertSpaceDistance.cu:199-205)
+template <class P, class E>
+static void hs_run(const P& psi, const P& psi_prime, const Operator& op, bool is_unitary, E& ens, int want_grad, double nu,
+                   double* d, double* g) {
+    py::gil_scoped_release nogil;
+    if (psi.free_quantum_axis)
+        check(rbmhip_hs_distance_axis(psi.h, psi_prime.h, op.h, is_unitary, ens.h, want_grad, nu, psi.alpha.data(),
+                                      psi.beta.data(), psi_prime.alpha.data(), psi_prime.beta.data(), d, g));
+    else
+        check(rbmhip_hs_distance(psi.h, psi_prime.h, op.h, is_unitary, ens.h, want_grad, nu, d, g));
+}
+
 template <class P, class E>
 static double hs_call(HilbertSpaceDistance& hs, const P& psi, const P& psi_prime, const Operator& op, bool is_unitary, E& ens) {
-    if (psi.free_quantum_axis) not_yet("HilbertSpaceDistance with free_quantum_axis=True");
     double d = 0;
-    py::gil_scoped_release nogil;
-    check(rbmhip_hs_distance(psi.h, psi_prime.h, op.h, is_unitary, ens.h, 0, 1.0, &d, nullptr));
+    hs_run(psi, psi_prime, op, is_unitary, ens, 0, 1.0, &d, nullptr);
     return d;
 }
 
 template <class P, class E>
 static std::pair<carr, double> hs_gradient(HilbertSpaceDistance& hs, const P& psi, const P& psi_prime, const Operator& op,
                                            bool is_unitary, E& ens, float nu) {
-    if (psi.free_quantum_axis) not_yet("HilbertSpaceDistance with free_quantum_axis=True");
     const int64_t n = psi_prime.num_params();
     if ((int64_t)hs.num_params != n) throw py::value_error("HilbertSpaceDistance was built for a different num_params");
     carr g((py::ssize_t)n);
     double d = 0;
-    double* gp = reinterpret_cast<double*>(g.mutable_data());
-    {
-        py::gil_scoped_release nogil;
-        check(rbmhip_hs_distance(psi.h, psi_prime.h, op.h, is_unitary, ens.h, 1, (double)nu, &d, gp));
-    }
+    hs_run(psi, psi_prime, op, is_unitary, ens, 1, (double)nu, &d, reinterpret_cast<double*>(g.mutable_data()));
     return {g, d};
 }
 

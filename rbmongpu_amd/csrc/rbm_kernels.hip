@@ -451,6 +451,87 @@ __global__ __launch_bounds__(256) void k_ek_coef(int64_t i0, int64_t cnt, int T1
     }
 }
 
+// ---- free quantum axis of HilbertSpaceDistance (HilbertSpaceDistance.cu:47-80,
+// 109-122).  axis holds N values each of da = alpha' - alpha, db = beta' - beta,
+// cos(alpha' + alpha), sin(alpha' + alpha).  Samples are configs[2s..2s+1], or
+// index_base + s when configs is null (the basis states of an exact sum).
+RBM_DI void sample_words(const uint64_t* configs, uint64_t index_base, int64_t s, uint64_t& s0, uint64_t& s1) {
+    if (configs) { s0 = configs[2 * s]; s1 = configs[2 * s + 1]; }
+    else { s0 = index_base + (uint64_t)s; s1 = 0; }
+}
+
+// out[(i - i0) n + s] = sample s with spin i flipped, i in [i0, i0 + cnt)
+__global__ __launch_bounds__(256) void k_flip_configs(int64_t n, const uint64_t* __restrict__ configs, uint64_t index_base,
+                                                      int i0, int cnt, uint64_t* __restrict__ out) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n * cnt; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = v % n;
+        uint64_t s0, s1;
+        sample_words(configs, index_base, s, s0, s1);
+        flip_bit(s0, s1, i0 + (int)(v / n));
+        out[2 * v] = s0;
+        out[2 * v + 1] = s1;
+    }
+}
+
+// O of the axis angles of spin i: alpha_i -> s_i r_i, beta_i -> i (r_i cos_i - s_i sin_i)
+RBM_DI cx<double> axis_o_beta(cx<double> r, double sp, double ca, double sa) {
+    return mk(-r.im * ca, r.re * ca - sp * sa);
+}
+
+// r[i n + s] holds log psi'(s^i) on entry and r_i = psi'(s^i) / psi'(s) on exit;
+// lp[s] (log psi'(s)) gets sum_i [da_i s_i r_i + db_i O_beta_i].
+__global__ __launch_bounds__(256) void k_axis_log_psi(int64_t n, int N, const uint64_t* __restrict__ configs,
+                                                      uint64_t index_base, const double* __restrict__ axis,
+                                                      cx<double>* __restrict__ lp, cx<double>* __restrict__ r) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t s0, s1;
+        sample_words(configs, index_base, s, s0, s1);
+        const cx<double> lp0 = lp[s];
+        cx<double> corr = mk(0.0, 0.0);
+        for (int i = 0; i < N; i++) {
+            const cx<double> ri = cexp(r[(int64_t)i * n + s] - lp0);
+            r[(int64_t)i * n + s] = ri;
+            const double sp = spin_of<double>(s0, s1, i);
+            corr = corr + (axis[i] * sp) * ri + axis[N + i] * axis_o_beta(ri, sp, axis[2 * N + i], axis[3 * N + i]);
+        }
+        lp[s] = lp0 + corr;
+    }
+}
+
+// partials[y][k] = sum over samples of slice y of coef[s] conj(O_k(s)), k < 2N;
+// workgroup (i, y) produces k = i and k = N + i.
+__global__ __launch_bounds__(256) void k_axis_contract(int64_t n, int N, const uint64_t* __restrict__ configs,
+                                                       uint64_t index_base, const double* __restrict__ axis,
+                                                       const cx<double>* __restrict__ r, const cx<double>* __restrict__ coef,
+                                                       cx<double>* __restrict__ partials) {
+    __shared__ double red[4][256];
+    const int i = blockIdx.x;
+    const double ca = axis[2 * N + i], sa = axis[3 * N + i];
+    cx<double> ga = mk(0.0, 0.0), gb = mk(0.0, 0.0);
+    for (int64_t s = blockIdx.y * (int64_t)blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.y * blockDim.x) {
+        uint64_t s0, s1;
+        sample_words(configs, index_base, s, s0, s1);
+        const double sp = spin_of<double>(s0, s1, i);
+        const cx<double> ri = r[(int64_t)i * n + s];
+        const cx<double> c = coef[s];
+        ga = ga + c * conj(sp * ri);
+        gb = gb + c * conj(axis_o_beta(ri, sp, ca, sa));
+    }
+    red[0][threadIdx.x] = ga.re; red[1][threadIdx.x] = ga.im;
+    red[2][threadIdx.x] = gb.re; red[3][threadIdx.x] = gb.im;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[(int64_t)blockIdx.y * 2 * N + i] = mk(red[0][0], red[1][0]);
+        partials[(int64_t)blockIdx.y * 2 * N + N + i] = mk(red[2][0], red[3][0]);
+    }
+}
+
 // Block partial sums of the SC_STRIDE scalar records, fixed order.
 __global__ __launch_bounds__(256) void k_scalar_partials(int64_t n, const double* __restrict__ sc,
                                                          double* __restrict__ partials) {
@@ -961,6 +1042,32 @@ hipError_t launch_ek_coef(int64_t i0, int64_t cnt, int T1, const cx<double>* w, 
     if (g < 1) g = 1;
     hipLaunchKernelGGL(k_ek_coef, dim3((unsigned)g), dim3(256), 0, stream, i0, cnt, T1, w, lp, el, gamma, delta, vlp, vfac,
                        vcoef);
+    return hipGetLastError();
+}
+
+hipError_t launch_flip_configs(int64_t n, const uint64_t* configs, uint64_t index_base, int i0, int cnt, uint64_t* out,
+                               hipStream_t stream) {
+    int64_t g = (n * cnt + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_flip_configs, dim3((unsigned)g), dim3(256), 0, stream, n, configs, index_base, i0, cnt, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_axis_log_psi(int64_t n, int N, const uint64_t* configs, uint64_t index_base, const double* axis,
+                               cx<double>* lp, cx<double>* r, hipStream_t stream) {
+    int64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_axis_log_psi, dim3((unsigned)g), dim3(256), 0, stream, n, N, configs, index_base, axis, lp, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_axis_contract(int64_t n, int N, const uint64_t* configs, uint64_t index_base, const double* axis,
+                                const cx<double>* r, const cx<double>* coef, int slices, cx<double>* partials,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(k_axis_contract, dim3((unsigned)N, (unsigned)slices), dim3(256), 0, stream, n, N, configs, index_base,
+                       axis, r, coef, partials);
     return hipGetLastError();
 }
 

@@ -102,6 +102,17 @@ hipError_t launch_ek_expand(const OpDev& op, int64_t i0, int64_t cnt, const uint
 hipError_t launch_ek_coef(int64_t i0, int64_t cnt, int T1, const cx<double>* w, const cx<double>* lp, const cx<double>* el,
                           cx<double> gamma, cx<double> delta, const cx<double>* vlp, const cx<double>* vfac,
                           cx<double>* vcoef, hipStream_t stream);
+// Free quantum axis of HilbertSpaceDistance: configurations with one spin
+// flipped, the first-order axis term of log psi' (flip ratios kept in r, flip
+// i at r + i n), and the alpha / beta rows of sum_s coef conj(O_k) as
+// slices x 2N partials (rbm_kernels.hip).
+hipError_t launch_flip_configs(int64_t n, const uint64_t* configs, uint64_t index_base, int i0, int cnt, uint64_t* out,
+                               hipStream_t stream);
+hipError_t launch_axis_log_psi(int64_t n, int N, const uint64_t* configs, uint64_t index_base, const double* axis,
+                               cx<double>* lp, cx<double>* r, hipStream_t stream);
+hipError_t launch_axis_contract(int64_t n, int N, const uint64_t* configs, uint64_t index_base, const double* axis,
+                                const cx<double>* r, const cx<double>* coef, int slices, cx<double>* partials,
+                                hipStream_t stream);
 hipError_t launch_scalar_partials(int64_t n, const double* sc, int nblocks, double* partials, hipStream_t stream);
 hipError_t launch_lse_partials(int64_t n, const cx<double>* lp, double log_prefactor, int nblocks, double* partials,
                                hipStream_t stream);

@@ -97,3 +97,28 @@ def test_deep_hs_distance():
                                O.Exact(N), want_grad=True, nu=1.0)
         assert abs(d - dr) < TOL[precision] * dr
         assert rel_err(g, gr) < TOL[precision] * 10
+
+
+def test_deep_hs_distance_free_quantum_axis():
+    """free_quantum_axis with PsiDeep on both sides: distance and gradient,
+    alpha / beta entries included, against the oracle on the exact sum."""
+    N, M, C = 6, [12, 6], [3, 12]
+    b1, c1, w1, f1 = deep_layers(N, M, C, seed=5, scale=0.2)
+    b2, _, w2, f2 = deep_layers(N, M, C, seed=6, scale=0.2)
+    rng = np.random.default_rng(72)
+    alpha, beta = rng.uniform(-0.4, 0.4, N), rng.uniform(-0.4, 0.4, N)
+    alpha_p, beta_p = alpha + rng.uniform(-0.1, 0.1, N), beta + rng.uniform(-0.1, 0.1, N)
+    ident = (np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    cs = [np.asarray(c, np.uint32) for c in c1]
+    for precision in (64, 32):
+        print(f"[deep hs free axis] precision={precision}", flush=True)
+        psi = R.PsiDeep(alpha, beta, list(b1), cs, list(w1), np.asarray(f1, complex), 1.0, True, True, precision=precision)
+        psi2 = R.PsiDeep(alpha_p, beta_p, list(b2), cs, list(w2), np.asarray(f2, complex), 1.0, False, True,
+                         precision=precision)
+        hs = R.HilbertSpaceDistance(N, psi2.num_params, True)
+        g, d = hs.gradient(psi, psi2, R.Operator(*ident, True), True, R.ExactSummation(N, True), 1.0)
+        gr, dr = O.hs_distance(O.Deep(N, b1, c1, w1, f1), O.Deep(N, b2, c1, w2, f2), O.Operator(*ident), True,
+                               O.Exact(N), want_grad=True, nu=1.0, axis=(alpha, beta, alpha_p, beta_p))
+        assert abs(d - dr) < TOL[precision] * dr
+        assert rel_err(g, gr) < TOL[precision] * 10
+        assert rel_err(g[:2 * N], gr[:2 * N]) < TOL[precision] * 10

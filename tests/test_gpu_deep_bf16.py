@@ -236,6 +236,20 @@ def test_bf16_hs_distance_and_learned_ensemble():
     gr, dr = O.hs_distance(r1, r2, O.Operator(*ident), True, O.Exact(N), want_grad=True, nu=1.0)
     assert abs(d - dr) < 2e-3 * dr
     assert rel_err(g, gr) < 3e-2
+    # free quantum axis: psi' on the single-flip neighbours through the MFMA forward pass
+    rng = np.random.default_rng(73)
+    alpha, beta = rng.uniform(-0.4, 0.4, N), rng.uniform(-0.4, 0.4, N)
+    alpha_p, beta_p = alpha + rng.uniform(-0.1, 0.1, N), beta + rng.uniform(-0.1, 0.1, N)
+    cs = [np.asarray(c, np.uint32) for c in c1]
+    psia = R.PsiDeep(alpha, beta, list(b1), cs, list(w1), np.asarray(f1, complex), 1.0, True, True, precision=16,
+                     ref_compat_final=False)
+    psia2 = R.PsiDeep(alpha_p, beta_p, list(b2), cs, list(w2), np.asarray(f2, complex), 1.0, False, True, precision=16,
+                      ref_compat_final=False)
+    g, d = hs.gradient(psia, psia2, R.Operator(*ident, True), True, R.ExactSummation(N, True), 1.0)
+    gr, dr = O.hs_distance(r1, r2, O.Operator(*ident), True, O.Exact(N), want_grad=True, nu=1.0,
+                           axis=(alpha, beta, alpha_p, beta_p))
+    assert abs(d - dr) < 2e-3 * dr
+    assert rel_err(g, gr) < 3e-2
     # learned-proposal ensemble on the BF16 model
     tab = HAM.j1j2_chain(N)
     ex = R.ExactSummation(N, True)

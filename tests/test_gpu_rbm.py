@@ -218,6 +218,52 @@ def test_hs_distance_nonunitary_and_mc(precision):
         assert rel_err(gm, grm) < 1e-9
 
 
+def axis_angles(N, seed):
+    rng = np.random.default_rng(seed)
+    alpha, beta = rng.uniform(-0.4, 0.4, N), rng.uniform(-0.4, 0.4, N)
+    return alpha, beta, alpha + rng.uniform(-0.1, 0.1, N), beta + rng.uniform(-0.1, 0.1, N)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_hs_distance_free_quantum_axis(precision):
+    """free_quantum_axis on psi: distance and gradient, alpha / beta entries
+    included, against the oracle on the exact sum, on a magnetisation sector
+    and, in FP64, on the oracle's own chains."""
+    print(f"[hs free axis] precision={precision}", flush=True)
+    N, M = 7, 9
+    b, W, _ = random_rbm(N, M, seed=5, scale=0.2)
+    b2, W2, _ = random_rbm(N, M, seed=6, scale=0.2)
+    alpha, beta, alpha_p, beta_p = axis = axis_angles(N, 71)
+    tab = HAM.heisenberg([(i, (i + 1) % N) for i in range(N)])
+    op_t = (1e-2j * tab[0], tab[1], tab[2])
+    psi = R.Psi(alpha, beta, b, W, 1.0, True, True, precision=precision)
+    psi2 = R.Psi(alpha_p, beta_p, b2, W2, 1.0, False, True, precision=precision)
+    hs = R.HilbertSpaceDistance(N, psi2.num_params, True)
+    ex = R.ExactSummation(N, True)
+    g, d = hs.gradient(psi, psi2, gpu_op(op_t), False, ex, 0.5)
+    gr, dr = O.hs_distance(O.RBM(b, W), O.RBM(b2, W2), O.Operator(*op_t), False, O.Exact(N), want_grad=True, nu=0.5,
+                           axis=axis)
+    assert abs(d - dr) < TOL[precision] * dr
+    assert rel_err(g, gr) < TOL[precision] * 10
+    assert rel_err(g[:2 * N], gr[:2 * N]) < TOL[precision] * 10
+    assert abs(hs(psi, psi2, gpu_op(op_t), False, ex) - d) < 1e-12 * d
+    ex = R.ExactSummation(N, True)
+    ex.set_total_z_symmetry(1)
+    g, d = hs.gradient(psi, psi2, gpu_op(op_t), False, ex, 0.5)
+    gr, dr = O.hs_distance(O.RBM(b, W), O.RBM(b2, W2), O.Operator(*op_t), False, O.Exact(N, 1),
+                           want_grad=True, nu=0.5, axis=axis)
+    assert abs(d - dr) < TOL[precision] * dr
+    assert rel_err(g, gr) < TOL[precision] * 10
+    if precision == 64:
+        mc = R.MonteCarloLoop(32 * 40, 1, 5, 32, True, seed=11)
+        gm, dm = hs.gradient(psi, psi2, gpu_op(op_t), False, mc, 0.5)
+        omc = O.MonteCarlo(N, 32 * 40, 1, 5, 32, seed=11, call_index=0)
+        grm, drm = O.hs_distance(O.RBM(b, W), O.RBM(b2, W2), O.Operator(*op_t), False, omc, want_grad=True, nu=0.5,
+                                 axis=axis)
+        assert abs(dm - drm) < 1e-10 * drm
+        assert rel_err(gm, grm) < 1e-9
+
+
 # ------------------------------------------------------------------ Monte Carlo
 @pytest.mark.parametrize("visible", [False, True])
 def test_mc_chains_bit_identical_fp64(visible):

@@ -318,6 +318,70 @@ def test_hs_distance_identity_closed_form_and_gradient():
         assert abs(g[2 * N + k] - fd) < 1e-6
 
 
+def _site_op(A, i, N):
+    """Single-site operator A on spin i of N as a dense matrix over basis index
+    x = sum_j bit_j 2^j (spin j = +1 iff bit j is set)."""
+    out = np.ones((1, 1))
+    for j in reversed(range(N)):
+        out = np.kron(out, A if j == i else np.eye(2))
+    return out
+
+
+def _axis_closed_form(v, vp, N, alpha, beta, alpha_p, beta_p):
+    """Identity-operator, unitary distance with psi'_c = psi' exp((K psi') / psi'),
+    K = sum_i [da_i Z_i X_i + i db_i (cos_i X_i - sin_i Z_i)] built from dense
+    Pauli matrices (angles may be complex)."""
+    Z = np.diag([-1.0, 1.0])
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    K = np.zeros((2 ** N, 2 ** N), complex)
+    for i in range(N):
+        da, db = alpha_p[i] - alpha[i], beta_p[i] - beta[i]
+        c, s = np.cos(alpha_p[i] + alpha[i]), np.sin(alpha_p[i] + alpha[i])
+        K += da * _site_op(Z @ X, i, N) + 1j * db * (c * _site_op(X, i, N) - s * _site_op(Z, i, N))
+    vc = vp * np.exp((K @ vp) / vp)
+    return np.sqrt(1 - abs(np.vdot(vc, v)) ** 2 / (np.vdot(v, v).real * np.vdot(vc, vc).real))
+
+
+def test_hs_distance_free_axis_closed_form_and_gradient():
+    """free_quantum_axis (HilbertSpaceDistance.cu:47-80, 109-122): the distance
+    against the dense-operator closed form above, the alpha / beta gradient
+    entries against complex central differences of that closed form (beta at
+    any delta beta; alpha at delta beta = 0, where the reference's O_alpha is
+    the exact derivative), and zero axis change giving the plain distance."""
+    N, M = 4, 5
+    b, W, _ = random_rbm(N, M, seed=61, scale=0.3)
+    b2, W2, _ = random_rbm(N, M, seed=62, scale=0.3)
+    ident = O.Operator(np.array([1.0 + 0j]), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    psi, psi2 = O.RBM(b, W), O.RBM(b2, W2)
+    v, vp = O.psi_vector(psi), O.psi_vector(psi2)
+    rng = np.random.default_rng(63)
+    alpha, beta = rng.uniform(-0.4, 0.4, N), rng.uniform(-0.4, 0.4, N)
+    alpha_p, beta_p = alpha + rng.uniform(-0.1, 0.1, N), beta + rng.uniform(-0.1, 0.1, N)
+    g, d = O.hs_distance(psi, psi2, ident, True, O.Exact(N), want_grad=True, nu=1.0,
+                         axis=(alpha, beta, alpha_p, beta_p))
+    assert abs(d - _axis_closed_form(v, vp, N, alpha, beta, alpha_p, beta_p)) < 1e-12
+    h = 1e-6
+
+    def fd(f, x, i):
+        e = np.zeros(N)
+        e[i] = h
+        return (f(x + e) - f(x - e)) / (2 * h) + 1j * (f(x + 1j * e) - f(x - 1j * e)) / (2 * h)
+
+    for i in range(N):
+        gb = fd(lambda x: _axis_closed_form(v, vp, N, alpha, beta, alpha_p, x), beta_p.astype(complex), i)
+        assert abs(g[N + i] - gb) < 1e-6, (i, g[N + i], gb)
+    g0, d0 = O.hs_distance(psi, psi2, ident, True, O.Exact(N), want_grad=True, nu=1.0,
+                           axis=(alpha, beta, alpha_p, beta))
+    for i in range(N):
+        ga = fd(lambda x: _axis_closed_form(v, vp, N, alpha, beta, x, beta), alpha_p.astype(complex), i)
+        assert abs(g0[i] - ga) < 1e-6, (i, g0[i], ga)
+    # no change of axis: the plain distance and the plain parameter gradient
+    gz, dz = O.hs_distance(psi, psi2, ident, True, O.Exact(N), want_grad=True, nu=1.0,
+                           axis=(alpha, beta, alpha, beta))
+    gp, dp = O.hs_distance(psi, psi2, ident, True, O.Exact(N), want_grad=True, nu=1.0)
+    assert abs(dz - dp) < 1e-14 and np.allclose(gz[2 * N:], gp[2 * N:], rtol=0, atol=1e-14)
+
+
 def test_hs_distance_small_time_step():
     """Non-unitary operator i t H against sqrt(1 - |<psi|exp(iHt)|psi>|^2),
     the reference's test_distance1 (test_HilbertSpaceDistance.py:10-29)."""
