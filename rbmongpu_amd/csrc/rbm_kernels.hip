@@ -634,10 +634,12 @@ __global__ __launch_bounds__(256) void k_ok_reduce(int64_t n, int N, int M, cons
 
 // ---------------------------------------------------------------------------
 // O_k contraction on FP32 MFMA (v_mfma_f32_16x16x4f32).  Real GEMM over the
-// samples:  P[m][n] = sum_s A_m(s) B_n(s),
-//   m = 2r + p: A = (p ? Im : Re)(coef(s)) * row_r(s), rows {1, s_0..s_{N-1}}
-//   n = 2j + p: B = (p ? Im : Re)(col_j(s)), columns {T_0..T_{M-1}, 1}
-// so that R[r][j] = sum_s coef row_r conj(col_j) = (P_rr + P_ii) + i (P_ir - P_ri).
+// samples:  P[r][n] = sum_s A_r(s) B_n(s),
+//   A = row_r(s), rows {1, s_0..s_{N-1}} (exact +-1 values),
+//   n = 2j + p: B = (p ? Im : Re)(coef(s) conj(col_j(s))), columns {T_0..T_{M-1}, 1}
+// so that R[r][j] = sum_s coef row_r conj(col_j) = P[r][2j] + i P[r][2j+1].
+// The complex coefficient rides on the column side, so the rows are the N + 1
+// spin rows once rather than twice.
 // A K-step takes 4 samples: lane l feeds sample l/16 and tile row/column l%16
 // (A: lane l holds A[l%16][l/16], B: B[l/16][l%16], D: D[4(l/16)+v][l%16]).
 // One wavefront owns up to 4 x 8 output tiles of one sample chunk, accumulates
@@ -649,7 +651,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // RT x CT output tiles per wavefront, fixed at compile time so the K loop has
 // no branches and the next K-step's loads overlap the current MFMAs.  Rows
-// past 2(N+1) and columns past 2(M+1) are padding and contribute zero.
+// past N + 1 and columns past 2(M+1) are padding and contribute zero.
 //
 // Workgroups are numbered so that all tile groups of one sample chunk land on
 // the same XCD (workgroup b runs on XCD b mod 8) and start together: they read
@@ -673,20 +675,22 @@ __global__ __launch_bounds__(64) void k_ok_mfma(int64_t n, int N, int M, const f
     for (int a = 0; a < RT; a++)
 #pragma unroll
         for (int b = 0; b < CT; b++) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-    int rrow[RT], rpart[RT], coff[CT];
+    int rrow[RT], coff[CT];
     float cfix[CT], cload[CT];
 #pragma unroll
-    for (int a = 0; a < RT; a++) { const int m = 16 * (rt0 + a) + ii; rrow[a] = m >> 1; rpart[a] = m & 1; }
+    for (int a = 0; a < RT; a++) rrow[a] = 16 * (rt0 + a) + ii;
+    // column q = 2j + part: part = ii & 1 for every tile of this lane
+    const int part = ii & 1;
 #pragma unroll
     for (int b = 0; b < CT; b++) {
-        const int q = 16 * (ct0 + b) + ii, j = q >> 1;
-        coff[b] = j < M ? q : 0;                         // clamped: always a valid load
+        const int j = (16 * (ct0 + b) + ii) >> 1;
+        coff[b] = j < M ? j : 0;                         // clamped: always a valid load
         cload[b] = j < M ? 1.f : 0.f;                    // take the loaded value
-        cfix[b] = (j == M && (q & 1) == 0) ? 1.f : 0.f;  // the constant column
+        cfix[b] = j == M ? 1.f : 0.f;                    // the constant column
     }
     const int64_t s_begin = (int64_t)chunk_id * chunk;
     const int64_t s_end = s_begin + chunk < n ? s_begin + chunk : n;
-    const int twoM = 2 * M;
+    const float2* T2 = reinterpret_cast<const float2*>(Tf);
 #pragma unroll 2
     for (int64_t s0 = s_begin; s0 < s_end; s0 += 4) {
         const int64_t s = s0 + kk;
@@ -695,16 +699,20 @@ __global__ __launch_bounds__(64) void k_ok_mfma(int64_t n, int N, int M, const f
         const cx<double> c = coef[sc];
         const uint64_t w0 = spins[2 * sc], w1 = spins[2 * sc + 1];
         const float cre = ok ? (float)c.re : 0.f, cim = ok ? (float)c.im : 0.f;
-        const float* Tr = Tf + sc * twoM;
+        // Re(c conj t) = cre tr + cim ti,  Im(c conj t) = cim tr - cre ti
+        const float p = part ? cim : cre, q = part ? -cre : cim;
+        const float2* Tr = T2 + sc * M;
         float bv[CT];
 #pragma unroll
-        for (int b = 0; b < CT; b++) bv[b] = Tr[coff[b]] * cload[b] + cfix[b];
+        for (int b = 0; b < CT; b++) {
+            const float2 t = Tr[coff[b]];
+            bv[b] = __builtin_fmaf(p, __builtin_fmaf(t.x, cload[b], cfix[b]), q * (t.y * cload[b]));
+        }
         float av[RT];
 #pragma unroll
         for (int a = 0; a < RT; a++) {
             const int r = rrow[a];
-            const float sg = r == 0 ? 1.f : (r <= N ? spin_of<float>(w0, w1, (r - 1) & 127) : 0.f);
-            av[a] = (rpart[a] ? cim : cre) * sg;
+            av[a] = r == 0 ? 1.f : (r <= N ? spin_of<float>(w0, w1, (r - 1) & 127) : 0.f);
         }
 #pragma unroll
         for (int a = 0; a < RT; a++)
@@ -742,20 +750,18 @@ __global__ __launch_bounds__(256) void k_mfma_combine(int N, int M, int cols_p, 
     const int C = M + 1;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < (N + 1) * C; e += gridDim.x * blockDim.x) {
         const int r = e / C, j = e % C;
-        double rr = 0, ri = 0, ir = 0, ii = 0;
+        double re = 0, im = 0;
         for (int y = 0; y < S; y++) {
-            const double* Py = P + (int64_t)y * len;
-            rr += Py[(int64_t)(2 * r) * cols_p + 2 * j];
-            ri += Py[(int64_t)(2 * r) * cols_p + 2 * j + 1];
-            ir += Py[(int64_t)(2 * r + 1) * cols_p + 2 * j];
-            ii += Py[(int64_t)(2 * r + 1) * cols_p + 2 * j + 1];
+            const double* Py = P + (int64_t)y * len + (int64_t)r * cols_p + 2 * j;
+            re += Py[0];
+            im += Py[1];
         }
-        R[e] = mk(rr + ii, ir - ri);
+        R[e] = mk(re, im);
     }
 }
 
 static void mfma_tiling(int N, int M, int& RT, int& CT, int& rows_p, int& cols_p) {
-    const int rt = (2 * (N + 1) + 15) / 16, ct = (2 * (M + 1) + 15) / 16;
+    const int rt = (N + 1 + 15) / 16, ct = (2 * (M + 1) + 15) / 16;
     RT = rt < 4 ? rt : 4;
     static const int cts[] = {4, 8, 11, 13};
     const int groups = (ct + 12) / 13;
