@@ -614,11 +614,11 @@ struct rbmhip_made {
     int64_t off_b1 = 0, off_W2 = 0, off_b2 = 0, padded = 0;
     DevBuf params, grad, m, v;
     int t = 0;                  // Adam steps taken
-    DevBuf W1t_bf, W2_bf;       // bfloat16 inference copies
+    DevBuf W1t_bf, W2_bf, b1_sorted;   // bfloat16 inference copies, sampler layout
     bool inf_valid = false;
     ~rbmhip_made() {
         params.release(); grad.release(); m.release(); v.release();
-        W1t_bf.release(); W2_bf.release();
+        W1t_bf.release(); W2_bf.release(); b1_sorted.release();
     }
     int64_t num_params() const { return 2 * (int64_t)s.N * s.H + s.H + s.N; }
     float* P() { return reinterpret_cast<float*>(params.p); }
@@ -691,11 +691,14 @@ static MadeInf made_inf(Ctx& c, rbmhip_made* md) {
     const size_t bytes = (size_t)md->s.N * md->s.Hp * sizeof(uint16_t);
     md->W1t_bf.ensure(bytes);
     md->W2_bf.ensure(bytes);
+    md->b1_sorted.ensure((size_t)md->s.Hp * sizeof(float));
     if (!md->inf_valid) {
         const float* P = md->P();
         c.launch("made_inf_weights", [&] {
-            return launch_made_inference_weights(md->s, P, P + md->off_W2, reinterpret_cast<uint16_t*>(md->W1t_bf.p),
-                                                 reinterpret_cast<uint16_t*>(md->W2_bf.p), c.stream);
+            return launch_made_inference_weights(md->s, P, P + md->off_b1, P + md->off_W2,
+                                                 reinterpret_cast<uint16_t*>(md->W1t_bf.p),
+                                                 reinterpret_cast<uint16_t*>(md->W2_bf.p),
+                                                 reinterpret_cast<float*>(md->b1_sorted.p), c.stream);
         });
         md->inf_valid = true;
     }
@@ -703,7 +706,7 @@ static MadeInf made_inf(Ctx& c, rbmhip_made* md) {
     mi.N = md->s.N; mi.H = md->s.H; mi.Hp = md->s.Hp;
     mi.W1t = reinterpret_cast<const uint16_t*>(md->W1t_bf.p);
     mi.W2 = reinterpret_cast<const uint16_t*>(md->W2_bf.p);
-    mi.b1 = md->P() + md->off_b1;
+    mi.b1 = reinterpret_cast<const float*>(md->b1_sorted.p);
     mi.b2 = md->P() + md->off_b2;
     return mi;
 }

@@ -276,35 +276,50 @@ hipError_t launch_adam(int64_t n, float* p, const float* g, float* m, float* v, 
 }
 
 __global__ __launch_bounds__(256) void k_made_inf_weights(MadeShape s, const float* __restrict__ W1,
-                                                          const float* __restrict__ W2, uint16_t* __restrict__ W1t,
-                                                          uint16_t* __restrict__ W2b) {
+                                                          const float* __restrict__ b1, const float* __restrict__ W2,
+                                                          uint16_t* __restrict__ W1t, uint16_t* __restrict__ W2b,
+                                                          float* __restrict__ b1s) {
     const int64_t total = (int64_t)s.N * s.Hp;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(t / s.Hp), k = (int)(t % s.Hp);
+        const int i = (int)(t / s.Hp), e = (int)(t % s.Hp);
+        const int w = e >> 1, half = e & 1;                        // word w = 32 t + 2 l + b
+        const int v = 2 * (w >> 5) + (w & 1), l = (w >> 1) & 15;   // pair v = 2 t + b of lane l
+        const int p = 32 * v + 16 * half + l;                      // sorted position
+        const int k = made_sorted_unit(p, s.N, s.H);
         const __bf16 a = (__bf16)W1[(long)k * s.Np + i];
         const __bf16 b = (__bf16)W2[(long)i * s.Hp + k];
         W1t[t] = __builtin_bit_cast(uint16_t, a);
         W2b[t] = __builtin_bit_cast(uint16_t, b);
     }
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < s.Hp; p += gridDim.x * blockDim.x)
+        b1s[p] = b1[made_sorted_unit(p, s.N, s.H)];
 }
 
-hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, const float* W2, uint16_t* W1t_bf,
-                                         uint16_t* W2_bf, hipStream_t stream) {
-    hipLaunchKernelGGL(k_made_inf_weights, dim3(256), dim3(256), 0, stream, s, W1, W2, W1t_bf, W2_bf);
+hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, const float* b1, const float* W2,
+                                         uint16_t* W1t_bf, uint16_t* W2_bf, float* b1_sorted, hipStream_t stream) {
+    hipLaunchKernelGGL(k_made_inf_weights, dim3(256), dim3(256), 0, stream, s, W1, b1, W2, W1t_bf, W2_bf, b1_sorted);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // Autoregressive sampling (SCORE = false) and scoring (SCORE = true).
-// A 16-lane row owns one configuration; lane l holds hidden units
-// [l HPL, (l + 1) HPL) as FP32 pre-activations a and bfloat16 pairs of relu(a).
-// Per position j:
-//   l_j = b2_j + sum_k W2[j][k] relu(a_k)   (v_dot2c_f32_bf16, then a DPP row sum)
+// A 16-lane row owns two configurations (one above 256 hidden units),
+// interleaved so that their serial per-position chains (dot product, row sum,
+// sigmoid, update) overlap and each weight word read from LDS serves both.
+// Hidden units are in degree-sorted order: lane l holds sorted units 16 u + l
+// (u < HPL) as FP32 pre-activations a[u] and bfloat16 pairs hb[v] of
+// relu(a[2v]), relu(a[2v + 1]).  With D_j units of degree <= j, the masks make
+// output j read units [0, D_j) only and input j feed units [D_j, Hp) only, so
+// per position j (D_j is the same for every lane, and whole 64-unit groups of
+// zero weights are skipped):
+//   l_j = b2_j + sum_{k < D_j} W2[j][k] relu(a_k)   (v_dot2c_f32_bf16, DPP row sum)
 //   x_j drawn (or read), log q += log sigmoid(+-l_j)
-//   if x_j = 1: a += W1[:, j] (FP32 adds of the bfloat16 weights), repack relu(a)
-// Both weight matrices sit in LDS as bfloat16 (N x Hp each).
-// Uniforms: lane l of the row computes Philox blocks l and 16 + l of the
-// configuration; position j reads word j % 4 of block j / 4 from its lane.
+//   a_k += x_j W1[k][j] for k >= D_j (an FMA with x_j in {0, 1}: a + W exactly
+//   when x_j = 1, a unchanged when x_j = 0), relu(a) repacked
+// Both weight matrices sit in LDS as bfloat16 (N x Hp each, the layout of
+// launch_made_inference_weights).
+// Uniforms: lane l of the row computes Philox blocks l and 16 + l of each of
+// its configurations; position j reads word j % 4 of block j / 4 from its lane.
 // ---------------------------------------------------------------------------
 RBM_DI float dot2_bf16(uint32_t w, uint32_t h, float c) {
     return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, h), c, false);
@@ -318,11 +333,17 @@ RBM_DI uint32_t pack_relu_bf16(float lo, float hi) {
 
 int made_hidden_per_lane(int Hp) { return Hp / 16; }
 
+// configurations per 16-lane row: two while both fit in the 128 registers a
+// 1024-thread workgroup allows (up to 256 hidden units), else one
+constexpr int made_cpr(int hpl) { return hpl <= 16 ? 2 : 1; }
+
 template <int HPL, bool SCORE>
 __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint32_t seed_lo, uint32_t seed_hi,
                                                       uint32_t call, const uint64_t* __restrict__ in_spins,
                                                       uint64_t* __restrict__ out_spins, double* __restrict__ log_q) {
     static_assert(HPL % 4 == 0, "hidden units per lane must be a multiple of 4");
+    constexpr int C = made_cpr(HPL);
+    constexpr int V = HPL / 2;   // bfloat16 pairs per lane
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int N = m.N, Hp = m.Hp;
     uint16_t* sW1t = reinterpret_cast<uint16_t*>(smem_raw);
@@ -336,94 +357,133 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
         for (int k = threadIdx.x; k < nv; k += blockDim.x) { l1[k] = g1[k]; l2[k] = g2[k]; }
     }
     __syncthreads();
+    const uint32_t* sW1w = reinterpret_cast<const uint32_t*>(sW1t);
+    const uint32_t* sW2w = reinterpret_cast<const uint32_t*>(sW2);
     const int lane = threadIdx.x & 15, rows = blockDim.x >> 4;
     const int row_base = (threadIdx.x & 63) & ~15;
-    const int k0 = lane * HPL;
     float b1v[HPL];
 #pragma unroll
-    for (int u = 0; u < HPL; u++) b1v[u] = m.b1[k0 + u];
+    for (int u = 0; u < HPL; u++) b1v[u] = m.b1[16 * u + lane];
 
-    for (int64_t g = (int64_t)blockIdx.x * rows + (threadIdx.x >> 4); g < n; g += (int64_t)gridDim.x * rows) {
-        uint32_t uw[8];
-        uint64_t x0 = 0, x1 = 0;
-        if (SCORE) {
-            x0 = in_spins[2 * g];
-            x1 = in_spins[2 * g + 1];
-        } else {
-            const u32x4 r0 = philox((uint32_t)g, (uint32_t)(g >> 32), call, 0x100u + lane, seed_lo, seed_hi);
-            uw[0] = r0.x; uw[1] = r0.y; uw[2] = r0.z; uw[3] = r0.w;
-            if (N > 64) {
-                const u32x4 r1 = philox((uint32_t)g, (uint32_t)(g >> 32), call, 0x110u + lane, seed_lo, seed_hi);
-                uw[4] = r1.x; uw[5] = r1.y; uw[6] = r1.z; uw[7] = r1.w;
+    for (int64_t g0 = ((int64_t)blockIdx.x * rows + (threadIdx.x >> 4)) * C; g0 < n; g0 += (int64_t)gridDim.x * rows * C) {
+        int64_t gc[C];
+        uint32_t uw[C][8];
+        uint64_t x0[C], x1[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            gc[c] = g0 + c < n ? g0 + c : g0;      // a missing partner repeats the first (not written)
+            x0[c] = 0; x1[c] = 0;
+            if (SCORE) {
+                x0[c] = in_spins[2 * gc[c]];
+                x1[c] = in_spins[2 * gc[c] + 1];
             } else {
-                uw[4] = uw[5] = uw[6] = uw[7] = 0u;
+                const int64_t g = gc[c];
+                const u32x4 r0 = philox((uint32_t)g, (uint32_t)(g >> 32), call, 0x100u + lane, seed_lo, seed_hi);
+                uw[c][0] = r0.x; uw[c][1] = r0.y; uw[c][2] = r0.z; uw[c][3] = r0.w;
+                if (N > 64) {
+                    const u32x4 r1 = philox((uint32_t)g, (uint32_t)(g >> 32), call, 0x110u + lane, seed_lo, seed_hi);
+                    uw[c][4] = r1.x; uw[c][5] = r1.y; uw[c][6] = r1.z; uw[c][7] = r1.w;
+                } else {
+                    uw[c][4] = uw[c][5] = uw[c][6] = uw[c][7] = 0u;
+                }
             }
         }
-        float a[HPL];
-        uint32_t hb[HPL / 2];
+        float a[C][HPL];
+        uint32_t hb[C][V];
+        double lq[C];
 #pragma unroll
-        for (int u = 0; u < HPL; u++) a[u] = b1v[u];
+        for (int c = 0; c < C; c++) {
+            lq[c] = 0.0;
 #pragma unroll
-        for (int u = 0; u < HPL / 2; u++) hb[u] = pack_relu_bf16(a[2 * u], a[2 * u + 1]);
-        double lq = 0.0;
+            for (int u = 0; u < HPL; u++) a[c][u] = b1v[u];
+#pragma unroll
+            for (int v = 0; v < V; v++) hb[c][v] = pack_relu_bf16(a[c][2 * v], a[c][2 * v + 1]);
+        }
         for (int jb = 0; jb < N; jb += 4) {
-            uint32_t cur[4];
+            uint32_t cur[C][4];
             if (!SCORE) {
                 const int b = jb >> 2;
                 const int src = row_base | (b & 15);
 #pragma unroll
-                for (int q = 0; q < 4; q++) cur[q] = (uint32_t)__shfl((int)((b >> 4) ? uw[4 + q] : uw[q]), src, 64);
+                for (int c = 0; c < C; c++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        cur[c][q] = (uint32_t)__shfl((int)((b >> 4) ? uw[c][4 + q] : uw[c][q]), src, 64);
             }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int j = jb + q;
                 if (j >= N) break;
-                const uint2* w2 = reinterpret_cast<const uint2*>(sW2 + (size_t)j * Hp + k0);
-                float s0 = 0.0f, s1 = 0.0f;
+                const int D = made_prefix(j, N, m.H);
+                // pairs 2t, 2t + 1 (64 sorted units) per branch and LDS read
+                const uint2* w2 = reinterpret_cast<const uint2*>(sW2w + (size_t)j * (Hp / 2)) + lane;
+                const uint2* w1 = reinterpret_cast<const uint2*>(sW1w + (size_t)j * (Hp / 2)) + lane;
+                float s0[C], s1[C];
 #pragma unroll
-                for (int c = 0; c < HPL / 4; c++) {
-                    const uint2 w = w2[c];
-                    s0 = dot2_bf16(w.x, hb[2 * c], s0);
-                    s1 = dot2_bf16(w.y, hb[2 * c + 1], s1);
-                }
-                const float l = row_sum16(s0 + s1) + m.b2[j];
-                const float e = __expf(-fabsf(l));
-                const float L1 = __logf(1.0f + e);
-                bool x;
-                if (SCORE) {
-                    x = bit_of(x0, x1, j) != 0;
-                } else {
-                    const float r = __builtin_amdgcn_rcpf(1.0f + e);
-                    const float p = l >= 0.0f ? r : e * r;
-                    const float u = __builtin_fmaf((float)cur[q], 2.3283064365386963e-10f, 1.1641532182693481e-10f);
-                    x = u < p;
-                    if (x) {
-                        if (j < 64) x0 |= (uint64_t)1 << j; else x1 |= (uint64_t)1 << (j - 64);
+                for (int c = 0; c < C; c++) { s0[c] = 0.0f; s1[c] = 0.0f; }
+#pragma unroll
+                for (int t = 0; t < V / 2; t++) {
+                    if (64 * t < D) {
+                        const uint2 w = w2[16 * t];
+#pragma unroll
+                        for (int c = 0; c < C; c++) {
+                            s0[c] = dot2_bf16(w.x, hb[c][2 * t], s0[c]);
+                            s1[c] = dot2_bf16(w.y, hb[c][2 * t + 1], s1[c]);
+                        }
                     }
                 }
-                // log sigmoid(l) = -(max(-l, 0) + log(1 + e^-|l|)), log(1 - sigmoid(l)) = -(max(l, 0) + ...)
-                lq -= (double)((x ? fmaxf(-l, 0.0f) : fmaxf(l, 0.0f)) + L1);
-                if (x) {
-                    const uint2* w1 = reinterpret_cast<const uint2*>(sW1t + (size_t)j * Hp + k0);
+                const float b2j = m.b2[j];
+                float xf[C];
 #pragma unroll
-                    for (int c = 0; c < HPL / 4; c++) {
-                        const uint2 w = w1[c];
-                        a[4 * c + 0] += __uint_as_float(w.x << 16);
-                        a[4 * c + 1] += __uint_as_float(w.x & 0xFFFF0000u);
-                        a[4 * c + 2] += __uint_as_float(w.y << 16);
-                        a[4 * c + 3] += __uint_as_float(w.y & 0xFFFF0000u);
+                for (int c = 0; c < C; c++) {
+                    const float l = row_sum16(s0[c] + s1[c]) + b2j;
+                    const float e = __expf(-fabsf(l));
+                    const float L1 = __logf(1.0f + e);
+                    bool x;
+                    if (SCORE) {
+                        x = bit_of(x0[c], x1[c], j) != 0;
+                    } else {
+                        const float r = __builtin_amdgcn_rcpf(1.0f + e);
+                        const float p = l >= 0.0f ? r : e * r;
+                        const float u = __builtin_fmaf((float)cur[c][q], 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+                        x = u < p;
+                        const uint64_t bit = x ? (uint64_t)1 << (j & 63) : 0u;
+                        if (j < 64) x0[c] |= bit; else x1[c] |= bit;
                     }
+                    // log sigmoid(l) = -(max(-l, 0) + log(1 + e^-|l|)), log(1 - sigmoid(l)) = -(max(l, 0) + ...)
+                    lq[c] -= (double)((x ? fmaxf(-l, 0.0f) : fmaxf(l, 0.0f)) + L1);
+                    xf[c] = x ? 1.0f : 0.0f;
+                }
 #pragma unroll
-                    for (int u = 0; u < HPL / 2; u++) hb[u] = pack_relu_bf16(a[2 * u], a[2 * u + 1]);
+                for (int t = 0; t < V / 2; t++) {
+                    if (64 * t + 64 > D) {
+                        const uint2 w = w1[16 * t];
+#pragma unroll
+                        for (int b = 0; b < 2; b++) {
+                            const uint32_t wb = b ? w.y : w.x;
+                            const int v = 2 * t + b;
+                            const float wlo = __uint_as_float(wb << 16), whi = __uint_as_float(wb & 0xFFFF0000u);
+#pragma unroll
+                            for (int c = 0; c < C; c++) {
+                                a[c][2 * v] = __builtin_fmaf(xf[c], wlo, a[c][2 * v]);
+                                a[c][2 * v + 1] = __builtin_fmaf(xf[c], whi, a[c][2 * v + 1]);
+                                hb[c][v] = pack_relu_bf16(a[c][2 * v], a[c][2 * v + 1]);
+                            }
+                        }
+                    }
                 }
             }
         }
         if (lane == 0) {
-            if (!SCORE) {
-                out_spins[2 * g] = x0;
-                out_spins[2 * g + 1] = x1;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                if (g0 + c >= n) break;
+                if (!SCORE) {
+                    out_spins[2 * (g0 + c)] = x0[c];
+                    out_spins[2 * (g0 + c) + 1] = x1[c];
+                }
+                log_q[g0 + c] = lq[c];
             }
-            log_q[g] = lq;
         }
     }
 }
@@ -440,7 +500,7 @@ static hipError_t launch_made_common(const MadeInf& m, int64_t n, uint32_t seed_
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const int hpl = made_hidden_per_lane(m.Hp);
     const int rows = 64;
-    int64_t blocks = (n + rows - 1) / rows;
+    int64_t blocks = (n + rows * made_cpr(hpl) - 1) / (rows * made_cpr(hpl));
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

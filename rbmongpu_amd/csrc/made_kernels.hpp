@@ -26,6 +26,26 @@ inline int made_pad(int x) { return (x + 63) / 64 * 64; }
 // degree of hidden unit k: 1 + k mod (N - 1) (0 for N = 1)
 __host__ __device__ inline int made_degree(int k, int N) { return N > 1 ? 1 + k % (N - 1) : 0; }
 
+// Number of hidden units of degree <= j: the units output j reads.  Degree d
+// (1 <= d <= N - 1) holds q + [d <= r] units, q = H / (N - 1), r = H mod (N - 1).
+__host__ __device__ inline int made_prefix(int j, int N, int H) {
+    if (N <= 1 || j >= N - 1) return H;
+    if (j <= 0) return 0;
+    const int q = H / (N - 1), r = H % (N - 1);
+    return j * q + (j < r ? j : r);
+}
+
+// Hidden unit at position p of the degree-sorted order (ties by unit index);
+// positions >= H are the zero padding units and map to themselves.
+__host__ __device__ inline int made_sorted_unit(int p, int N, int H) {
+    if (N <= 1 || p >= H) return p;
+    const int q = H / (N - 1), r = H % (N - 1);
+    int d1, t;   // degree - 1 and rank among the units of that degree
+    if (p < r * (q + 1)) { d1 = p / (q + 1); t = p % (q + 1); }
+    else { const int p2 = p - r * (q + 1); d1 = r + p2 / q; t = p2 % q; }
+    return d1 + t * (N - 1);
+}
+
 enum GemmEpi {
     EPI_STORE = 0,       // C = acc
     EPI_BIAS_RELU = 1,   // C = acc + bias[n] (pre-activation), C2 = relu(C)
@@ -75,15 +95,19 @@ hipError_t launch_zero(int64_t n, float* p, hipStream_t stream);
 // Adam (Kingma & Ba 2015) over n parameters; t is the 1-based step
 hipError_t launch_adam(int64_t n, float* p, const float* g, float* m, float* v, int t, float lr, float b1, float b2,
                        float eps, hipStream_t stream);
-// bfloat16 inference copies of W1 (transposed) and W2
-hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, const float* W2, uint16_t* W1t_bf,
-                                         uint16_t* W2_bf, hipStream_t stream);
+// bfloat16 inference copies of W1 (transposed) and W2 in the sampler's layout:
+// hidden units in degree-sorted order (made_sorted_unit), row j of either
+// matrix as Hp / 2 words, word 32 t + 2 l + b holding pair v = 2 t + b of
+// lane l: sorted units 32 v + l (low half) and 32 v + 16 + l (high half);
+// b1 permuted to the sorted order.
+hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, const float* b1, const float* W2,
+                                         uint16_t* W1t_bf, uint16_t* W2_bf, float* b1_sorted, hipStream_t stream);
 
 struct MadeInf {
     int N, H, Hp;
-    const uint16_t* W1t;   // N x Hp
-    const uint16_t* W2;    // N x Hp
-    const float* b1;       // Hp (zero padded)
+    const uint16_t* W1t;   // N x Hp, sampler layout
+    const uint16_t* W2;    // N x Hp, sampler layout
+    const float* b1;       // Hp, degree-sorted (zero padded)
     const float* b2;       // >= N
 };
 size_t made_sampler_lds(const MadeInf& m);

@@ -115,6 +115,19 @@ def test_sampler_log_q_is_its_own_score(N, H):
     assert not np.array_equal(spins, spins2)
 
 
+@pytest.mark.parametrize("N,H", [(20, 64), (100, 256), (30, 320)])
+def test_sampler_draws_depend_on_the_proposal_index_only(N, H):
+    """Proposal g's uniforms come from its own Philox counter, so a sample of
+    n + 1 (odd) starts with the sample of n, whatever rows share a wavefront."""
+    md = R.MADE(N, H, params=rand_params(N, H, seed=7, scale=0.3))
+    a, la = md.sample(2048, seed=3)
+    b, lb = md.sample(2049, seed=3)
+    assert np.array_equal(a, b[:2048]) and np.array_equal(la, lb[:2048])
+    c, lc = md.sample(1, seed=3)
+    assert np.array_equal(c, a[:1]) and np.array_equal(lc, la[:1])
+    assert np.array_equal(lb, md.log_q(b))
+
+
 def test_sampler_draws_q():
     N, H = 6, 16
     p = rand_params(N, H, seed=21, scale=0.8)
