@@ -35,6 +35,8 @@ from .json_numpy import NumpyEncoder, NumpyDecoder  # noqa: E402
 from .new_neural_network import new_neural_network, new_deep_neural_network, complex_noise, real_noise  # noqa: E402,F401
 from . import hamiltonians  # noqa: E402,F401
 from . import gradient_descent  # noqa: E402,F401
+from .gradient_descent import *  # noqa: E402,F401,F403
+from .learning import LearningByGradientDescent, DidNotConverge, L2Regularization  # noqa: E402,F401
 
 
 def _normalize_psi(self, exact_summation):
@@ -85,7 +87,14 @@ def _deep_from_json(json_obj, gpu):
 
 
 def _first_layer_params_slice(self):
+    """PsiDeep.first_layer_params_slice (pyRBMonGPU/PsiDeep.py:79-81)."""
     return slice(2 * self.N, 2 * self.N + self.W[0].size + self.b[0].size)
+
+
+def _rbm_params_slice(self):
+    """Psi has one layer: b, W (and a) -- the parameters the driver's L2
+    regularisation acts on (the reference defines the slice for PsiDeep only)."""
+    return slice(2 * self.N, self.num_params)
 
 
 setattr(Psi, "normalize", _normalize_psi)
@@ -95,3 +104,4 @@ setattr(PsiDeep, "normalize", _normalize_deep)
 setattr(PsiDeep, "to_json", _deep_to_json)
 setattr(PsiDeep, "from_json", _deep_from_json)
 setattr(PsiDeep, "first_layer_params_slice", property(_first_layer_params_slice))
+setattr(Psi, "first_layer_params_slice", property(_rbm_params_slice))

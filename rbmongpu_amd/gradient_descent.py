@@ -25,6 +25,10 @@ from itertools import islice
 
 import numpy as np
 
+__all__ = ["pack", "unpack", "vanilla_generator", "vanilla", "momentum", "nag_generator", "nag", "adagrad", "adadelta",
+           "rmsprop_generator", "rmsprop", "adam_generator", "adam", "padam_generator", "padam", "adamax_generator",
+           "adamax", "sr_generator"]
+
 
 def pack(array):
     """[Re p, Im p] -> p."""
