@@ -370,7 +370,8 @@ def run_c4(args, R, HAM, F):
     mc = R.MonteCarloLoop(CH * spc, 1, THERM, CH, True, seed=4)
     ev = R.ExpectationValue(True)
     mc_label = "deep_mfma_mc" if prec == 16 else "deep_mc"
-    kernels = (mc_label, "deep_mfma_eval", "deep_ok", "coef", "ok_reduce", "partials_sum", "scalar_reduce")
+    kernels = (mc_label, "deep_mfma_eval", "deep_grad", "deep_grad_rows", "deep_grad_gemm", "deep_ok", "coef", "ok_reduce",
+               "partials_sum", "scalar_reduce")
     ms, E, kms = time_steps(R, ev, psi, op, mc, args.steps, args.warmup, kernels)
     _, _, el = mc.last_samples()
     tau, mean, se = chain_stats(el, spc, CH)
@@ -504,7 +505,7 @@ def run_c3(args, R, HAM, F):
         "tau_int": tau, "energy": E.real, "energy_se": se,
         "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_gemm (S Gram blocks)",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_gram_lower (S Gram, lower 128x128 tiles)",
                      "flop_per_step": flop, "ms_per_step": gram_ms / args.steps},
         "fp64_recomputation": {"samples": len(s), "S_rel_err": rel(S_gpu, S_ref),
                                "gradient_rel_err": rel(g_gpu[2 * N:], g_ref),
