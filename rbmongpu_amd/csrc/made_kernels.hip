@@ -361,9 +361,6 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
     const uint32_t* sW2w = reinterpret_cast<const uint32_t*>(sW2);
     const int lane = threadIdx.x & 15, rows = blockDim.x >> 4;
     const int row_base = (threadIdx.x & 63) & ~15;
-    float b1v[HPL];
-#pragma unroll
-    for (int u = 0; u < HPL; u++) b1v[u] = m.b1[16 * u + lane];
 
     for (int64_t g0 = ((int64_t)blockIdx.x * rows + (threadIdx.x >> 4)) * C; g0 < n; g0 += (int64_t)gridDim.x * rows * C) {
         int64_t gc[C];
@@ -391,11 +388,12 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
         float a[C][HPL];
         uint32_t hb[C][V];
         double lq[C];
+        // b1 is re-read (cache hits) rather than held in registers for the whole loop
 #pragma unroll
         for (int c = 0; c < C; c++) {
             lq[c] = 0.0;
 #pragma unroll
-            for (int u = 0; u < HPL; u++) a[c][u] = b1v[u];
+            for (int u = 0; u < HPL; u++) a[c][u] = m.b1[16 * u + lane];
 #pragma unroll
             for (int v = 0; v < V; v++) hb[c][v] = pack_relu_bf16(a[c][2 * v], a[c][2 * v + 1]);
         }

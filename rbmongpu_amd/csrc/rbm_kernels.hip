@@ -35,14 +35,19 @@ template <typename T> RBM_DI T spin_factor(uint64_t s0, uint64_t s1, int p) {
 // Each lane sums lc(theta'_j) over its hidden units and subtracts lsum, the
 // same sum at theta (padded units, theta = theta' = 0, cancel); the row adds
 // the lanes.  Every off-diagonal term flips at least one spin, so theta' is
-// initialised from the first flip without a copy of theta.
+// initialised from the first flip without a copy of theta.  The per-term
+// tail (coefficient, exp of the log ratio, FP64 accumulation) is spread over
+// the row: lane t mod 16 keeps term t's log ratio, and after every 16 terms
+// each lane finishes its own term; the lanes' sums are added at the end.
 // ---------------------------------------------------------------------------
 template <typename T, int JPL>
 RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const OpDev& op,
                                    uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], cx<T> lsum, int lane) {
     const int ld = md.ld;
     const cx<T>* __restrict__ Wl = Wsrc + lane;
-    cx<double> E = diag_energy(op, s0, s1);
+    cx<double> E = mk(0.0, 0.0);          // this lane's terms
+    cx<T> dmine = mk(T(0), T(0));
+    int tmine = -1;
     for (int t = 0; t < op.n_terms; t++) {
         const int fb = ldc(op.flip_begin, t), fe = ldc(op.flip_begin, t + 1);
         cx<T> v[JPL];
@@ -68,10 +73,13 @@ RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict_
                 d = d + (T(-2) * spin_of<T>(s0, s1, pos)) * md.a[pos];
             }
         }
-        const cx<double> c = term_coef(op, t, s0, s1);
-        E = E + c * cexp_wide(d);
+        if ((t & 15) == lane) { dmine = d; tmine = t; }
+        if ((t & 15) == 15 || t == op.n_terms - 1) {
+            if (tmine >= 0) E = E + term_coef(op, tmine, s0, s1) * cexp_wide(dmine);
+            tmine = -1;
+        }
     }
-    return E;
+    return row_sum16(E) + diag_energy(op, s0, s1);
 }
 
 template <typename T>
