@@ -38,7 +38,8 @@ template <typename T> RBM_DI T spin_factor(uint64_t s0, uint64_t s1, int p) {
 // initialised from the first flip without a copy of theta.  The per-term
 // tail (coefficient, exp of the log ratio, FP64 accumulation) is spread over
 // the row: lane t mod 16 keeps term t's log ratio, and after every 16 terms
-// each lane finishes its own term; the lanes' sums are added at the end.
+// each lane finishes its own term; the diagonal strings are dealt out the same
+// way, and the lanes' sums are added at the end.
 // ---------------------------------------------------------------------------
 template <typename T, int JPL>
 RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const OpDev& op,
@@ -46,6 +47,9 @@ RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict_
     const int ld = md.ld;
     const cx<T>* __restrict__ Wl = Wsrc + lane;
     cx<double> E = mk(0.0, 0.0);          // this lane's terms
+    // diagonal strings d = lane, lane + 16, ... of this lane
+    for (int d = lane; d < op.n_diag; d += 16)
+        E = E + zy_sign(op.diag_zy[2 * d], op.diag_zy[2 * d + 1], s0, s1) * op.diag_coef[d];
     cx<T> dmine = mk(T(0), T(0));
     int tmine = -1;
     for (int t = 0; t < op.n_terms; t++) {
@@ -79,7 +83,7 @@ RBM_DI cx<double> local_energy_rbm(const RbmDev<T>& md, const cx<T>* __restrict_
             tmine = -1;
         }
     }
-    return row_sum16(E) + diag_energy(op, s0, s1);
+    return row_sum16(E);
 }
 
 template <typename T>
