@@ -1112,90 +1112,125 @@ hipError_t launch_s_matrix_ref(int64_t n, int N, int M, int has_a, const void* T
 }
 
 // ---------------------------------------------------------------------------
-// log psi of n configurations as an FP32 MFMA GEMM (RBM without visible bias):
-// theta = b + W^T s over 64 configurations per wave, the hidden units in
-// chunks of 32 whose W columns are staged in LDS, my_logcosh summed over the
-// units in the epilogue (Psi::log_psi_s, include/quantum_state/Psi.hpp:55-63).
-// MFMA 16x16x4 f32: lane l supplies A[l % 16][l / 16] (spin s_k of
-// configuration row l % 16, k = k0 + l / 16, as +-1) and B[l / 16][l % 16]
+// log psi of n configurations (RBM without visible bias), theta = b + W^T s on
+// the bfloat16 matrix cores with FP32 arithmetic: the +-1 spins are exact in
+// bfloat16 and every FP32 entry of W is split exactly into three bfloat16
+// parts, W = hi + mid + lo (hi = bf16(W), mid = bf16(W - hi), lo = W - hi -
+// mid), so each product s_k W_kj is exact and the MFMA sums it in FP32, as an
+// FP32 GEMM does.  my_logcosh is summed over the units in the epilogue
+// (Psi::log_psi_s, include/quantum_state/Psi.hpp:55-63).
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[l % 16][8 (l / 16) + q] (spin k =
+// 32 kb + 8 (l / 16) + q of configuration row l % 16) and B[8 (l / 16) + q][l % 16]
 // (W[k][unit]); accumulator element v of lane l is D[4 (l / 16) + v][l % 16].
+// A workgroup of 8 waves takes 256 configurations (two 16-row tiles per wave)
+// and stages the split W of 16 units at a time in LDS as B fragments (24 KiB).
 // ---------------------------------------------------------------------------
-constexpr int TL_ROWS = 64, TL_UNITS = 32, TL_LDS_STRIDE = TL_UNITS + 1;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+constexpr int TL_WAVES = 8, TL_TILES = 2, TL_CT = 1, TL_UNITS = 16 * TL_CT, TL_KBMAX = 4;
 
-__global__ __launch_bounds__(256) void k_theta_logpsi(RbmDev<float> md, int64_t n, const uint64_t* __restrict__ spins,
+RBM_DI uint32_t bf16_pair(float lo, float hi) {
+    const __bf16 a = (__bf16)lo, b = (__bf16)hi;
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+// three-part split of two FP32 values into packed bfloat16 pairs
+RBM_DI void split3(float x0, float x1, uint32_t& hi, uint32_t& mid, uint32_t& lo) {
+    hi = bf16_pair(x0, x1);
+    const float r0 = x0 - __uint_as_float(hi << 16), r1 = x1 - __uint_as_float(hi & 0xFFFF0000u);
+    mid = bf16_pair(r0, r1);
+    lo = bf16_pair(r0 - __uint_as_float(mid << 16), r1 - __uint_as_float(mid & 0xFFFF0000u));
+}
+
+__global__ __launch_bounds__(512) void k_theta_logpsi(RbmDev<float> md, int64_t n, const uint64_t* __restrict__ spins,
                                                       cx<double>* __restrict__ lp) {
-    __shared__ float2 Ws[128 * TL_LDS_STRIDE];
+    // B fragments of one chunk: [kb][column tile c][re, im][hi, mid, lo][lane]
+    __shared__ uint4 Wf[TL_KBMAX][TL_CT][2][3][64];
     const int N = md.N, M = md.M, ld = md.ld;
-    const int Kp = (N + 3) & ~3;
+    const int KB = (N + 31) >> 5;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * TL_ROWS;
-    // spin words of the configuration of A-operand row (lane % 16) in each row tile
-    uint32_t w[4][4];
+    const int g = lane >> 4;
+    const int64_t base = ((int64_t)blockIdx.x * TL_WAVES + wave) * (16 * TL_TILES);
+    // A fragments: 8 spins per K block as bfloat16 +-1 (0 past N)
+    uint32_t af[TL_TILES][TL_KBMAX][4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < TL_TILES; r++) {
         const int64_t p = base + 16 * r + (lane & 15);
         uint64_t s0 = 0, s1 = 0;
         if (p < n) { s0 = spins[2 * p]; s1 = spins[2 * p + 1]; }
-        w[r][0] = (uint32_t)s0; w[r][1] = (uint32_t)(s0 >> 32);
-        w[r][2] = (uint32_t)s1; w[r][3] = (uint32_t)(s1 >> 32);
-    }
-    float lre[4][4], lim[4][4];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+        for (int kb = 0; kb < TL_KBMAX; kb++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k0 = 32 * kb + 8 * g + 2 * q;
+                const uint32_t e0 = k0 < N ? (bit_of(s0, s1, k0) ? 0x3F80u : 0xBF80u) : 0u;
+                const uint32_t e1 = k0 + 1 < N ? (bit_of(s0, s1, k0 + 1) ? 0x3F80u : 0xBF80u) : 0u;
+                af[r][kb][q] = e0 | (e1 << 16);
+            }
+    }
+    float lre[TL_TILES][4], lim[TL_TILES][4];
+#pragma unroll
+    for (int r = 0; r < TL_TILES; r++)
 #pragma unroll
         for (int v = 0; v < 4; v++) { lre[r][v] = 0.0f; lim[r][v] = 0.0f; }
-    const int kq = lane >> 4;
     for (int u0 = 0; u0 < ld; u0 += TL_UNITS) {
         __syncthreads();
-        for (int t = threadIdx.x; t < Kp * TL_UNITS; t += blockDim.x) {
-            const int k = t / TL_UNITS, j = t % TL_UNITS;
-            float2 x = make_float2(0.0f, 0.0f);
-            if (k < N && u0 + j < ld) {
-                const cx<float> c = md.W[k * ld + u0 + j];
-                x = make_float2(c.re, c.im);
+        // fragment slot (kb, c, L): unit u0 + 16 c + L % 16, k = 32 kb + 8 (L / 16) + q
+        for (int f = threadIdx.x; f < KB * TL_CT * 64; f += blockDim.x) {
+            const int kb = f / (TL_CT * 64), c = (f >> 6) % TL_CT, L = f & 63;
+            const int unit = u0 + 16 * c + (L & 15);
+            const int kbase = 32 * kb + 8 * (L >> 4);
+            uint32_t h[2][4], m[2][4], o[2][4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                cx<float> w0 = mk(0.0f, 0.0f), w1 = mk(0.0f, 0.0f);
+                const int k0 = kbase + 2 * q;
+                if (unit < ld) {
+                    if (k0 < N) w0 = md.W[k0 * ld + unit];
+                    if (k0 + 1 < N) w1 = md.W[(k0 + 1) * ld + unit];
+                }
+                split3(w0.re, w1.re, h[0][q], m[0][q], o[0][q]);
+                split3(w0.im, w1.im, h[1][q], m[1][q], o[1][q]);
             }
-            Ws[k * TL_LDS_STRIDE + j] = x;
+#pragma unroll
+            for (int ri = 0; ri < 2; ri++) {
+                Wf[kb][c][ri][0][L] = make_uint4(h[ri][0], h[ri][1], h[ri][2], h[ri][3]);
+                Wf[kb][c][ri][1][L] = make_uint4(m[ri][0], m[ri][1], m[ri][2], m[ri][3]);
+                Wf[kb][c][ri][2][L] = make_uint4(o[ri][0], o[ri][1], o[ri][2], o[ri][3]);
+            }
         }
         __syncthreads();
-        floatx4 acc[4][2][2];
+        floatx4 acc[TL_TILES][TL_CT][2];
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+        for (int r = 0; r < TL_TILES; r++)
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
+            for (int c = 0; c < TL_CT; c++) {
                 acc[r][c][0] = floatx4{0.f, 0.f, 0.f, 0.f};
                 acc[r][c][1] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
-        for (int k0 = 0; k0 < Kp; k0 += 4) {
-            const int k = k0 + kq;
-            const int word = k0 >> 5, sh = k & 31;   // k0 .. k0 + 3 share one 32-bit word
-            float a[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const uint32_t x = word == 0 ? w[r][0] : word == 1 ? w[r][1] : word == 2 ? w[r][2] : w[r][3];
-                a[r] = ((x >> sh) & 1u) ? 1.0f : -1.0f;
-            }
-            float bre[2], bim[2];
+        for (int kb = 0; kb < TL_KBMAX; kb++) {
+            if (kb >= KB) break;
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
-                const float2 x = Ws[k * TL_LDS_STRIDE + 16 * c + (lane & 15)];
-                bre[c] = x.x;
-                bim[c] = x.y;
-            }
+            for (int c = 0; c < TL_CT; c++)
 #pragma unroll
-            for (int r = 0; r < 4; r++)
+                for (int ri = 0; ri < 2; ri++)
 #pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    acc[r][c][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bre[c], acc[r][c][0], 0, 0, 0);
-                    acc[r][c][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], bim[c], acc[r][c][1], 0, 0, 0);
-                }
+                    for (int part = 2; part >= 0; part--) {   // smallest parts first
+                        const bf16x8_t B = __builtin_bit_cast(bf16x8_t, Wf[kb][c][ri][part][lane]);
+#pragma unroll
+                        for (int r = 0; r < TL_TILES; r++) {
+                            const bf16x8_t A = __builtin_bit_cast(
+                                bf16x8_t, make_uint4(af[r][kb][0], af[r][kb][1], af[r][kb][2], af[r][kb][3]));
+                            acc[r][c][ri] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[r][c][ri], 0, 0, 0);
+                        }
+                    }
         }
 #pragma unroll
-        for (int c = 0; c < 2; c++) {
+        for (int c = 0; c < TL_CT; c++) {
             const int j = u0 + 16 * c + (lane & 15);
             if (j < M) {
                 const cx<float> bj = md.b[j];
 #pragma unroll
-                for (int r = 0; r < 4; r++)
+                for (int r = 0; r < TL_TILES; r++)
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         const cx<float> lc = logcosh(mk(acc[r][c][0][v] + bj.re, acc[r][c][1][v] + bj.im));
@@ -1206,21 +1241,22 @@ __global__ __launch_bounds__(256) void k_theta_logpsi(RbmDev<float> md, int64_t 
         }
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int r = 0; r < TL_TILES; r++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const float sr = row_sum16(lre[r][v]);
             const float si = row_sum16(lim[r][v]);
-            const int64_t p = base + 16 * r + 4 * kq + v;
+            const int64_t p = base + 16 * r + 4 * g + v;
             if ((lane & 15) == 0 && p < n) lp[p] = mk((double)sr, (double)si);
         }
 }
 
 hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_t* spins, cx<double>* lp,
                                hipStream_t stream) {
-    if (md.a || md.N > 128 || n <= 0) return hipErrorInvalidValue;
-    const int64_t blocks = (n + 4 * TL_ROWS - 1) / (4 * TL_ROWS);
-    hipLaunchKernelGGL(k_theta_logpsi, dim3((unsigned)blocks), dim3(256), 0, stream, md, n, spins, lp);
+    if (md.a || md.N > 32 * TL_KBMAX || n <= 0) return hipErrorInvalidValue;
+    const int64_t per_block = (int64_t)TL_WAVES * 16 * TL_TILES;
+    const int64_t blocks = (n + per_block - 1) / per_block;
+    hipLaunchKernelGGL(k_theta_logpsi, dim3((unsigned)blocks), dim3(64 * TL_WAVES), 0, stream, md, n, spins, lp);
     return hipGetLastError();
 }
 

@@ -130,8 +130,9 @@ hipError_t launch_o_realified(int64_t s0, int64_t nb, int64_t nbp, int N, int M,
                               hipStream_t stream);
 hipError_t launch_accum_f64(int64_t n, const float* g, double* acc, hipStream_t stream);
 hipError_t launch_s_assemble(int P, int ld2, double inv, double eps, const double* G, cx<double>* S, hipStream_t stream);
-// log psi of n configurations of an FP32 RBM without visible bias (N <= 128) as
-// an MFMA GEMM theta = b + W^T s with a log-cosh epilogue.
+// log psi of n configurations of an FP32 RBM without visible bias (N <= 128):
+// theta = b + W^T s on bfloat16 MFMA with W split exactly into three bfloat16
+// parts (exact products, FP32 sums), my_logcosh summed in the epilogue.
 hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_t* spins, cx<double>* lp,
                                hipStream_t stream);
 
