@@ -6,7 +6,7 @@ proposals.
 Default workload (--config c5, the learned-proposal configuration): 2-D
 transverse-field Ising model on a 10x10 periodic lattice, J = 1, h = 3.04;
 RBM with M = 200 hidden units, complex64 parameters drawn i.i.d. N(0, 0.01^2)
-(seed 1238), then 50 Adam steps (lr 1e-2) on the energy gradient from 8192
+(seed 1238), then 50 Adam steps (lr 5e-3) on the energy gradient from 8192
 chains x 10 samples.  Both samplers then run 8192 chains:
   plain    MonteCarloLoop: 200 thermalisation sweeps, one sample per sweep of N
            single-spin-flip proposals;
@@ -176,7 +176,10 @@ def run_c5(args, R, HAM, F):
     psi = R.Psi(np.zeros(N), np.zeros(N), b, W, 1.0, False, True, precision=args.precision)
     op = R.Operator(*tab, True)
     ev = R.ExpectationValue(True)
-    # 50 Adam steps on the energy gradient (b and W; the first 2N entries are alpha, beta)
+    # 50 Adam steps on the energy gradient (b and W; the first 2N entries are alpha, beta).
+    # lr 5e-3 reaches E/N = -3.229 to -3.230 from every chain seed tried; at lr 1e-2 the
+    # outcome depends on rounding-level differences of the gradient (one seed in three ends
+    # at E/N = -2.5 with 33 % acceptance)
     log("c5: pre-training the RBM (50 Adam steps)")
     t0 = time.perf_counter()
     mc = R.MonteCarloLoop(C * 10, 1, 20, C, True, seed=11)
@@ -187,7 +190,7 @@ def run_c5(args, R, HAM, F):
         g, E_pre = ev.gradient(psi, op, mc)
         m = 0.9 * m + 0.1 * g
         v = 0.999 * v + 0.001 * np.abs(g) ** 2
-        p = p - 1e-2 * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+        p = p - 5e-3 * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
         p[:2 * N] = 0
         psi.params = p
     R.synchronize()
@@ -244,7 +247,7 @@ def run_c5(args, R, HAM, F):
         "ms_per_step": l_ms * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64" if args.precision == 32 else "c128",
-        "data": "synthetic (seeded N(0, 0.01^2) RBM, 50 Adam steps; proposal network trained on the plain chain)",
+        "data": "synthetic (seeded N(0, 0.01^2) RBM, 50 Adam steps at lr 5e-3; proposal network trained on the plain chain)",
         "config": {"workload": "2-D TFIM 10x10 periodic J=1 h=3.04, RBM M=200 after 50 Adam steps; learned "
                                "(MADE 100-256-100 independent MH) vs plain single-spin-flip, 8192 chains, "
                                f"{spc} samples per chain, gradient step (sampling + E_loc + O_k + FP64 contraction)",

@@ -1016,8 +1016,9 @@ static Scalars reduce_scalars(Ctx& c, int64_t n, const double* sc) {
 }
 
 // R[r][col] = sum_s coef[s] row_r(s) conj(col(s)) for the RBM rows
-// {1, s_0..s_{N-1}} and columns {T_0..T_{M-1}, 1}: FP32 MFMA on complex64
-// records (chunk partials added in FP64), FP64 VALU on complex128 records.
+// {1, s_0..s_{N-1}} and columns {T_0..T_{M-1}, 1}: FP32 arithmetic on the
+// bfloat16 matrix cores (exact three-part split) for complex64 records, chunk
+// partials added in FP64; FP64 VALU on complex128 records.
 template <typename T>
 static std::vector<cd> ok_contract(Ctx& c, int N, int M, int64_t n, const void* tanh, const uint64_t* spins,
                                    const cx<double>* coef, bool fp64_sums = false) {
@@ -1030,7 +1031,7 @@ static std::vector<cd> ok_contract(Ctx& c, int N, int M, int64_t n, const void* 
         int64_t nch = std::min<int64_t>(2048, std::max<int64_t>(1, (256ll << 20) / (per_chunk * 4)));
         nch = std::min<int64_t>(nch, std::max<int64_t>(1, (n + 63) / 64));
         int64_t chunk = (n + nch - 1) / nch;
-        chunk = (chunk + 3) / 4 * 4;
+        chunk = (chunk + 31) / 32 * 32;   // whole 32-sample K-steps
         nch = (n + chunk - 1) / chunk;
         float* part = reinterpret_cast<float*>(c.buf("mfpart", (size_t)(nch * per_chunk) * sizeof(float)));
         double* psum = reinterpret_cast<double*>(c.buf("mfsum", (size_t)per_chunk * MFMA_SUM_SPLIT * sizeof(double)));

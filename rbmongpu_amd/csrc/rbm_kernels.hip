@@ -645,32 +645,146 @@ __global__ __launch_bounds__(256) void k_ok_reduce(int64_t n, int N, int M, cons
 }
 
 // ---------------------------------------------------------------------------
-// O_k contraction on FP32 MFMA (v_mfma_f32_16x16x4f32).  Real GEMM over the
-// samples:  P[r][n] = sum_s A_r(s) B_n(s),
+// O_k contraction.  Real GEMM over the samples:  P[r][n] = sum_s A_r(s) B_n(s),
 //   A = row_r(s), rows {1, s_0..s_{N-1}} (exact +-1 values),
 //   n = 2j + p: B = (p ? Im : Re)(coef(s) conj(col_j(s))), columns {T_0..T_{M-1}, 1}
 // so that R[r][j] = sum_s coef row_r conj(col_j) = P[r][2j] + i P[r][2j+1].
-// The complex coefficient rides on the column side, so the rows are the N + 1
-// spin rows once rather than twice.
-// A K-step takes 4 samples: lane l feeds sample l/16 and tile row/column l%16
-// (A: lane l holds A[l%16][l/16], B: B[l/16][l%16], D: D[4(l/16)+v][l%16]).
-// One wavefront owns up to 4 x 8 output tiles of one sample chunk, accumulates
-// the chunk in FP32 and writes its partials; chunks are added in FP64.  The
-// coefficients are centred (zero mean), so the FP32 chunk sums carry no large
-// cancellation.
+// From 32 spins up it runs on the bfloat16 matrix cores with FP32 arithmetic: A is exact in
+// bfloat16 and each FP32 value of B is split exactly into three bfloat16
+// parts (hi = bf16(B), mid = bf16(B - hi), lo = B - hi - mid), so every
+// product is exact and v_mfma_f32_16x16x32_bf16 sums it in FP32.
+// A K-step takes 32 samples: lane l feeds samples 8 (l/16) .. 8 (l/16) + 7 for
+// tile row / column l % 16 (A: lane l holds A[l%16][8(l/16) + q], B:
+// B[8(l/16) + q][l%16], D: D[4(l/16)+v][l%16]).  One wavefront owns all RT row
+// tiles and CT column tiles of one sample chunk, accumulates the chunk in FP32
+// and writes its partials; chunks are added in FP64.  The coefficients are
+// centred (zero mean), so the FP32 chunk sums carry no large cancellation.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+// bfloat16 MFMA operands and the exact three-part split of FP32 values
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-// RT x CT output tiles per wavefront, fixed at compile time so the K loop has
-// no branches and the next K-step's loads overlap the current MFMAs.  Rows
-// past N + 1 and columns past 2(M+1) are padding and contribute zero.
-//
-// Workgroups are numbered so that all tile groups of one sample chunk land on
-// the same XCD (workgroup b runs on XCD b mod 8) and start together: they read
-// the same my_tanh records at about the same time, so the repeats come from
-// that XCD's L2 instead of HBM.
+RBM_DI uint32_t bf16_pair(float lo, float hi) {
+    const __bf16 a = (__bf16)lo, b = (__bf16)hi;
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+// three-part split of two FP32 values into packed bfloat16 pairs
+RBM_DI void split3(float x0, float x1, uint32_t& hi, uint32_t& mid, uint32_t& lo) {
+    hi = bf16_pair(x0, x1);
+    const float r0 = x0 - __uint_as_float(hi << 16), r1 = x1 - __uint_as_float(hi & 0xFFFF0000u);
+    mid = bf16_pair(r0, r1);
+    lo = bf16_pair(r0 - __uint_as_float(mid << 16), r1 - __uint_as_float(mid & 0xFFFF0000u));
+}
+
+
+// Rows past N + 1 and columns past 2(M+1) are padding and contribute zero.
+// Workgroups are numbered so that all column groups of one sample chunk land
+// on the same XCD (workgroup b runs on XCD b mod 8) and start together: they
+// read the same my_tanh records at about the same time, so the repeats come
+// from that XCD's L2 instead of HBM.
 template <int RT, int CT>
-__global__ __launch_bounds__(64) void k_ok_mfma(int64_t n, int N, int M, const float* __restrict__ Tf,
+__global__ __launch_bounds__(64) void k_ok_mfma_bf3(int64_t n, int N, int M, const float* __restrict__ Tf,
+                                                const uint64_t* __restrict__ spins,
+                                                const cx<double>* __restrict__ coef, int64_t chunk, int nchunks,
+                                                int ry, int cz, int rows_p, int cols_p, float* __restrict__ partials) {
+    const int lane = threadIdx.x;
+    const int kk = lane >> 4, ii = lane & 15;
+    const int G = ry * cz;
+    const int slot = blockIdx.x >> 3;
+    const int chunk_id = (slot / G) * 8 + (blockIdx.x & 7);
+    const int tile = slot % G;
+    if (chunk_id >= nchunks) return;
+    const int ct0 = (tile / ry) * CT;
+    floatx4 acc[RT][CT];
+#pragma unroll
+    for (int a = 0; a < RT; a++)
+#pragma unroll
+        for (int b = 0; b < CT; b++) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // column q = 2j + part: part = ii & 1 for every tile of this lane
+    const int part = ii & 1;
+    int coff[CT];
+    float cfix[CT], cload[CT];
+#pragma unroll
+    for (int b = 0; b < CT; b++) {
+        const int j = (16 * (ct0 + b) + ii) >> 1;
+        coff[b] = j < M ? j : 0;                         // clamped: always a valid load
+        cload[b] = j < M ? 1.f : 0.f;                    // take the loaded value
+        cfix[b] = j == M ? 1.f : 0.f;                    // the constant column
+    }
+    const int64_t s_begin = (int64_t)chunk_id * chunk;
+    const int64_t s_end = s_begin + chunk < n ? s_begin + chunk : n;
+    const float2* T2 = reinterpret_cast<const float2*>(Tf);
+    for (int64_t s0 = s_begin; s0 < s_end; s0 += 32) {
+        uint32_t bh[CT][4], bm[CT][4], bl[CT][4];
+        uint64_t w0[8], w1[8];
+#pragma unroll
+        for (int q2 = 0; q2 < 4; q2++) {
+            float bv[2][CT];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int64_t s = s0 + 8 * kk + 2 * q2 + h;
+                const bool ok = s < s_end;
+                const int64_t sc = ok ? s : s_begin;
+                const cx<double> c = coef[sc];
+                w0[2 * q2 + h] = spins[2 * sc];
+                w1[2 * q2 + h] = spins[2 * sc + 1];
+                const float cre = ok ? (float)c.re : 0.f, cim = ok ? (float)c.im : 0.f;
+                // Re(c conj t) = cre tr + cim ti,  Im(c conj t) = cim tr - cre ti
+                const float p = part ? cim : cre, qq = part ? -cre : cim;
+                const float2* Tr = T2 + sc * M;
+#pragma unroll
+                for (int b = 0; b < CT; b++) {
+                    const float2 t = Tr[coff[b]];
+                    bv[h][b] = __builtin_fmaf(p, __builtin_fmaf(t.x, cload[b], cfix[b]), qq * (t.y * cload[b]));
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < CT; b++) split3(bv[0][b], bv[1][b], bh[b][q2], bm[b][q2], bl[b][q2]);
+        }
+#pragma unroll
+        for (int a = 0; a < RT; a++) {
+            const int r = 16 * a + ii;
+            uint32_t av[4];
+#pragma unroll
+            for (int q2 = 0; q2 < 4; q2++) {
+                uint32_t e[2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int q = 2 * q2 + h;
+                    e[h] = r == 0 ? 0x3F80u
+                                  : (r <= N ? (bit_of(w0[q], w1[q], (r - 1) & 127) ? 0x3F80u : 0xBF80u) : 0u);
+                }
+                av[q2] = e[0] | (e[1] << 16);
+            }
+            const bf16x8_t A = __builtin_bit_cast(bf16x8_t, make_uint4(av[0], av[1], av[2], av[3]));
+#pragma unroll
+            for (int b = 0; b < CT; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    A, __builtin_bit_cast(bf16x8_t, make_uint4(bl[b][0], bl[b][1], bl[b][2], bl[b][3])), acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    A, __builtin_bit_cast(bf16x8_t, make_uint4(bm[b][0], bm[b][1], bm[b][2], bm[b][3])), acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    A, __builtin_bit_cast(bf16x8_t, make_uint4(bh[b][0], bh[b][1], bh[b][2], bh[b][3])), acc[a][b], 0, 0, 0);
+            }
+        }
+    }
+    float* out = partials + (int64_t)chunk_id * rows_p * cols_p;
+#pragma unroll
+    for (int a = 0; a < RT; a++)
+#pragma unroll
+        for (int b = 0; b < CT; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                out[(int64_t)(16 * a + 4 * kk + v) * cols_p + 16 * (ct0 + b) + ii] = acc[a][b][v];
+}
+
+// The same contraction on FP32 MFMA (v_mfma_f32_16x16x4f32, a K-step of 4
+// samples; lane l holds A[l%16][l/16], B[l/16][l%16]) for models of at most 31
+// spins: there the row tiles are few, and the bfloat16 kernel's per-sample
+// column work (the exact split of 2(M+1) values) is not repaid by its faster
+// matrix products.  RT x CT output tiles per wavefront.
+template <int RT, int CT>
+__global__ __launch_bounds__(64) void k_ok_mfma_f32(int64_t n, int N, int M, const float* __restrict__ Tf,
                                                 const uint64_t* __restrict__ spins,
                                                 const cx<double>* __restrict__ coef, int64_t chunk, int nchunks,
                                                 int ry, int cz, int rows_p, int cols_p, float* __restrict__ partials) {
@@ -772,15 +886,26 @@ __global__ __launch_bounds__(256) void k_mfma_combine(int N, int M, int cols_p, 
     }
 }
 
+// Tiling of the O_k contraction: rt = ceil((N + 1) / 16) row tiles and
+// ct = ceil(2 (M + 1) / 16) column tiles.  FP32 kernel (rt <= 2): one or two
+// row tiles and up to 13 column tiles per wavefront.  bfloat16 kernel: every
+// row tile (N <= 128: at most 9) and two column tiles per wavefront.
+static bool ok_uses_bf3(int N) { return (N + 1 + 15) / 16 > 2; }
+
 static void mfma_tiling(int N, int M, int& RT, int& CT, int& rows_p, int& cols_p) {
     const int rt = (N + 1 + 15) / 16, ct = (2 * (M + 1) + 15) / 16;
-    RT = rt < 4 ? rt : 4;
-    static const int cts[] = {4, 8, 11, 13};
-    const int groups = (ct + 12) / 13;
-    const int need = (ct + groups - 1) / groups;
-    CT = 13;
-    for (int c : cts) if (c >= need) { CT = c; break; }
-    rows_p = 16 * RT * ((rt + RT - 1) / RT);
+    if (ok_uses_bf3(N)) {
+        RT = rt;
+        CT = 2;
+    } else {
+        RT = rt;
+        static const int cts[] = {4, 8, 11, 13};
+        const int groups = (ct + 12) / 13;
+        const int need = (ct + groups - 1) / groups;
+        CT = 13;
+        for (int c : cts) if (c >= need) { CT = c; break; }
+    }
+    rows_p = 16 * RT;
     cols_p = 16 * CT * ((ct + CT - 1) / CT);
 }
 
@@ -797,15 +922,23 @@ hipError_t launch_ok_mfma(int64_t n, int N, int M, const void* Trec, const uint6
     const int ry = rows_p / (16 * RT), cz = cols_p / (16 * CT);
     const dim3 grid((unsigned)(8 * ((nchunks + 7) / 8) * ry * cz));
     const float* Tf = reinterpret_cast<const float*>(Trec);
-#define RBM_MF_CASE(R_, C_)                                                                                        \
+#define RBM_MF_CASE(K_, R_, C_)                                                                                   \
     if (RT == R_ && CT == C_) {                                                                                    \
-        hipLaunchKernelGGL((k_ok_mfma<R_, C_>), grid, dim3(64), 0, stream, n, N, M, Tf, spins, coef, chunk, nchunks, \
-                           ry, cz, rows_p, cols_p, partials);                                                      \
+        hipLaunchKernelGGL((K_<R_, C_>), grid, dim3(64), 0, stream, n, N, M, Tf, spins, coef, chunk, nchunks, ry, cz, \
+                           rows_p, cols_p, partials);                                                              \
         return hipGetLastError();                                                                                  \
     }
-#define RBM_MF_ROW(R_) RBM_MF_CASE(R_, 4) RBM_MF_CASE(R_, 8) RBM_MF_CASE(R_, 11) RBM_MF_CASE(R_, 13)
-    RBM_MF_ROW(1) RBM_MF_ROW(2) RBM_MF_ROW(3) RBM_MF_ROW(4)
+    if (ok_uses_bf3(N)) {
+#define RBM_MF_ROW(R_) RBM_MF_CASE(k_ok_mfma_bf3, R_, 2)
+        RBM_MF_ROW(3) RBM_MF_ROW(4) RBM_MF_ROW(5) RBM_MF_ROW(6) RBM_MF_ROW(7) RBM_MF_ROW(8) RBM_MF_ROW(9)
 #undef RBM_MF_ROW
+    } else {
+#define RBM_MF_ROW(R_)                                                                                             \
+    RBM_MF_CASE(k_ok_mfma_f32, R_, 4) RBM_MF_CASE(k_ok_mfma_f32, R_, 8) RBM_MF_CASE(k_ok_mfma_f32, R_, 11)           \
+    RBM_MF_CASE(k_ok_mfma_f32, R_, 13)
+        RBM_MF_ROW(1) RBM_MF_ROW(2)
+#undef RBM_MF_ROW
+    }
 #undef RBM_MF_CASE
     return hipErrorInvalidValue;
 }
@@ -1125,20 +1258,7 @@ hipError_t launch_s_matrix_ref(int64_t n, int N, int M, int has_a, const void* T
 // A workgroup of 8 waves takes 256 configurations (two 16-row tiles per wave)
 // and stages the split W of 16 units at a time in LDS as B fragments (24 KiB).
 // ---------------------------------------------------------------------------
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 constexpr int TL_WAVES = 8, TL_TILES = 2, TL_CT = 1, TL_UNITS = 16 * TL_CT, TL_KBMAX = 4;
-
-RBM_DI uint32_t bf16_pair(float lo, float hi) {
-    const __bf16 a = (__bf16)lo, b = (__bf16)hi;
-    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
-}
-// three-part split of two FP32 values into packed bfloat16 pairs
-RBM_DI void split3(float x0, float x1, uint32_t& hi, uint32_t& mid, uint32_t& lo) {
-    hi = bf16_pair(x0, x1);
-    const float r0 = x0 - __uint_as_float(hi << 16), r1 = x1 - __uint_as_float(hi & 0xFFFF0000u);
-    mid = bf16_pair(r0, r1);
-    lo = bf16_pair(r0 - __uint_as_float(mid << 16), r1 - __uint_as_float(mid & 0xFFFF0000u));
-}
 
 __global__ __launch_bounds__(512) void k_theta_logpsi(RbmDev<float> md, int64_t n, const uint64_t* __restrict__ spins,
                                                       cx<double>* __restrict__ lp) {

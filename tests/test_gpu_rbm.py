@@ -389,3 +389,33 @@ def test_c2_full_size_properties():
     _, _, el = mc.last_samples()
     m, se = _chain_mean_se(el.real, 4096)
     assert abs(E_mc.real - res[64][1].real) < 3 * se, (E_mc, res[64][1], se)
+
+
+@pytest.mark.parametrize("N,M", [(100, 200), (36, 144), (128, 16), (50, 7), (20, 80)])
+def test_mc_gradient_contraction_shapes(N, M):
+    """The O_k contraction at the benchmark shapes (up to 9 row tiles and 26
+    column tiles): the complex64 gradient of recorded Metropolis samples against
+    the FP64 centred estimator 2/n sum_s (E_s - <E>) conj(O_k(s)) on the same
+    samples and local energies, O_k from the oracle's my_tanh.  The complex64
+    local energies of these larger models (sums of up to 100 ratios over 200
+    units) are checked against the oracle at 5e-5."""
+    print(f"[gradient shapes] N={N} M={M}", flush=True)
+    b, W, _ = random_rbm(N, M, seed=3 * N + M, scale=0.05)
+    psi = gpu_psi(b, W, precision=32)
+    tab = HAM.tfim_1d(N, 1.0, 1.0)
+    chains, per = 512, 6
+    mc = R.MonteCarloLoop(chains * per, 1, 5, chains, True, seed=N)
+    g, E = R.ExpectationValue(True).gradient(psi, gpu_op(tab), mc)
+    spins, _, el = mc.last_samples()
+    s = np.where((spins[:, :1] >> np.arange(min(N, 64), dtype=np.uint64)) & np.uint64(1), 1.0, -1.0)
+    if N > 64:
+        s = np.concatenate([s, np.where((spins[:, 1:] >> np.arange(N - 64, dtype=np.uint64)) & np.uint64(1), 1.0, -1.0)],
+                           axis=1)
+    T = O.tanh(b[None, :] + s @ W)
+    Ok = np.concatenate([T, (s[:, :, None] * T[:, None, :]).reshape(len(s), N * M)], axis=1)
+    El = O.local_energy(O.RBM(b, W), O.Operator(*tab), spins)
+    assert rel_err(el, El) < 5e-5
+    ref = (2.0 / len(s)) * ((el - el.mean())[:, None] * Ok.conj()).sum(axis=0)
+    err = rel_err(g[2 * N:], ref)
+    print(f"  gradient rel err {err:.2e}", flush=True)
+    assert err < 1e-5
