@@ -1066,15 +1066,16 @@ static std::vector<cd> deep_mfma_contract(Ctx& c, rbmhip_psi* psi, int64_t n, co
     RBM_REQUIRE(spins != nullptr, "PsiDeep O_k needs the recorded configurations");
     const DeepMfmaDev md = deep_mfma_dev(c, psi);
     const int L = md.L, N = psi->N;
-    const int SPLITS = 32, KQ = 16 * SPLITS;
-    auto pad64 = [](int x) { return (x + 63) / 64 * 64; };
+    // 256 K-splits: the per-layer products have 2 to 6 output tiles of 128 x 128
+    const int SPLITS = 256, KQ = 16 * SPLITS;
+    auto pad128 = [](int x) { return (x + 127) / 128 * 128; };   // 128 x 128 MFMA tiles
     std::vector<int> in(L + 1), out(L + 1), ldx(L + 1), ldy(L + 1);
     int mx = 0, my = 0;
     for (int l = 0; l <= L; l++) {
         in[l] = l == 0 ? N : psi->sizes[l - 1];
         out[l] = l < L ? psi->sizes[l] : 1;
-        ldx[l] = pad64(2 * in[l] + (l < L ? 1 : 0));
-        ldy[l] = pad64(2 * out[l]);
+        ldx[l] = pad128(2 * in[l] + (l < L ? 1 : 0));
+        ldy[l] = pad128(2 * out[l]);
         mx = std::max(mx, ldx[l]);
         my = std::max(my, ldy[l]);
     }
@@ -1083,7 +1084,6 @@ static std::vector<cd> deep_mfma_contract(Ctx& c, rbmhip_psi* psi, int64_t n, co
     cx<float>* rec = reinterpret_cast<cx<float>*>(c.buf("dg_records", (size_t)chunk * md.rec_stride * sizeof(cx<float>)));
     float* X = reinterpret_cast<float*>(c.buf("dg_x", (size_t)rows_max * mx * sizeof(float)));
     float* Y = reinterpret_cast<float*>(c.buf("dg_y", (size_t)rows_max * my * sizeof(float)));
-    float* G = reinterpret_cast<float*>(c.buf("dg_g", (size_t)mx * my * sizeof(float)));
     float* part = reinterpret_cast<float*>(c.buf("dg_part", (size_t)SPLITS * mx * my * sizeof(float)));
     std::vector<size_t> goff(L + 2, 0);
     for (int l = 0; l <= L; l++) goff[l + 1] = goff[l] + (size_t)ldx[l] * ldy[l];
@@ -1097,12 +1097,12 @@ static std::vector<cd> deep_mfma_contract(Ctx& c, rbmhip_psi* psi, int64_t n, co
                 return launch_deep_grad_rows(md, l, in[l], out[l], 0, cnt, rows, spins + 2 * s0, rec, coef + s0,
                                              psi->ref_compat_final, X, ldx[l], Y, ldy[l], c.stream);
             });
-            GemmEpiArgs e{};
             c.launch("deep_grad_gemm", [&] {
-                return launch_gemm_splitk(EPI_STORE, ldx[l], ldy[l], (int)rows, X, 1, ldx[l], Y, ldy[l], 1, G, ldy[l], e,
-                                          SPLITS, part, c.stream);
+                return launch_gemm_tn_splitk(ldx[l], ldy[l], rows, SPLITS, X, Y, part, c.stream);
             });
-            c.launch("deep_grad_gemm", [&] { return launch_accum_f64((int64_t)ldx[l] * ldy[l], G, Gd + goff[l], c.stream); });
+            c.launch("deep_grad_gemm", [&] {
+                return launch_accum_splits_f64((int64_t)ldx[l] * ldy[l], SPLITS, part, Gd + goff[l], c.stream);
+            });
         }
     }
     std::vector<double> h(goff[L + 1]);

@@ -131,26 +131,24 @@ __global__ __launch_bounds__(256) void k_herk_update(int n, int k0, cx<double>* 
 }
 
 // ---------------------------------------------------------------------------
-// Gram product C = A^T A, lower tiles only (the symmetric S-matrix blocks).
+// Products C = X^T Y of row-major K x ld operands on FP32 MFMA: the Gram
+// product A^T A of the S-matrix (lower tiles only) and the per-layer gradient
+// products of PsiDeep (all tiles, split over K).
 // A 256-thread workgroup computes a 128 x 128 tile as 2 x 2 waves of 64 x 64,
 // each 2 x 2 v_mfma_f32_32x32x2f32 tiles (lane l supplies A[l % 32][l / 32]
 // and B[l / 32][l % 32]; accumulator v of lane l is row
 // 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).  K advances 16 rows per LDS
 // stage; the next stage's rows are loaded into registers while the current
-// one is multiplied.  Workgroup b covers lower tile (bm, bn), b = bm (bm + 1) / 2 + bn.
+// one is multiplied.
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int GR_T = 128, GR_K = 16, GR_LD = GR_T + 4;
 
-__global__ __launch_bounds__(256) void k_gram_lower(int ld, int K, const float* __restrict__ A, float* __restrict__ C) {
+// C[m0.., n0..] = X[k_begin..k_end)^T Y[k_begin..k_end) for one 128 x 128 tile
+RBM_DI void tile_tn(const float* __restrict__ X, int ldx, int m0, const float* __restrict__ Y, int ldy, int n0,
+                    int k_begin, int k_end, float* __restrict__ C, int ldc) {
     __shared__ float As[GR_K][GR_LD];
     __shared__ float Bs[GR_K][GR_LD];
-    const int t = blockIdx.x;
-    int bm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((bm + 1) * (bm + 2) / 2 <= t) bm++;
-    while (bm * (bm + 1) / 2 > t) bm--;
-    const int bn = t - bm * (bm + 1) / 2;
-    const int m0 = bm * GR_T, n0 = bn * GR_T;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     // staging: 16 rows x 128 columns per operand = 512 float4; thread tid loads
@@ -159,24 +157,24 @@ __global__ __launch_bounds__(256) void k_gram_lower(int ld, int K, const float* 
     const int r0 = q0 >> 5, c0 = (q0 & 31) * 4, r1 = q1 >> 5, c1 = (q1 & 31) * 4;
     float4 ra0, ra1, rb0, rb1;
 #define GR_FETCH(k0)                                                                        \
-    ra0 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r0) * ld + m0 + c0);         \
-    ra1 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r1) * ld + m0 + c1);         \
-    rb0 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r0) * ld + n0 + c0);         \
-    rb1 = *reinterpret_cast<const float4*>(A + (int64_t)((k0) + r1) * ld + n0 + c1);
+    ra0 = *reinterpret_cast<const float4*>(X + (int64_t)((k0) + r0) * ldx + m0 + c0);        \
+    ra1 = *reinterpret_cast<const float4*>(X + (int64_t)((k0) + r1) * ldx + m0 + c1);        \
+    rb0 = *reinterpret_cast<const float4*>(Y + (int64_t)((k0) + r0) * ldy + n0 + c0);        \
+    rb1 = *reinterpret_cast<const float4*>(Y + (int64_t)((k0) + r1) * ldy + n0 + c1);
     f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x16{};
-    GR_FETCH(0)
-    for (int k0 = 0; k0 < K; k0 += GR_K) {
+    GR_FETCH(k_begin)
+    for (int k0 = k_begin; k0 < k_end; k0 += GR_K) {
         __syncthreads();
         *reinterpret_cast<float4*>(&As[r0][c0]) = ra0;
         *reinterpret_cast<float4*>(&As[r1][c1]) = ra1;
         *reinterpret_cast<float4*>(&Bs[r0][c0]) = rb0;
         *reinterpret_cast<float4*>(&Bs[r1][c1]) = rb1;
         __syncthreads();
-        if (k0 + GR_K < K) { GR_FETCH(k0 + GR_K) }
+        if (k0 + GR_K < k_end) { GR_FETCH(k0 + GR_K) }
 #pragma unroll
         for (int kk = 0; kk < GR_K; kk += 2) {
             float a[2], b[2];
@@ -190,6 +188,7 @@ __global__ __launch_bounds__(256) void k_gram_lower(int ld, int K, const float* 
                 for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
+#undef GR_FETCH
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -198,15 +197,62 @@ __global__ __launch_bounds__(256) void k_gram_lower(int ld, int K, const float* 
             for (int v = 0; v < 16; v++) {
                 const int row = m0 + wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int col = n0 + wn + 32 * j + (lane & 31);
-                C[(int64_t)row * ld + col] = acc[i][j][v];
+                C[(int64_t)row * ldc + col] = acc[i][j][v];
             }
-#undef GR_FETCH
+}
+
+// Gram product, lower tiles: workgroup b covers tile (bm, bn), b = bm (bm + 1) / 2 + bn.
+__global__ __launch_bounds__(256) void k_gram_lower(int ld, int K, const float* __restrict__ A, float* __restrict__ C) {
+    const int t = blockIdx.x;
+    int bm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((bm + 1) * (bm + 2) / 2 <= t) bm++;
+    while (bm * (bm + 1) / 2 > t) bm--;
+    const int bn = t - bm * (bm + 1) / 2;
+    tile_tn(A, ld, bm * GR_T, A, ld, bn * GR_T, 0, K, C, ld);
+}
+
+// Split-K product: workgroup (x, z) covers tile x of the (ldx / 128) x (ldy / 128)
+// grid over rows [z Kper, (z + 1) Kper); out[z] is ldx x ldy.
+__global__ __launch_bounds__(256) void k_gemm_tn(int ldx, int ldy, int Kper, const float* __restrict__ X,
+                                                 const float* __restrict__ Y, float* __restrict__ out) {
+    const int tn = ldy / GR_T;
+    const int bm = blockIdx.x / tn, bn = blockIdx.x % tn;
+    const int k0 = (int)blockIdx.z * Kper;
+    tile_tn(X, ldx, bm * GR_T, Y, ldy, bn * GR_T, k0, k0 + Kper, out + (int64_t)blockIdx.z * ldx * ldy, ldy);
 }
 
 hipError_t launch_gram_lower(int ld, int K, const float* A, float* C, hipStream_t stream) {
     if (ld <= 0 || ld % GR_T || K <= 0 || K % GR_K) return hipErrorInvalidValue;
     const int T = ld / GR_T;
     hipLaunchKernelGGL(k_gram_lower, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, stream, ld, K, A, C);
+    return hipGetLastError();
+}
+
+// acc[i] += sum_z part[z n + i], the split partials added in FP64 in a fixed order
+__global__ __launch_bounds__(256) void k_accum_splits_f64(int64_t n, int splits, const float* __restrict__ part,
+                                                          double* __restrict__ acc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int z = 0; z < splits; z++) s += (double)part[(int64_t)z * n + i];
+        acc[i] += s;
+    }
+}
+
+hipError_t launch_accum_splits_f64(int64_t n, int splits, const float* part, double* acc, hipStream_t stream) {
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_accum_splits_f64, dim3((unsigned)g), dim3(256), 0, stream, n, splits, part, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_tn_splitk(int ldx, int ldy, int64_t K, int splits, const float* X, const float* Y, float* out,
+                                 hipStream_t stream) {
+    if (ldx <= 0 || ldy <= 0 || ldx % GR_T || ldy % GR_T || splits < 1 || K <= 0 || K % ((int64_t)GR_K * splits) ||
+        K > ((int64_t)1 << 30))
+        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((ldx / GR_T) * (ldy / GR_T)), 1, (unsigned)splits);
+    hipLaunchKernelGGL(k_gemm_tn, grid, dim3(256), 0, stream, ldx, ldy, (int)(K / splits), X, Y, out);
     return hipGetLastError();
 }
 

@@ -17,5 +17,12 @@ hipError_t launch_cholesky_solve(int n, const cx<double>* L, cx<double>* x, hipS
 // (FP32, v_mfma_f32_32x32x2f32); ld a multiple of 128, K of 16.  Tiles above
 // the diagonal are not written.
 hipError_t launch_gram_lower(int ld, int K, const float* A, float* C, hipStream_t stream);
+// out[z] = X_z^T Y_z (ldx x ldy) for the row ranges z = 0 .. splits - 1 of
+// length K / splits of row-major K x ldx and K x ldy operands (same MFMA
+// tiles); ldx and ldy multiples of 128, K a multiple of 16 splits.
+hipError_t launch_gemm_tn_splitk(int ldx, int ldy, int64_t K, int splits, const float* X, const float* Y, float* out,
+                                 hipStream_t stream);
+// acc[i] += sum over z of part[z n + i] (FP64, fixed order)
+hipError_t launch_accum_splits_f64(int64_t n, int splits, const float* part, double* acc, hipStream_t stream);
 
 }  // namespace rbmhip
