@@ -523,49 +523,48 @@ __global__ __launch_bounds__(512) void k_deep_mfma_grad(DeepMfmaDev md, int64_t 
 }
 
 // Rows of the contraction GEMM G = X^T Y for one layer (see deep_kernels.hpp).
+// A workgroup writes whole sample rows (grid-stride over the rows), so the
+// row's coefficient, spin words and record pointer are read once per row and
+// the columns need no division.
 __global__ __launch_bounds__(256) void k_deep_grad_rows(DeepMfmaDev md, int layer, int in, int out, int64_t s0, int64_t cnt,
                                                         int64_t rows, const uint64_t* __restrict__ configs,
                                                         const cx<float>* __restrict__ records,
                                                         const cx<double>* __restrict__ coef, int lp_factor,
                                                         float* __restrict__ X, int ldx, float* __restrict__ Y, int ldy) {
-    const int64_t totx = rows * ldx, toty = rows * ldy;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < totx + toty; t += (int64_t)gridDim.x * blockDim.x) {
-        const bool isx = t < totx;
-        const int64_t tt = isx ? t : t - totx;
-        const int ld = isx ? ldx : ldy;
-        const int64_t r = tt / ld;
-        const int c = (int)(tt - r * ld);
-        float v = 0.0f;
-        if (r < cnt) {
-            const int64_t s = s0 + r;
-            const cx<float>* row = records + s * md.rec_stride;
-            if (isx) {
-                if (layer == 0) {
-                    if (c < in) v = (float)spin_of<float>(configs[2 * s], configs[2 * s + 1], c);
-                    else if (c == 2 * in) v = 1.0f;
-                } else {
-                    const cx<float>* x = row + md.rec_x[layer - 1];
-                    if (c < in) v = x[c].re;
-                    else if (c < 2 * in) v = x[c - in].im;
-                    else if (c == 2 * in && layer < md.L) v = 1.0f;
-                }
-            } else {
-                const cx<double> cf = conj(coef[s]);
-                cx<double> y = mk(0.0, 0.0);
-                int j = -1;
-                if (layer < md.L) {
-                    if (c < 2 * out) {
-                        j = c < out ? c : c - out;
-                        y = cf * to_d(row[md.rec_d[layer] + j]);
-                    }
-                } else if (c < 2) {
-                    y = lp_factor ? cf * to_d(row[md.rec_lp]) : cf;
-                    j = 0;
-                }
-                if (j >= 0) v = (float)((layer < md.L ? c < out : c == 0) ? y.re : y.im);
-            }
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        float* xr = X + r * ldx;
+        float* yr = Y + r * ldy;
+        if (r >= cnt) {   // padding rows of the last K block
+            for (int c = threadIdx.x; c < ldx; c += blockDim.x) xr[c] = 0.0f;
+            for (int c = threadIdx.x; c < ldy; c += blockDim.x) yr[c] = 0.0f;
+            continue;
         }
-        (isx ? X : Y)[tt] = v;
+        const int64_t s = s0 + r;
+        const cx<float>* row = records + s * md.rec_stride;
+        if (layer == 0) {
+            const uint64_t w0 = configs[2 * s], w1 = configs[2 * s + 1];
+            for (int c = threadIdx.x; c < ldx; c += blockDim.x)
+                xr[c] = c < in ? spin_of<float>(w0, w1, c) : (c == 2 * in ? 1.0f : 0.0f);
+        } else {
+            const cx<float>* x = row + md.rec_x[layer - 1];
+            for (int c = threadIdx.x; c < ldx; c += blockDim.x)
+                xr[c] = c < in ? x[c].re : (c < 2 * in ? x[c - in].im : (c == 2 * in && layer < md.L ? 1.0f : 0.0f));
+        }
+        const cx<double> cf = conj(coef[s]);
+        if (layer < md.L) {
+            const cx<float>* d = row + md.rec_d[layer];
+            for (int c = threadIdx.x; c < ldy; c += blockDim.x) {
+                float v = 0.0f;
+                if (c < 2 * out) {
+                    const cx<double> y = cf * to_d(d[c < out ? c : c - out]);
+                    v = (float)(c < out ? y.re : y.im);
+                }
+                yr[c] = v;
+            }
+        } else {
+            const cx<double> y = lp_factor ? cf * to_d(row[md.rec_lp]) : cf;
+            for (int c = threadIdx.x; c < ldy; c += blockDim.x) yr[c] = c == 0 ? (float)y.re : (c == 1 ? (float)y.im : 0.0f);
+        }
     }
 }
 
@@ -667,9 +666,8 @@ hipError_t launch_deep_mfma_grad(const DeepMfmaDev& md, int64_t n, const uint64_
 hipError_t launch_deep_grad_rows(const DeepMfmaDev& md, int layer, int in, int out, int64_t s0, int64_t cnt, int64_t rows,
                                  const uint64_t* configs, const cx<float>* records, const cx<double>* coef,
                                  int lp_factor, float* X, int ldx, float* Y, int ldy, hipStream_t stream) {
-    const int64_t total = rows * (ldx + ldy);
-    int64_t blocks = (total + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
+    int64_t blocks = rows < 65536 ? rows : 65536;
+    if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_deep_grad_rows, dim3((unsigned)blocks), dim3(256), 0, stream, md, layer, in, out, s0, cnt, rows,
                        configs, records, coef, lp_factor, X, ldx, Y, ldy);
     return hipGetLastError();
