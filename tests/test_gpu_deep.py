@@ -122,3 +122,12 @@ def test_deep_hs_distance_free_quantum_axis():
         assert abs(d - dr) < TOL[precision] * dr
         assert rel_err(g, gr) < TOL[precision] * 10
         assert rel_err(g[:2 * N], gr[:2 * N]) < TOL[precision] * 10
+        if precision == 64:
+            # Metropolis samples of psi: the oracle's own chains, bit for bit
+            mc = R.MonteCarloLoop(32 * 20, 1, 4, 32, True, seed=23)
+            gm, dm = hs.gradient(psi, psi2, R.Operator(*ident, True), True, mc, 1.0)
+            omc = O.MonteCarlo(N, 32 * 20, 1, 4, 32, seed=23, call_index=0)
+            grm, drm = O.hs_distance(O.Deep(N, b1, c1, w1, f1), O.Deep(N, b2, c1, w2, f2), O.Operator(*ident), True,
+                                     omc, want_grad=True, nu=1.0, axis=(alpha, beta, alpha_p, beta_p))
+            assert abs(dm - drm) < 1e-10 * drm
+            assert rel_err(gm, grm) < 1e-9
