@@ -3,7 +3,7 @@
 learned (MADE independent Metropolis-Hastings) against plain single-spin-flip
 proposals.
 
-Default workload (--config c5, the learned-proposal configuration): 2-D
+Default workload (--config tfim10x10-learned, the learned-proposal configuration): 2-D
 transverse-field Ising model on a 10x10 periodic lattice, J = 1, h = 3.04;
 RBM with M = 200 hidden units, complex64 parameters drawn i.i.d. N(0, 0.01^2)
 (seed 1238), then 50 Adam steps (lr 5e-3) on the energy gradient from 8192
@@ -24,10 +24,10 @@ the per-chain E_loc series; the plain sampler is measured the same way in the
 same run (plain.value), and the two energies are compared in units of their
 combined standard error.
 
---config c2 measures the plain GPU configuration (1-D TFIM N = 20, M = 80,
-4096 chains x (200 + 1000 sweeps)) instead; --config c3 one stochastic-
+--config tfim20-plain measures the plain GPU configuration (1-D TFIM N = 20, M = 80,
+4096 chains x (200 + 1000 sweeps)) instead; --config heisenberg6x6-sr one stochastic-
 reconfiguration step of the 6x6 Heisenberg RBM (M = 144) with its errors
-against an FP64 recomputation; --config c4 the deep-network tier (J1-J2 N = 40,
+against an FP64 recomputation; --config j1j2-deep the deep-network tier (J1-J2 N = 40,
 PsiDeep 40 -> 160 -> 80 on BF16 MFMA layers, 32768 chains) with the
 BF16 / FP32 / FP64 forward comparison on 10^5 configurations.
 
@@ -168,7 +168,7 @@ def cpu_baseline(psi_b, psi_W, tab, N, therm, budget_s):
 
 
 # ---------------------------------------------------------------------------
-def run_c5(args, R, HAM, F):
+def run_tfim10x10_learned(args, R, HAM, F):
     L, N, M, C, H = 10, 100, 200, 8192, 256
     spc = args.samples_per_chain or 500
     tab = HAM.tfim_2d(L, L, 1.0, 3.04)
@@ -180,7 +180,7 @@ def run_c5(args, R, HAM, F):
     # lr 5e-3 reaches E/N = -3.229 to -3.230 from every chain seed tried; at lr 1e-2 the
     # outcome depends on rounding-level differences of the gradient (one seed in three ends
     # at E/N = -2.5 with 33 % acceptance)
-    log("c5: pre-training the RBM (50 Adam steps)")
+    log("learned-proposal workload: pre-training the RBM (50 Adam steps)")
     t0 = time.perf_counter()
     mc = R.MonteCarloLoop(C * 10, 1, 20, C, True, seed=11)
     p = psi.params
@@ -197,7 +197,7 @@ def run_c5(args, R, HAM, F):
     pretrain_s = time.perf_counter() - t0
 
     kernels = ("mc", "eval", "coef", "scalar_reduce", "ok_reduce", "partials_sum", "made_sample", "proposal_eval", "imh")
-    log("c5: plain single-flip steps")
+    log("learned-proposal workload: plain single-flip steps")
     plain = R.MonteCarloLoop(C * spc, 1, 200, C, True, seed=21)
     p_ms, p_E, p_k = time_steps(R, ev, psi, op, plain, args.steps, args.warmup, kernels)
     _, _, el = plain.last_samples()
@@ -205,11 +205,11 @@ def run_c5(args, R, HAM, F):
     p_ess = C * spc / (2 * p_tau) / p_ms
 
     # 10^6 training samples from the plain chain, then the proposal network
-    log("c5: training data and proposal network")
+    log("learned-proposal workload: training data and proposal network")
     data_mc = R.MonteCarloLoop(C * 123, 1, 200, C, True, seed=31)
     ev(psi, op, data_mc)
     data, _, _ = data_mc.last_samples()
-    log(f"c5: {data.shape[0]} training samples")
+    log(f"learned-proposal workload: {data.shape[0]} training samples")
     if args.made_params:
         # a network trained by an earlier run of this workload (profiling runs)
         made = R.MADE(N, H, params=np.load(args.made_params))
@@ -217,7 +217,7 @@ def run_c5(args, R, HAM, F):
     else:
         made = R.MADE(N, H, seed=5)
         R.synchronize()
-        log("c5: training the proposal network")
+        log("learned-proposal workload: training the proposal network")
         t0 = time.perf_counter()
         losses = made.train(data, 2000, batch=4096, lr=1e-3, seed=6)
         R.synchronize()
@@ -225,7 +225,7 @@ def run_c5(args, R, HAM, F):
         if args.save_made_params:
             np.save(args.save_made_params, made.params)
 
-    log("c5: learned-proposal steps")
+    log("learned-proposal workload: learned-proposal steps")
     learned = R.LearnedMonteCarloLoop(made, C * spc, 20, C, True, seed=41)
     l_ms, l_E, l_k = time_steps(R, ev, psi, op, learned, args.steps, args.warmup, kernels)
     _, _, el = learned.last_samples()
@@ -264,13 +264,13 @@ def run_c5(args, R, HAM, F):
                   "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in p_k.items() if v[1]}},
         "pretrain": {"seconds": pretrain_s, "energy_per_site": E_pre.real / N},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": stored_traffic("k_eval_rbm@c5"),
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": stored_traffic("k_eval_rbm@tfim10x10-learned"),
                      "kernel": "k_eval_rbm (E_loc, log psi and my_tanh of one configuration per run)",
                      "flop_per_launch": ev_flop, "avg_launch_ms": ev_avg,
                      "algorithmic_bytes_per_launch": F.eval_bytes(M, runs)},
     }
     if not args.no_cpu_baseline:
-        log("c5: CPU baseline (FP64 oracle, one core)")
+        log("learned-proposal workload: CPU baseline (FP64 oracle, one core)")
         cps, cchains, cspc, cwall = cpu_baseline(*R_params_of(psi, N, M), tab, N, 200, args.cpu_budget)
         out["cpu_baseline"] = {
             "value": cps / (2.0 * p_tau), "unit": "effective samples/s", "cores": 1, "kind": "port",
@@ -289,7 +289,7 @@ def R_params_of(psi, N, M):
     return b, W
 
 
-def run_c2(args, R, HAM, F):
+def run_tfim20_plain(args, R, HAM, F):
     N, M, C, THERM = 20, 80, 4096, 200
     spc = args.samples_per_chain or 1000
     b, W = rbm_params(N, M, 1236)
@@ -325,7 +325,7 @@ def run_c2(args, R, HAM, F):
         "acceptance_rate": mc.acceptance_rate,
         "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": stored_traffic("k_mc_rbm@c2"),
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": stored_traffic("k_mc_rbm@tfim20-plain"),
                      "kernel": "k_mc_rbm", "flop_per_launch": flop, "avg_launch_ms": mc_avg,
                      "algorithmic_bytes_per_launch": F.mc_launch_bytes(C, spc)},
         "eval_kernel": {"kernel": "k_eval_rbm (log psi, E_loc, my_tanh of the recorded samples)",
@@ -358,7 +358,7 @@ def deep_c4_model(seed=1237):
     return N, M, C, conns, biases, weights, final
 
 
-def run_c4(args, R, HAM, F):
+def run_j1j2_deep(args, R, HAM, F):
     N, M, C, conns, biases, weights, final = deep_c4_model()
     prec = args.precision if args.precision in (16, 32) else 16
     CH, THERM = 32768, 50
@@ -411,7 +411,7 @@ def run_c4(args, R, HAM, F):
         "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
         "roofline": {"bound": "mfma" if prec == 16 else "valu", "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": stored_traffic("k_deep_mfma_mc@c4") if prec == 16 else None, "kernel": "k_deep_mfma_mc"
+                     "traffic": stored_traffic("k_deep_mfma_mc@j1j2-deep") if prec == 16 else None, "kernel": "k_deep_mfma_mc"
                      if prec == 16 else "k_deep_mc", "flop_per_launch": flop, "avg_launch_ms": mc_avg,
                      "valu_flop_per_launch": CH * (THERM + spc) * N * F.deep_forward_valu_flops(M),
                      "algorithmic_bytes_per_launch": F.mc_launch_bytes(CH, spc)},
@@ -441,7 +441,7 @@ def run_c4(args, R, HAM, F):
     return out
 
 
-def run_c3(args, R, HAM, F):
+def run_heisenberg6x6_sr(args, R, HAM, F):
     """6x6 Heisenberg, RBM M = 144: one stochastic-reconfiguration step
     (sampling, E_loc, O_k, gradient, centred S on FP32 MFMA with FP64
     accumulation, FP64 Cholesky solve of S + 1e-4 I).  Errors of S, the
@@ -526,14 +526,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=("c5", "c2", "c3", "c4"), default="c5")
+    ap.add_argument("--config", choices=("tfim10x10-learned", "tfim20-plain", "heisenberg6x6-sr", "j1j2-deep"),
+                    default="tfim10x10-learned")
     ap.add_argument("--precision", type=int, default=None, choices=(16, 32, 64),
-                    help="c2/c5: 32 (default) or 64; c4: 16 (BF16 MFMA, default) or 32")
+                    help="RBM workloads: 32 (default) or 64; j1j2-deep: 16 (BF16 MFMA, default) or 32")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--samples-per-chain", type=int, default=0)
-    ap.add_argument("--made-params", default="", help="c5: load the proposal network (.npy) instead of training it")
-    ap.add_argument("--save-made-params", default="", help="c5: save the trained proposal network (.npy)")
+    ap.add_argument("--made-params", default="", help="tfim10x10-learned: load the proposal network (.npy) instead of training it")
+    ap.add_argument("--save-made-params", default="", help="tfim10x10-learned: save the trained proposal network (.npy)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for the FETCH_SIZE/WRITE_SIZE passes)")
     args = ap.parse_args()
@@ -547,8 +548,9 @@ def main():
     from rbmongpu_amd import hamiltonians as HAM
 
     if args.precision is None:
-        args.precision = 16 if args.config == "c4" else 32
-    run = {"c5": run_c5, "c2": run_c2, "c3": run_c3, "c4": run_c4}[args.config]
+        args.precision = 16 if args.config == "j1j2-deep" else 32
+    run = {"tfim10x10-learned": run_tfim10x10_learned, "tfim20-plain": run_tfim20_plain,
+           "heisenberg6x6-sr": run_heisenberg6x6_sr, "j1j2-deep": run_j1j2_deep}[args.config]
     out = run(args, R, HAM, F)
     if args.copy_calibration:
         copy_calibration()
