@@ -432,25 +432,63 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                 }
                 const float b2j = m.b2[j];
                 float xf[C];
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    const float l = row_sum16(s0[c] + s1[c]) + b2j;
+                if constexpr (C == 2) {
+                    // The two configurations share one logit / sampling pass: lanes 0-7
+                    // of the row carry configuration 0 and lanes 8-15 configuration 1.
+                    // Half sums: a = t0[l] + t0[l + 8] (l < 8) or t1[l] + t1[l - 8], then
+                    // half-row mirror and two quad permutations add each half's 8 lanes.
+                    const bool lo = lane < 8;
+                    const float t0 = s0[0] + s1[0], t1 = s0[1] + s1[1];
+                    float a = (lo ? t0 : t1) + dpp<0x128>(lo ? t1 : t0);   // row_ror:8
+                    a += dpp<0x141>(a);                                     // row_half_mirror
+                    a += dpp<0xB1>(a);                                      // quad_perm 1,0,3,2
+                    a += dpp<0x4E>(a);                                      // quad_perm 2,3,0,1
+                    const float l = a + b2j;
                     const float e = __expf(-fabsf(l));
                     const float L1 = __logf(1.0f + e);
                     bool x;
                     if (SCORE) {
-                        x = bit_of(x0[c], x1[c], j) != 0;
+                        x = lo ? bit_of(x0[0], x1[0], j) != 0 : bit_of(x0[1], x1[1], j) != 0;
                     } else {
                         const float r = __builtin_amdgcn_rcpf(1.0f + e);
                         const float p = l >= 0.0f ? r : e * r;
-                        const float u = __builtin_fmaf((float)cur[c][q], 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+                        const uint32_t w = lo ? cur[0][q] : cur[1][q];
+                        const float u = __builtin_fmaf((float)w, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
                         x = u < p;
-                        const uint64_t bit = x ? (uint64_t)1 << (j & 63) : 0u;
-                        if (j < 64) x0[c] |= bit; else x1[c] |= bit;
                     }
                     // log sigmoid(l) = -(max(-l, 0) + log(1 + e^-|l|)), log(1 - sigmoid(l)) = -(max(l, 0) + ...)
-                    lq[c] -= (double)((x ? fmaxf(-l, 0.0f) : fmaxf(l, 0.0f)) + L1);
-                    xf[c] = x ? 1.0f : 0.0f;
+                    lq[0] -= (double)((x ? fmaxf(-l, 0.0f) : fmaxf(l, 0.0f)) + L1);
+                    // each half holds one decision: swap halves to give every lane both
+                    const float xm = x ? 1.0f : 0.0f, xo = dpp<0x128>(xm);
+                    xf[0] = lo ? xm : xo;
+                    xf[1] = lo ? xo : xm;
+                    if (!SCORE) {
+#pragma unroll
+                        for (int c = 0; c < 2; c++) {
+                            const uint64_t bit = xf[c] != 0.0f ? (uint64_t)1 << (j & 63) : 0u;
+                            if (j < 64) x0[c] |= bit; else x1[c] |= bit;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        const float l = row_sum16(s0[c] + s1[c]) + b2j;
+                        const float e = __expf(-fabsf(l));
+                        const float L1 = __logf(1.0f + e);
+                        bool x;
+                        if (SCORE) {
+                            x = bit_of(x0[c], x1[c], j) != 0;
+                        } else {
+                            const float r = __builtin_amdgcn_rcpf(1.0f + e);
+                            const float p = l >= 0.0f ? r : e * r;
+                            const float u = __builtin_fmaf((float)cur[c][q], 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+                            x = u < p;
+                            const uint64_t bit = x ? (uint64_t)1 << (j & 63) : 0u;
+                            if (j < 64) x0[c] |= bit; else x1[c] |= bit;
+                        }
+                        lq[c] -= (double)((x ? fmaxf(-l, 0.0f) : fmaxf(l, 0.0f)) + L1);
+                        xf[c] = x ? 1.0f : 0.0f;
+                    }
                 }
 #pragma unroll
                 for (int t = 0; t < V / 2; t++) {
@@ -472,15 +510,17 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                 }
             }
         }
-        if (lane == 0) {
+        // configuration c is written by lane 8 c (with two configurations per row,
+        // lanes 0-7 accumulated configuration 0's log q in lq[0], lanes 8-15
+        // configuration 1's)
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                if (g0 + c >= n) break;
+        for (int c = 0; c < C; c++) {
+            if (lane == 8 * c && g0 + c < n) {
                 if (!SCORE) {
                     out_spins[2 * (g0 + c)] = x0[c];
                     out_spins[2 * (g0 + c) + 1] = x1[c];
                 }
-                log_q[g0 + c] = lq[c];
+                log_q[g0 + c] = C == 2 ? lq[0] : lq[c];
             }
         }
     }
