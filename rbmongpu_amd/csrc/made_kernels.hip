@@ -315,7 +315,8 @@ hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, co
 //   l_j = b2_j + sum_{k < D_j} W2[j][k] relu(a_k)   (v_dot2c_f32_bf16, DPP row sum)
 //   x_j drawn (or read), log q += log sigmoid(+-l_j)
 //   a_k += x_j W1[k][j] for k >= D_j (an FMA with x_j in {0, 1}: a + W exactly
-//   when x_j = 1, a unchanged when x_j = 0), relu(a) repacked
+//   when x_j = 1, a unchanged when x_j = 0), relu(a) repacked in the groups
+//   that output j + 1 reads
 // Both weight matrices sit in LDS as bfloat16 (N x Hp each, the layout of
 // launch_made_inference_weights).
 // Uniforms: lane l of the row computes Philox blocks l and 16 + l of each of
@@ -490,10 +491,16 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                         xf[c] = x ? 1.0f : 0.0f;
                     }
                 }
+                // Only the groups that output j + 1 reads ([0, D_{j+1})) are
+                // repacked: a group above that bound is packed by a later
+                // update before any output reads it, and a group below D_j
+                // receives no more updates.
+                const int Dn = made_prefix(j + 1, N, m.H);
 #pragma unroll
                 for (int t = 0; t < V / 2; t++) {
                     if (64 * t + 64 > D) {
                         const uint2 w = w1[16 * t];
+                        const bool repack = 64 * t < Dn;
 #pragma unroll
                         for (int b = 0; b < 2; b++) {
                             const uint32_t wb = b ? w.y : w.x;
@@ -503,8 +510,14 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                             for (int c = 0; c < C; c++) {
                                 a[c][2 * v] = __builtin_fmaf(xf[c], wlo, a[c][2 * v]);
                                 a[c][2 * v + 1] = __builtin_fmaf(xf[c], whi, a[c][2 * v + 1]);
-                                hb[c][v] = pack_relu_bf16(a[c][2 * v], a[c][2 * v + 1]);
                             }
+                        }
+                        if (repack) {
+#pragma unroll
+                            for (int b = 0; b < 2; b++)
+#pragma unroll
+                                for (int c = 0; c < C; c++)
+                                    hb[c][2 * t + b] = pack_relu_bf16(a[c][4 * t + 2 * b], a[c][4 * t + 2 * b + 1]);
                         }
                     }
                 }
