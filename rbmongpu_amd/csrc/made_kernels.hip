@@ -325,10 +325,18 @@ hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, co
 RBM_DI float dot2_bf16(uint32_t w, uint32_t h, float c) {
     return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, h), c, false);
 }
+// relu as one v_max_f32.  fmaxf(x, 0) compiles to two (a v_max_f32 x, x that
+// quiets a possible signalling NaN, then the max with 0); the pre-activations
+// here are FMA results, never NaN, and both forms agree on every other value.
+RBM_DI float relu(float x) {
+    float r;
+    asm("v_max_f32 %0, 0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 RBM_DI uint32_t pack_relu_bf16(float lo, float hi) {
     bf16x2 v;
-    v.x = (__bf16)fmaxf(lo, 0.0f);
-    v.y = (__bf16)fmaxf(hi, 0.0f);
+    v.x = (__bf16)relu(lo);
+    v.y = (__bf16)relu(hi);
     return __builtin_bit_cast(uint32_t, v);
 }
 
