@@ -5,7 +5,7 @@
 //   k_made_batch_index    Philox-drawn minibatch indices
 //   k_colsum, k_adam      bias gradients and the Adam update
 //   k_made_inf_weights    bfloat16 inference copies of W1 (transposed) and W2
-//   k_made_sample<HPL,S>  autoregressive sampling / scoring, bfloat16 weights in LDS
+//   k_made_sample<HPL,S,W1F>  autoregressive sampling / scoring, weights in LDS
 //   k_imh                 independent Metropolis-Hastings over precomputed proposals
 #include "made_kernels.hpp"
 
@@ -318,7 +318,7 @@ hipError_t launch_made_inference_weights(const MadeShape& s, const float* W1, co
 //   when x_j = 1, a unchanged when x_j = 0), relu(a) repacked in the groups
 //   that output j + 1 reads
 // Both weight matrices sit in LDS as bfloat16 (N x Hp each, the layout of
-// launch_made_inference_weights).
+// launch_made_inference_weights), W1 expanded to FP32 when it fits (W1F).
 // Uniforms: lane l of the row computes Philox blocks l and 16 + l of each of
 // its configurations; position j reads word j % 4 of block j / 4 from its lane.
 // ---------------------------------------------------------------------------
@@ -346,7 +346,13 @@ int made_hidden_per_lane(int Hp) { return Hp / 16; }
 // 1024-thread workgroup allows (up to 256 hidden units), else one
 constexpr int made_cpr(int hpl) { return hpl <= 16 ? 2 : 1; }
 
-template <int HPL, bool SCORE>
+RBM_DI float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+RBM_DI float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+// W1F: W1 sits in LDS as FP32, expanded from its bfloat16 copy at the start
+// (the same values), so the hidden-unit updates need no unpacking; used when
+// N x Hp x 6 bytes fit in LDS.  Otherwise W1 stays bfloat16 like W2.
+template <int HPL, bool SCORE, bool W1F>
 __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint32_t seed_lo, uint32_t seed_hi,
                                                       uint32_t call, const uint64_t* __restrict__ in_spins,
                                                       uint64_t* __restrict__ out_spins, double* __restrict__ log_q) {
@@ -355,18 +361,27 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
     constexpr int V = HPL / 2;   // bfloat16 pairs per lane
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int N = m.N, Hp = m.Hp;
-    uint16_t* sW1t = reinterpret_cast<uint16_t*>(smem_raw);
-    uint16_t* sW2 = sW1t + (size_t)N * Hp;
+    uint16_t* sW2 = reinterpret_cast<uint16_t*>(smem_raw + (size_t)N * Hp * (W1F ? 4 : 2));
     {
         const uint4* g1 = reinterpret_cast<const uint4*>(m.W1t);
         const uint4* g2 = reinterpret_cast<const uint4*>(m.W2);
-        uint4* l1 = reinterpret_cast<uint4*>(sW1t);
         uint4* l2 = reinterpret_cast<uint4*>(sW2);
         const int nv = N * Hp / 8;
-        for (int k = threadIdx.x; k < nv; k += blockDim.x) { l1[k] = g1[k]; l2[k] = g2[k]; }
+        for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+            const uint4 w = g1[k];
+            if (W1F) {
+                float4* l1 = reinterpret_cast<float4*>(smem_raw) + 2 * k;
+                l1[0] = make_float4(bf16_lo(w.x), bf16_hi(w.x), bf16_lo(w.y), bf16_hi(w.y));
+                l1[1] = make_float4(bf16_lo(w.z), bf16_hi(w.z), bf16_lo(w.w), bf16_hi(w.w));
+            } else {
+                reinterpret_cast<uint4*>(smem_raw)[k] = w;
+            }
+            l2[k] = g2[k];
+        }
     }
     __syncthreads();
-    const uint32_t* sW1w = reinterpret_cast<const uint32_t*>(sW1t);
+    const uint32_t* sW1w = reinterpret_cast<const uint32_t*>(smem_raw);
+    const float* sW1f = reinterpret_cast<const float*>(smem_raw);
     const uint32_t* sW2w = reinterpret_cast<const uint32_t*>(sW2);
     const int lane = threadIdx.x & 15, rows = blockDim.x >> 4;
     const int row_base = (threadIdx.x & 63) & ~15;
@@ -425,6 +440,7 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                 // pairs 2t, 2t + 1 (64 sorted units) per branch and LDS read
                 const uint2* w2 = reinterpret_cast<const uint2*>(sW2w + (size_t)j * (Hp / 2)) + lane;
                 const uint2* w1 = reinterpret_cast<const uint2*>(sW1w + (size_t)j * (Hp / 2)) + lane;
+                const float4* w1f = reinterpret_cast<const float4*>(sW1f + (size_t)j * Hp) + lane;
                 float s0[C], s1[C];
 #pragma unroll
                 for (int c = 0; c < C; c++) { s0[c] = 0.0f; s1[c] = 0.0f; }
@@ -507,13 +523,20 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
 #pragma unroll
                 for (int t = 0; t < V / 2; t++) {
                     if (64 * t + 64 > D) {
-                        const uint2 w = w1[16 * t];
+                        // the 4 weights of pairs 2t, 2t + 1 of this lane
+                        float wv[4];
+                        if (W1F) {
+                            const float4 w = w1f[16 * t];
+                            wv[0] = w.x; wv[1] = w.y; wv[2] = w.z; wv[3] = w.w;
+                        } else {
+                            const uint2 w = w1[16 * t];
+                            wv[0] = bf16_lo(w.x); wv[1] = bf16_hi(w.x); wv[2] = bf16_lo(w.y); wv[3] = bf16_hi(w.y);
+                        }
                         const bool repack = 64 * t < Dn;
 #pragma unroll
                         for (int b = 0; b < 2; b++) {
-                            const uint32_t wb = b ? w.y : w.x;
                             const int v = 2 * t + b;
-                            const float wlo = __uint_as_float(wb << 16), whi = __uint_as_float(wb & 0xFFFF0000u);
+                            const float wlo = wv[2 * b], whi = wv[2 * b + 1];
 #pragma unroll
                             for (int c = 0; c < C; c++) {
                                 a[c][2 * v] = __builtin_fmaf(xf[c], wlo, a[c][2 * v]);
@@ -555,8 +578,12 @@ static hipError_t launch_made_common(const MadeInf& m, int64_t n, uint32_t seed_
                                      const uint64_t* in_spins, uint64_t* out_spins, double* log_q, bool score,
                                      hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    const size_t lds = made_sampler_lds(m);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (made_sampler_lds(m) > 160 * 1024) return hipErrorInvalidValue;
+    // sampling holds W1 as FP32 when it fits (scoring uses the bfloat16 layout;
+    // both read the same weight values)
+    const size_t lds_f = (size_t)m.N * m.Hp * 6;
+    const bool w1f = !score && lds_f <= 160 * 1024;
+    const size_t lds = w1f ? lds_f : made_sampler_lds(m);
     const int hpl = made_hidden_per_lane(m.Hp);
     const int rows = 64;
     int64_t blocks = (n + rows * made_cpr(hpl) - 1) / (rows * made_cpr(hpl));
@@ -568,15 +595,20 @@ static hipError_t launch_made_common(const MadeInf& m, int64_t n, uint32_t seed_
 #define RBM_MADE_CASE(H)                                                                                           \
     case H:                                                                                                        \
         if (score) {                                                                                               \
-            (void)hipFuncSetAttribute((const void*)&k_made_sample<H, true>,                                        \
+            (void)hipFuncSetAttribute((const void*)&k_made_sample<H, true, false>,                                 \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                        \
-            hipLaunchKernelGGL((k_made_sample<H, true>), grid, block, lds, stream, m, n, seed_lo, seed_hi, call,    \
-                               in_spins, out_spins, log_q);                                                         \
+            hipLaunchKernelGGL((k_made_sample<H, true, false>), grid, block, lds, stream, m, n, seed_lo, seed_hi,    \
+                               call, in_spins, out_spins, log_q);                                                   \
+        } else if (w1f) {                                                                                          \
+            (void)hipFuncSetAttribute((const void*)&k_made_sample<H, false, true>,                                 \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                        \
+            hipLaunchKernelGGL((k_made_sample<H, false, true>), grid, block, lds, stream, m, n, seed_lo, seed_hi,   \
+                               call, in_spins, out_spins, log_q);                                                   \
         } else {                                                                                                   \
-            (void)hipFuncSetAttribute((const void*)&k_made_sample<H, false>,                                       \
+            (void)hipFuncSetAttribute((const void*)&k_made_sample<H, false, false>,                                \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                        \
-            hipLaunchKernelGGL((k_made_sample<H, false>), grid, block, lds, stream, m, n, seed_lo, seed_hi, call,   \
-                               in_spins, out_spins, log_q);                                                         \
+            hipLaunchKernelGGL((k_made_sample<H, false, false>), grid, block, lds, stream, m, n, seed_lo, seed_hi,  \
+                               call, in_spins, out_spins, log_q);                                                   \
         }                                                                                                          \
         break;
     switch (hpl) {
