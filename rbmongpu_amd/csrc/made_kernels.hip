@@ -333,6 +333,10 @@ RBM_DI float relu(float x) {
     asm("v_max_f32 %0, 0, %1" : "=v"(r) : "v"(x));
     return r;
 }
+// log(1 + e) for e = exp(-|l|) in (0, 1]: the argument lies in (1, 2], so
+// v_log_f32 (log2, about 1 ulp there) times ln 2 needs none of the
+// denormal-range handling of logf.
+RBM_DI float log_1p_e(float e) { return __builtin_amdgcn_logf(1.0f + e) * 0.6931471805599453f; }
 RBM_DI uint32_t pack_relu_bf16(float lo, float hi) {
     bf16x2 v;
     v.x = (__bf16)relu(lo);
@@ -421,6 +425,11 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
 #pragma unroll
             for (int v = 0; v < V; v++) hb[c][v] = pack_relu_bf16(a[c][2 * v], a[c][2 * v + 1]);
         }
+        // C == 2 sampling: each lane records its own half's configuration
+        // (lanes 0-7 configuration 0, lanes 8-15 configuration 1), bits of the
+        // current 32-position word in wacc, whole words in own0 / own1
+        uint32_t wacc = 0;
+        uint64_t own0 = 0, own1 = 0;
         for (int jb = 0; jb < N; jb += 4) {
             uint32_t cur[C][4];
             if (!SCORE) {
@@ -470,7 +479,7 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                     a += dpp<0x4E>(a);                                      // quad_perm 2,3,0,1
                     const float l = a + b2j;
                     const float e = __expf(-fabsf(l));
-                    const float L1 = __logf(1.0f + e);
+                    const float L1 = log_1p_e(e);
                     bool x;
                     if (SCORE) {
                         x = lo ? bit_of(x0[0], x1[0], j) != 0 : bit_of(x0[1], x1[1], j) != 0;
@@ -487,19 +496,13 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                     const float xm = x ? 1.0f : 0.0f, xo = dpp<0x128>(xm);
                     xf[0] = lo ? xm : xo;
                     xf[1] = lo ? xo : xm;
-                    if (!SCORE) {
-#pragma unroll
-                        for (int c = 0; c < 2; c++) {
-                            const uint64_t bit = xf[c] != 0.0f ? (uint64_t)1 << (j & 63) : 0u;
-                            if (j < 64) x0[c] |= bit; else x1[c] |= bit;
-                        }
-                    }
+                    if (!SCORE) wacc |= x ? 1u << (j & 31) : 0u;
                 } else {
 #pragma unroll
                     for (int c = 0; c < C; c++) {
                         const float l = row_sum16(s0[c] + s1[c]) + b2j;
                         const float e = __expf(-fabsf(l));
-                        const float L1 = __logf(1.0f + e);
+                        const float L1 = log_1p_e(e);
                         bool x;
                         if (SCORE) {
                             x = bit_of(x0[c], x1[c], j) != 0;
@@ -553,16 +556,25 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                     }
                 }
             }
+            // positions jb .. jb + 3 share one 32-bit word (jb is a multiple of 4)
+            if (C == 2 && !SCORE && (((jb + 4) & 31) == 0 || jb + 4 >= N)) {
+                const int wi = jb >> 5;
+                if (wi == 0) own0 |= wacc;
+                else if (wi == 1) own0 |= (uint64_t)wacc << 32;
+                else if (wi == 2) own1 |= wacc;
+                else own1 |= (uint64_t)wacc << 32;
+                wacc = 0;
+            }
         }
         // configuration c is written by lane 8 c (with two configurations per row,
-        // lanes 0-7 accumulated configuration 0's log q in lq[0], lanes 8-15
-        // configuration 1's)
+        // lanes 0-7 accumulated configuration 0's log q in lq[0] and its spins in
+        // own0 / own1, lanes 8-15 configuration 1's)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             if (lane == 8 * c && g0 + c < n) {
                 if (!SCORE) {
-                    out_spins[2 * (g0 + c)] = x0[c];
-                    out_spins[2 * (g0 + c) + 1] = x1[c];
+                    out_spins[2 * (g0 + c)] = C == 2 ? own0 : x0[c];
+                    out_spins[2 * (g0 + c) + 1] = C == 2 ? own1 : x1[c];
                 }
                 log_q[g0 + c] = C == 2 ? lq[0] : lq[c];
             }
