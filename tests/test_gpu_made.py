@@ -103,8 +103,12 @@ def test_log_q_matches_oracle_inference(N, H):
     assert np.abs(got - MD.log_q(p.astype(np.float64), X, N, H)).max() < 0.05 * N ** 0.5
 
 
-@pytest.mark.parametrize("N,H", [(20, 64), (100, 256)])
+# (128, 256): W1 does not fit in LDS as FP32 (196 KB) and the sampler keeps it
+# in bfloat16; the other shapes take the FP32 copy.  N = 128 fills all four
+# 32-bit words of a configuration.
+@pytest.mark.parametrize("N,H", [(20, 64), (100, 256), (128, 256)])
 def test_sampler_log_q_is_its_own_score(N, H):
+    print(f"[made sampler score] N={N} H={H}", flush=True)
     md = R.MADE(N, H, params=rand_params(N, H, seed=5, scale=0.3))
     spins, lq = md.sample(4096, seed=9)
     assert np.array_equal(lq, md.log_q(spins))
@@ -115,7 +119,7 @@ def test_sampler_log_q_is_its_own_score(N, H):
     assert not np.array_equal(spins, spins2)
 
 
-@pytest.mark.parametrize("N,H", [(20, 64), (100, 256), (30, 320)])
+@pytest.mark.parametrize("N,H", [(20, 64), (100, 256), (30, 320), (128, 256)])
 def test_sampler_draws_depend_on_the_proposal_index_only(N, H):
     """Proposal g's uniforms come from its own Philox counter, so a sample of
     n + 1 (odd) starts with the sample of n, whatever rows share a wavefront."""
