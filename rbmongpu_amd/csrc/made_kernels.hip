@@ -411,6 +411,12 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                 } else {
                     uw[c][4] = uw[c][5] = uw[c][6] = uw[c][7] = 0u;
                 }
+                // each word becomes its uniform u = (w + 1/2) 2^-32 once, here,
+                // as FP32 bits, instead of at every position that reads it
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    uw[c][k] = __float_as_uint(__builtin_fmaf((float)uw[c][k], 2.3283064365386963e-10f,
+                                                              1.1641532182693481e-10f));
             }
         }
         float a[C][HPL];
@@ -486,8 +492,7 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                     } else {
                         const float r = __builtin_amdgcn_rcpf(1.0f + e);
                         const float p = l >= 0.0f ? r : e * r;
-                        const uint32_t w = lo ? cur[0][q] : cur[1][q];
-                        const float u = __builtin_fmaf((float)w, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+                        const float u = __uint_as_float(lo ? cur[0][q] : cur[1][q]);
                         x = u < p;
                     }
                     // log sigmoid(l) = -(max(-l, 0) + log(1 + e^-|l|)), log(1 - sigmoid(l)) = -(max(l, 0) + ...)
@@ -509,7 +514,7 @@ __global__ __launch_bounds__(1024) void k_made_sample(MadeInf m, int64_t n, uint
                         } else {
                             const float r = __builtin_amdgcn_rcpf(1.0f + e);
                             const float p = l >= 0.0f ? r : e * r;
-                            const float u = __builtin_fmaf((float)cur[c][q], 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+                            const float u = __uint_as_float(cur[c][q]);
                             x = u < p;
                             const uint64_t bit = x ? (uint64_t)1 << (j & 63) : 0u;
                             if (j < 64) x0[c] |= bit; else x1[c] |= bit;
