@@ -753,8 +753,9 @@ static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const 
     c.launch("made_gemm", [&] {
         return launch_gemm_splitk(EPI_MASK_W1, Hp, Np, Bp, dH, 1, Hp, X, Np, 1, G, Np, e, splits, scratch, c.stream);
     });
-    c.launch("made_colsum", [&] { return launch_colsum(Bp, Hp, H, dH, G + md->off_b1, c.stream); });
-    c.launch("made_colsum", [&] { return launch_colsum(Bp, Np, N, dL, G + md->off_b2, c.stream); });
+    float* part = reinterpret_cast<float*>(c.buf("made_colsum", (size_t)COLSUM_SLICES * (Hp > Np ? Hp : Np) * sizeof(float)));
+    c.launch("made_colsum", [&] { return launch_colsum(Bp, Hp, H, dH, G + md->off_b1, part, c.stream); });
+    c.launch("made_colsum", [&] { return launch_colsum(Bp, Np, N, dL, G + md->off_b2, part, c.stream); });
 }
 
 static int made_loss_tiles(const rbmhip_made* md, int Bp) { return (md->s.Np / 64) * (Bp / 64); }

@@ -3,7 +3,7 @@
 //                         for the training forward and backward passes
 //   k_made_gather         spin words -> 0/1 input rows
 //   k_made_batch_index    Philox-drawn minibatch indices
-//   k_colsum, k_adam      bias gradients and the Adam update
+//   k_colsum_part/_finish, k_adam   bias gradients and the Adam update
 //   k_made_inf_weights    bfloat16 inference copies of W1 (transposed) and W2
 //   k_made_sample<HPL,S,W1F>  autoregressive sampling / scoring, weights in LDS
 //   k_imh                 independent Metropolis-Hastings over precomputed proposals
@@ -218,25 +218,41 @@ hipError_t launch_made_batch_index(int B, int64_t n, uint32_t step, uint32_t see
 
 // Column sums: a 1024-thread workgroup owns 64 columns; 16 row groups stride
 // over the rows (coalesced 64-wide reads) and are added in a fixed order.
-__global__ __launch_bounds__(1024) void k_colsum(int rows, int cols, int n_live, const float* __restrict__ G,
-                                                 float* __restrict__ out) {
-    __shared__ float red[16][64];
+// Column sums of the rows x cols matrix G (the bias gradients: sums over the
+// batch), columns >= n_live zero.  Two deterministic stages: workgroup (x, y)
+// adds rows [y R, (y + 1) R) of columns [64 x, 64 x + 64) into part[y][c], then
+// k_colsum_finish adds the slices in order.  part holds COLSUM_SLICES x cols.
+__global__ __launch_bounds__(256) void k_colsum_part(int rows, int cols, int n_live, const float* __restrict__ G,
+                                                     float* __restrict__ part) {
+    __shared__ float red[4][64];
     const int cl = threadIdx.x & 63, grp = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + cl;
+    const int per = (rows + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int r0 = (int)blockIdx.y * per, r1 = r0 + per < rows ? r0 + per : rows;
     float s = 0.0f;
     if (c < cols && c < n_live)
-        for (int r = grp; r < rows; r += 16) s += G[(long)r * cols + c];
+        for (int r = r0 + grp; r < r1; r += 4) s += G[(long)r * cols + c];
     red[grp][cl] = s;
     __syncthreads();
-    if (grp == 0 && c < cols) {
-        float t = 0.0f;
-        for (int g = 0; g < 16; g++) t += red[g][cl];
-        out[c] = t;
-    }
+    if (grp == 0 && c < cols) part[(long)blockIdx.y * cols + c] = ((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl];
 }
 
-hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, hipStream_t stream) {
-    hipLaunchKernelGGL(k_colsum, dim3((cols + 63) / 64), dim3(1024), 0, stream, rows, cols, n_live, G, out);
+__global__ __launch_bounds__(256) void k_colsum_finish(int cols, int slices, const float* __restrict__ part,
+                                                       float* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float t = 0.0f;
+    for (int y = 0; y < slices; y++) t += part[(long)y * cols + c];
+    out[c] = t;
+}
+
+hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, float* part, hipStream_t stream) {
+    if (rows <= 0 || cols <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_colsum_part, dim3((cols + 63) / 64, COLSUM_SLICES), dim3(256), 0, stream, rows, cols, n_live, G,
+                       part);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_colsum_finish, dim3((cols + 255) / 256), dim3(256), 0, stream, cols, COLSUM_SLICES, part, out);
     return hipGetLastError();
 }
 

@@ -88,7 +88,8 @@ hipError_t launch_made_gather(int64_t rows, int n_live, int N, int Np, const uin
 hipError_t launch_made_batch_index(int B, int64_t n, uint32_t step, uint32_t seed_lo, uint32_t seed_hi, int64_t* idx,
                                    hipStream_t stream);
 // out[n] = sum_m G[m][n] over rows m < rows (ld = cols), fixed order
-hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, hipStream_t stream);
+constexpr int COLSUM_SLICES = 32;   // row slices of launch_colsum; part holds COLSUM_SLICES x cols floats
+hipError_t launch_colsum(int rows, int cols, int n_live, const float* G, float* out, float* part, hipStream_t stream);
 // p[0 .. n) = 0 (a fill kernel, so the optimiser state can be cleared inside
 // captured work)
 hipError_t launch_zero(int64_t n, float* p, hipStream_t stream);
