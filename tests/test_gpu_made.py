@@ -106,7 +106,8 @@ def test_log_q_matches_oracle_inference(N, H):
 # (128, 256): W1 does not fit in LDS as FP32 (196 KB) and the sampler keeps it
 # in bfloat16; the other shapes take the FP32 copy.  N = 128 fills all four
 # 32-bit words of a configuration.
-@pytest.mark.parametrize("N,H", [(20, 64), (100, 256), (128, 256)])
+# N = 33 and 65 end in a partial block of 4 positions just past a 32-bit word.
+@pytest.mark.parametrize("N,H", [(20, 64), (33, 64), (65, 128), (100, 256), (128, 256)])
 def test_sampler_log_q_is_its_own_score(N, H):
     print(f"[made sampler score] N={N} H={H}", flush=True)
     md = R.MADE(N, H, params=rand_params(N, H, seed=5, scale=0.3))
@@ -114,7 +115,11 @@ def test_sampler_log_q_is_its_own_score(N, H):
     assert np.array_equal(lq, md.log_q(spins))
     X = MD.bits(spins, N)
     assert np.abs(lq - MD.log_q_inference(md.params.astype(np.float64), X, N, H)).max() < 1e-4
-    assert np.all(spins[:, 0] >> np.uint64(min(N, 63)) == 0) if N < 64 else True
+    # no bit at or above N is set
+    if N < 64:
+        assert np.all(spins[:, 0] >> np.uint64(N) == 0) and np.all(spins[:, 1] == 0)
+    elif N < 128:
+        assert np.all(spins[:, 1] >> np.uint64(N - 64) == 0)
     spins2, _ = md.sample(4096, seed=10)
     assert not np.array_equal(spins, spins2)
 
