@@ -130,6 +130,17 @@ int rbmhip_ens_destroy(rbmhip_ens* ens);
 int rbmhip_log_psi(rbmhip_psi* psi, const uint64_t* spins, int64_t n, double* out);
 /* local energy of n configurations (Operator::local_energy, include/operator/Operator.hpp:90-115) */
 int rbmhip_local_energy(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, int64_t n, double* out);
+/* Allocation-free local energies: the same computation and results as
+ * rbmhip_local_energy, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The call allocates no device
+ * memory: the model, the operator tables and the per-configuration records
+ * live in a fixed 16 MiB device array per GPU that the library holds until the
+ * process ends.  Limits: RBM wavefunctions (FP32 or FP64), n <= 262144, model
+ * and operator tables <= 4 MiB; larger inputs fail with RBMHIP_ERR_ARG.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (from
+ * several host threads) race and must be serialised by the caller. */
+int rbmhip_local_energy_noalloc(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, int64_t n, double* out);
 /* psi_O_k_vector(psi, spins): source/network_functions/PsiOkVector.cu:9-44;
  * writes num_params complex values, entries below 2N are zero */
 int rbmhip_o_k_vector(rbmhip_psi* psi, const uint64_t* spins, double* out);

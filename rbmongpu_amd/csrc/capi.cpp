@@ -25,6 +25,7 @@
 #include "deep_kernels.hpp"
 #include "made_kernels.hpp"
 #include "linalg_kernels.hpp"
+#include "noalloc.hpp"
 
 using cd = std::complex<double>;
 
@@ -95,6 +96,10 @@ struct Ctx {
     hipStream_t stream = nullptr;
     size_t max_lds = 65536;
     int num_cus = 256;
+    // device address of this device's allocation-free scratch array (null
+    // with scratch_err set when the symbol could not be resolved)
+    void* scratch = nullptr;
+    std::string scratch_err;
     std::map<std::string, DevBuf> ws;
     bool timing = false;
     struct Pending {
@@ -164,6 +169,11 @@ static Ctx& ctx() {
         c->max_lds = (size_t)v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, g_device) == hipSuccess && v > 0)
         c->num_cus = v;
+    const hipError_t se = noalloc_scratch_address(&c->scratch);
+    if (se != hipSuccess) {
+        c->scratch = nullptr;
+        c->scratch_err = hipGetErrorString(se);
+    }
     Ctx& ref = *c;
     g_ctx[g_device] = std::move(c);
     return ref;
@@ -736,12 +746,12 @@ static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const 
     // weight gradients: small outputs with the batch as K, split over workgroups
     int splits = 1;
     while (splits < 64 && Bp % (16 * splits * 2) == 0 && Bp / (splits * 2) >= 64) splits *= 2;
-    float* scratch = reinterpret_cast<float*>(c.buf("made_splitk", (size_t)splits * Np * Hp * sizeof(float)));
+    float* splitk = reinterpret_cast<float*>(c.buf("made_splitk", (size_t)splits * Np * Hp * sizeof(float)));
     // dW2 = dl^T h (masked)
     e = GemmEpiArgs{};
     e.m_live = N; e.n_live = H;
     c.launch("made_gemm", [&] {
-        return launch_gemm_splitk(EPI_MASK_W2, Np, Hp, Bp, dL, 1, Np, Hh, Hp, 1, G + md->off_W2, Hp, e, splits, scratch, c.stream);
+        return launch_gemm_splitk(EPI_MASK_W2, Np, Hp, Bp, dL, 1, Np, Hh, Hp, 1, G + md->off_W2, Hp, e, splits, splitk, c.stream);
     });
     // dh = (dl W2) * [a > 0]
     e = GemmEpiArgs{};
@@ -751,7 +761,7 @@ static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const 
     e = GemmEpiArgs{};
     e.m_live = H; e.n_live = N;
     c.launch("made_gemm", [&] {
-        return launch_gemm_splitk(EPI_MASK_W1, Hp, Np, Bp, dH, 1, Hp, X, Np, 1, G, Np, e, splits, scratch, c.stream);
+        return launch_gemm_splitk(EPI_MASK_W1, Hp, Np, Bp, dH, 1, Hp, X, Np, 1, G, Np, e, splits, splitk, c.stream);
     });
     float* part = reinterpret_cast<float*>(c.buf("made_colsum", (size_t)COLSUM_SLICES * (Hp > Np ? Hp : Np) * sizeof(float)));
     c.launch("made_colsum", [&] { return launch_colsum(Bp, Hp, H, dH, G + md->off_b1, part, c.stream); });
@@ -1211,6 +1221,79 @@ static Scalars energy_scalars(Ctx& c, rbmhip_psi* psi, const Samples& s, rbmhip_
     return reduce_scalars(c, s.n, sc);
 }
 
+// Allocation-free local energies: rbmhip_local_energy with every device
+// buffer it needs (the model's padded W, b and a, the operator's canonical
+// tables, and the spins / log psi / E_loc records) placed in this device's
+// scratch array instead of library allocations.  Fixed layout: tables in
+// [0, NOALLOC_TABLE_BYTES), then three regions of NOALLOC_MAX_CONFIGS x 16
+// bytes.  Nothing here needs clearing: every byte a kernel reads is written
+// by this call first (the padded W columns are uploaded as zeros).
+namespace {
+struct ScratchTables {
+    unsigned char* base;
+    size_t used = 0;
+    template <class V>
+    void* put(Ctx& c, const V& v) {
+        const size_t bytes = v.size() * sizeof(typename V::value_type);
+        const size_t at = (used + 255) & ~(size_t)255;
+        RBM_REQUIRE(at + bytes <= NOALLOC_TABLE_BYTES,
+                    "the model and operator need more than the " + std::to_string(NOALLOC_TABLE_BYTES >> 20) +
+                        " MiB of table space of the allocation-free entry point");
+        used = at + bytes;
+        if (bytes) c.h2d(base + at, v.data(), bytes);
+        return base + at;
+    }
+};
+}  // namespace
+
+template <typename T>
+static void local_energy_noalloc_impl(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, int64_t n,
+                                      double* out) {
+    unsigned char* base = reinterpret_cast<unsigned char*>(c.scratch);
+    ScratchTables tab{base};
+    // model: W padded with zero columns to ld hidden units, b to ld, a (N)
+    const int N = psi->N, M = psi->M, ld = rbm_padded_width(M);
+    std::vector<cx<T>> Wp((size_t)N * ld, mk<T>(0, 0)), bp((size_t)ld, mk<T>(0, 0)), ap;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < M; j++) {
+            const cd w = psi->W[(size_t)i * M + j];
+            Wp[(size_t)i * ld + j] = mk<T>((T)w.real(), (T)w.imag());
+        }
+    for (int j = 0; j < M; j++) bp[j] = mk<T>((T)psi->b[j].real(), (T)psi->b[j].imag());
+    if (psi->has_a)
+        for (const cd& v : psi->a) ap.push_back(mk<T>((T)v.real(), (T)v.imag()));
+    RbmDev<T> md;
+    md.N = N;
+    md.M = M;
+    md.ld = ld;
+    md.W = reinterpret_cast<const cx<T>*>(tab.put(c, Wp));
+    md.b = reinterpret_cast<const cx<T>*>(tab.put(c, bp));
+    md.a = psi->has_a ? reinterpret_cast<const cx<T>*>(tab.put(c, ap)) : nullptr;
+    // operator tables in the canonical form of canonicalise()
+    OpDev od;
+    od.n_diag = (int)op->diag_coef.size();
+    od.diag_coef = reinterpret_cast<const cx<double>*>(tab.put(c, op->diag_coef));
+    od.diag_zy = reinterpret_cast<const uint64_t*>(tab.put(c, op->diag_zy));
+    od.n_terms = (int)op->flip_begin.size() - 1;
+    od.flip_begin = reinterpret_cast<const int*>(tab.put(c, op->flip_begin));
+    od.flips = reinterpret_cast<const int*>(tab.put(c, op->flips));
+    od.ent_begin = reinterpret_cast<const int*>(tab.put(c, op->ent_begin));
+    od.ent_coef = reinterpret_cast<const cx<double>*>(tab.put(c, op->ent_coef));
+    od.ent_zy = reinterpret_cast<const uint64_t*>(tab.put(c, op->ent_zy));
+    // records
+    const size_t region = (size_t)NOALLOC_MAX_CONFIGS * 16;
+    uint64_t* ds = reinterpret_cast<uint64_t*>(base + NOALLOC_TABLE_BYTES);
+    cx<double>* lp = reinterpret_cast<cx<double>*>(base + NOALLOC_TABLE_BYTES + region);
+    cx<double>* el = reinterpret_cast<cx<double>*>(base + NOALLOC_TABLE_BYTES + 2 * region);
+    c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));
+    const size_t lds = rbm_w_bytes<T>(psi);
+    Records rec{nullptr, lp, el, nullptr};
+    RBM_HIPCHK(launch_eval<T>(md, od, 1, n, ds, 0, rec, lds <= c.max_lds, lds, c.num_cus * 8, c.stream));
+    c.d2h(out, el, (size_t)n * sizeof(cx<double>));
+    // the host staging vectors above stay alive until the copies are done
+    RBM_HIPCHK(hipStreamSynchronize(c.stream));
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1629,6 +1712,27 @@ int rbmhip_local_energy(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, i
         });
         c.d2h(out, el, (size_t)n * sizeof(cx<double>));
         c.sync();
+    });
+}
+
+int rbmhip_local_energy_noalloc(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, int64_t n, double* out) {
+    return guard([&] {
+        check_psi(psi);
+        RBM_REQUIRE(op, "null operator");
+        RBM_REQUIRE(n >= 0 && (n == 0 || (spins && out)), "bad arguments");
+        RBM_REQUIRE(psi->kind == 0, "the allocation-free entry point evaluates RBM wavefunctions only");
+        RBM_REQUIRE(psi->precision == RBMHIP_FP32 || psi->precision == RBMHIP_FP64, "unsupported precision");
+        RBM_REQUIRE(n <= NOALLOC_MAX_CONFIGS, "n = " + std::to_string(n) + " exceeds the allocation-free limit of " +
+                                                  std::to_string(NOALLOC_MAX_CONFIGS) + " configurations per call");
+        check_op_fits(psi, op);
+        if (n == 0) return;
+        Ctx& c = ctx();
+        if (!c.scratch)
+            throw Error(RBMHIP_ERR_HIP, "the allocation-free scratch array is unavailable on this device: " + c.scratch_err);
+        with_precision(psi->precision, [&](auto tag) {
+            using T = decltype(tag);
+            local_energy_noalloc_impl<T>(c, psi, op, spins, n, out);
+        });
     });
 }
 
