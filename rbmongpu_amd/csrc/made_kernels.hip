@@ -216,12 +216,13 @@ hipError_t launch_made_batch_index(int B, int64_t n, uint32_t step, uint32_t see
     return hipGetLastError();
 }
 
-// Column sums: a 1024-thread workgroup owns 64 columns; 16 row groups stride
-// over the rows (coalesced 64-wide reads) and are added in a fixed order.
 // Column sums of the rows x cols matrix G (the bias gradients: sums over the
-// batch), columns >= n_live zero.  Two deterministic stages: workgroup (x, y)
-// adds rows [y R, (y + 1) R) of columns [64 x, 64 x + 64) into part[y][c], then
-// k_colsum_finish adds the slices in order.  part holds COLSUM_SLICES x cols.
+// batch), columns >= n_live zero.  Two deterministic stages: the 256-thread
+// workgroup (x, y) owns columns [64 x, 64 x + 64) of row slice y (rows
+// [y R, (y + 1) R), R = ceil(rows / COLSUM_SLICES)); its 4 row groups of 64
+// threads stride over the slice with coalesced 64-wide reads and are added in
+// a fixed order into part[y][c]; k_colsum_finish then adds the slices in
+// order.  part holds COLSUM_SLICES x cols.
 __global__ __launch_bounds__(256) void k_colsum_part(int rows, int cols, int n_live, const float* __restrict__ G,
                                                      float* __restrict__ part) {
     __shared__ float red[4][64];

@@ -1,5 +1,5 @@
 """Pauli-string tables for the benchmark Hamiltonians, in the reference
-Operator's array format (include/operator/Operator.hpp:309-324):
+Operator's array format (include/operator/Operator.hpp:212-227):
 coefficients[S] complex, pauli_types[S, L] (0=I, 1=X, 2=Y, 3=Z) and
 pauli_indices[S, L], each row terminated by -1.  The reference builds these
 through the unvendored QuantumExpression package; here they are plain lists."""

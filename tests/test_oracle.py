@@ -279,6 +279,64 @@ def test_corrected_identity_and_closed_form():
     assert abs(x - X) < 1e-12 * X and abs(v - (p * E).sum() / X) < 1e-12
 
 
+def _dense_local_energies(N, tab, b, W):
+    """psi vector without prefactor and E(s) = (H psi)(s) / psi(s) from the
+    dense matrix of the table."""
+    v = O.psi_vector(O.RBM(b, W))
+    H = dense_hamiltonian(N, *tab)
+    return v, (H @ v) / v
+
+
+def test_difference_dense_closed_form():
+    """ExpectationValue::difference (ExpectationValue.cu:334-446) on the exact
+    basis: with p = |prefactor psi|^2, r = |psi / psi'|^2 (log psi without
+    prefactor) and E, E' the local energies of psi and psi', the result is
+    sum p E' - (sum p r E) / (sum p r).  The same sums with r inverted, or with
+    E and E' swapped, differ by more than 1e-3, so either mistake fails."""
+    N, M = 5, 4
+    b, W, _ = random_rbm(N, M, seed=51, scale=0.3)
+    b2, W2, _ = random_rbm(N, M, seed=52, scale=0.3)
+    tabs = [HAM.tfim_1d(N, 1.0, 0.8), random_operator(N, 6, 3, seed=53)]
+    psi = O.RBM(b, W)
+    psi.prefactor = 1.0 / np.linalg.norm(O.psi_vector(psi))
+    got = O.difference(psi, O.RBM(b2, W2, prefactor=2.5), [O.Operator(*t) for t in tabs], O.Exact(N))
+    for k, tab in enumerate(tabs):
+        v, E = _dense_local_energies(N, tab, b, W)
+        v2, E2 = _dense_local_energies(N, tab, b2, W2)
+        p = np.abs(psi.prefactor * v) ** 2
+        r = np.abs(v) ** 2 / np.abs(v2) ** 2
+
+        def form(rr, e_own, e_other):
+            return (p * e_other).sum() - (p * rr * e_own).sum() / (p * rr).sum()
+
+        want = form(r, E, E2)
+        assert abs(got[k] - want) < 1e-12 * max(1.0, abs(want)), (k, got[k], want)
+        assert abs(form(1 / r, E, E2) - want) > 1e-3
+        assert abs(form(r, E2, E) - want) > 1e-3
+
+
+def test_corrected_dense_closed_form_with_complex_local_energy():
+    """ExpectationValue::corrected (ExpectationValue.cu:74-127) on the exact
+    basis with a complex psi, so E = (H psi)/psi has an imaginary part:
+    X = exp(2 Im E + Re(E^2 - E)) (both local energies from the first
+    operator), value = sum p E / sum p X, norm = sum p X.  Dropping the 2 Im E
+    term changes the norm by more than 1 %."""
+    N, M = 5, 4
+    b, W, _ = random_rbm(N, M, seed=55, scale=0.3)
+    psi = O.RBM(b, W)
+    psi.prefactor = 1.0 / np.linalg.norm(O.psi_vector(psi))
+    for tab in (HAM.tfim_1d(N, 1.0, 0.8), random_operator(N, 5, 2, seed=56)):
+        v, E = _dense_local_energies(N, tab, b, W)
+        assert np.max(np.abs(E.imag)) > 0.1
+        p = np.abs(psi.prefactor * v) ** 2
+        X = (p * np.exp(2 * E.imag + (E * E - E).real)).sum()
+        value, norm = O.corrected(psi, O.Operator(*tab), O.Exact(N))
+        assert abs(norm - X) < 1e-12 * X
+        assert abs(value - (p * E).sum() / X) < 1e-12 * max(1.0, abs(value))
+        X_no_im = (p * np.exp((E * E - E).real)).sum()
+        assert abs(X_no_im - X) > 1e-2 * X
+
+
 def test_s_matrix_outer_products():
     N, M = 3, 4
     b, W, _ = random_rbm(N, M, seed=31)
