@@ -51,7 +51,10 @@ int rbmhip_version(void);
 /* setDevice: source/RBMonGPU.cu:7-9 (bound at pyRBMonGPU/main.cpp:611) */
 int rbmhip_set_device(int device);
 int rbmhip_device_count(int* count);
-/* start_profiling / stop_profiling: source/RBMonGPU.cu:11-17 (main.cpp:612-613) */
+/* start_profiling / stop_profiling: source/RBMonGPU.cu:11-17 (main.cpp:612-613).
+ * The reference's cudaProfilerStart/Stop become roctxProfilerResume/Pause
+ * (collection control for rocprofv3 --selected-regions) around a roctx range
+ * named "rbmhip profiled region"; both synchronise the library stream first. */
 int rbmhip_start_profiling(void);
 int rbmhip_stop_profiling(void);
 /* activation_function (my_logcosh), pyRBMonGPU/main.cpp:607-609;
@@ -90,7 +93,7 @@ int rbmhip_psi_num_params(const rbmhip_psi* psi, int64_t* out);
 int rbmhip_psi_destroy(rbmhip_psi* psi);
 
 /* ---- operators ----------------------------------------------------------- */
-/* Operator(coefficients, pauli_types, pauli_indices, gpu): include/operator/Operator.hpp:309-324.
+/* Operator(coefficients, pauli_types, pauli_indices, gpu): include/operator/Operator.hpp:212-227.
  * types: S x L int32 (0 = I, 1 = X, 2 = Y, 3 = Z); idx: S x L int32, each
  * string ends at the first -1 (a string without one continues into the next
  * row, as in Operator.hpp:148-152). */
@@ -116,6 +119,25 @@ int rbmhip_mc_copy(const rbmhip_ens* other, rbmhip_ens** out);
 /* set_total_z_symmetry / set_fast_sweep: include/spin_ensembles/MonteCarloLoop.hpp:283-291 */
 int rbmhip_mc_set_total_z(rbmhip_ens* ens, int sector);
 int rbmhip_mc_set_fast_sweep(rbmhip_ens* ens, uint32_t num_tries);
+/* Persistent chains (opt-in, no counterpart in the reference, which restarts
+ * every chain from a random configuration on every call,
+ * include/spin_ensembles/MonteCarloLoop.hpp:79-105): with on = 1, each call
+ * after the first continues every chain from the configuration the previous
+ * call left it in and skips the thermalisation sweeps, as long as the chain
+ * count, N and the move type are unchanged.  RBM wavefunctions only. */
+int rbmhip_mc_set_persistent(rbmhip_ens* ens, int on);
+int rbmhip_mc_get_persistent(const rbmhip_ens* ens, int* on);
+/* complex64 chains recompute theta = b + W^T s from the configuration every
+ * `sweeps` sweeps (default 16; 0 never), bounding the rounding drift of the
+ * incremental updates theta_j += 2 s'_p W_pj (include/quantum_state/Psi.hpp:113-119,
+ * which the FP64 reference never re-synchronises).  complex128 chains follow
+ * the reference and never re-synchronise. */
+int rbmhip_mc_set_angle_resync(rbmhip_ens* ens, int sweeps);
+/* Diagnostics of the incremental angle updates (no reference counterpart):
+ * with on = 1 the RBM samplers keep each chain's final angles theta; read them
+ * as chains x M complex (out may be NULL to query the sizes). */
+int rbmhip_mc_record_angles(rbmhip_ens* ens, int on);
+int rbmhip_mc_last_angles(const rbmhip_ens* ens, int64_t* chains, int* M, double* out);
 /* num_steps (main.cpp:280, 290) */
 int rbmhip_ens_num_steps(const rbmhip_ens* ens, uint64_t* out);
 /* acceptance_rate counters of the last call (main.cpp:282-284) */
@@ -243,6 +265,12 @@ int rbmhip_made_loss_grad(rbmhip_made* made, const uint64_t* spins, int64_t n, d
 int rbmhip_made_train(rbmhip_made* made, const uint64_t* spins, int64_t n, int steps, int batch, double lr,
                       uint64_t seed, double* losses);
 int rbmhip_made_reset_optimizer(rbmhip_made* made);
+/* Host-side check, without a GPU, that every kernel launch of one training
+ * step (rbmhip_made_train / rbmhip_made_loss_grad) for this shape stays within
+ * the buffer it is given; RBMHIP_ERR_ARG names the launch otherwise.  The
+ * training calls run the same check before launching.  No reference
+ * counterpart. */
+int rbmhip_made_check_launches(int N, int H, int64_t n, int batch);
 /* log q of n configurations with the sampler's arithmetic: bfloat16 weights,
  * FP32 hidden pre-activations, bfloat16 relu(a) in the logit dot products. */
 int rbmhip_made_log_q(rbmhip_made* made, const uint64_t* spins, int64_t n, double* out);

@@ -66,7 +66,8 @@ def build(verbose: bool = False) -> None:
         cmds.append([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     _run_parallel(cmds)
     tmp = LIB + ".tmp"
-    subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], check=True)
+    subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
+                    "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"], check=True)
     os.replace(tmp, LIB)
 
     import pybind11

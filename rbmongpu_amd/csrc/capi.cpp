@@ -8,6 +8,7 @@
 #include "../../include/rbmhip.h"
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cmath>
@@ -724,6 +725,75 @@ static MadeInf made_inf(Ctx& c, rbmhip_made* md) {
 // Forward and backward pass of loss = -(1/B) sum_s log q(x_s) over B rows of X
 // (Bp = padded rows).  Leaves the gradient in md->grad and the per-tile log-q
 // partials in `partials`.
+// Launch extents of one training step (gather, forward and backward GEMMs,
+// bias column sums, Adam): for every buffer a kernel of the step touches, the
+// largest element offset its launch arguments allow, against the size the
+// step allocates for that buffer.  Pure host arithmetic, run before every
+// training call and by rbmhip_made_check_launches without a GPU.
+static int made_splits(int Bp) {
+    int splits = 1;
+    while (splits < 64 && Bp % (16 * splits * 2) == 0 && Bp / (splits * 2) >= 64) splits *= 2;
+    return splits;
+}
+
+struct MadeStepPlan {
+    int N, H, Np, Hp, B, Bp, splits, tiles;
+    int64_t n, off_b1, off_W2, off_b2, padded;
+};
+
+static MadeStepPlan made_plan(int N, int H, int64_t n, int batch) {
+    MadeStepPlan p{};
+    p.N = N; p.H = H; p.Np = made_pad(N); p.Hp = made_pad(H);
+    p.B = batch; p.Bp = made_pad(batch); p.n = n;
+    p.splits = made_splits(p.Bp);
+    p.tiles = (p.Np / 64) * (p.Bp / 64);
+    p.off_b1 = (int64_t)p.Hp * p.Np;
+    p.off_W2 = p.off_b1 + p.Hp;
+    p.off_b2 = p.off_W2 + (int64_t)p.Np * p.Hp;
+    p.padded = p.off_b2 + p.Np;
+    return p;
+}
+
+static void made_check_extents(const MadeStepPlan& p) {
+    auto need = [](int64_t touched, int64_t size, const char* what) {
+        if (touched > size)
+            throw Error(RBMHIP_ERR_ARG, std::string("launch extent exceeds its buffer: ") + what + " touches " +
+                                            std::to_string(touched) + " elements of " + std::to_string(size));
+    };
+    // k_gemm<EPI> over an M x N output, K deep, split `splits` ways: A at
+    // (m sam + k sak), B at (k sbk + n sbn), C at (z M ldc + m ldc + n)
+    auto gemm = [&](const char* what, int M, int N, int K, int64_t sam, int64_t sak, int64_t a_size, int64_t sbk,
+                    int64_t sbn, int64_t b_size, int64_t ldc, int64_t c_size, int splits) {
+        RBM_REQUIRE(M % 64 == 0 && N % 64 == 0 && K % (16 * splits) == 0, std::string("GEMM shape: ") + what);
+        need((M - 1) * sam + (K - 1) * sak + 1, a_size, what);
+        need((K - 1) * sbk + (N - 1) * sbn + 1, b_size, what);
+        need((int64_t)(splits - 1) * M * ldc + (M - 1) * ldc + N, c_size, what);
+    };
+    const int64_t X = (int64_t)p.Bp * p.Np, HB = (int64_t)p.Bp * p.Hp, W = (int64_t)p.Np * p.Hp;
+    // minibatch indices and the gather of Bp rows of Np inputs
+    need(p.B, p.B, "batch indices");
+    need(X, X, "gathered inputs");
+    need(2 * p.n, 2 * p.n, "spin words");
+    // a = X W1^T + b1 (W1: Hp x Np at 0, b1 at off_b1), h = relu(a)
+    gemm("a = X W1^T", p.Bp, p.Hp, p.Np, p.Np, 1, X, 1, p.Np, p.off_b1, p.Hp, HB, 1);
+    need(p.off_b1 + p.Hp, p.padded, "b1");
+    // l = h W2^T + b2 (W2: Np x Hp at off_W2), loss partials per tile
+    gemm("l = h W2^T", p.Bp, p.Np, p.Hp, p.Hp, 1, HB, 1, p.Hp, W, p.Np, X, 1);
+    need(p.off_W2 + W, p.padded, "W2");
+    need(p.off_b2 + p.Np, p.padded, "b2");
+    need((p.Bp / 64) * (p.Np / 64), p.tiles, "loss partials");
+    // dW2 = dl^T h split over K, reduced into G at off_W2
+    gemm("dW2 = dl^T h", p.Np, p.Hp, p.Bp, 1, p.Np, X, p.Hp, 1, HB, p.Hp, (int64_t)p.splits * W, p.splits);
+    // dh = (dl W2) relu'(a)
+    gemm("dh = dl W2", p.Bp, p.Hp, p.Np, p.Np, 1, X, p.Hp, 1, W, p.Hp, HB, 1);
+    // dW1 = dh^T X split over K, reduced into G at 0
+    gemm("dW1 = dh^T X", p.Hp, p.Np, p.Bp, 1, p.Hp, HB, p.Np, 1, X, p.Np, (int64_t)p.splits * W, p.splits);
+    // bias gradients: column sums of dh (Hp columns) and dl (Np columns)
+    need((int64_t)COLSUM_SLICES * std::max(p.Hp, p.Np), (int64_t)COLSUM_SLICES * std::max(p.Hp, p.Np), "column-sum slices");
+    need(p.off_b1 + p.Hp, p.padded, "db1");
+    need(p.off_b2 + p.Np, p.padded, "db2");
+}
+
 static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const float* X, double* partials) {
     const int N = md->s.N, H = md->s.H, Np = md->s.Np, Hp = md->s.Hp;
     float* Apre = reinterpret_cast<float*>(c.buf("made_apre", (size_t)Bp * Hp * sizeof(float)));
@@ -744,8 +814,7 @@ static void made_forward_backward(Ctx& c, rbmhip_made* md, int B, int Bp, const 
     e.partials = partials;
     c.launch("made_gemm", [&] { return launch_gemm(EPI_LOGITS, Bp, Np, Hp, Hh, Hp, 1, W2, 1, Hp, dL, Np, e, c.stream); });
     // weight gradients: small outputs with the batch as K, split over workgroups
-    int splits = 1;
-    while (splits < 64 && Bp % (16 * splits * 2) == 0 && Bp / (splits * 2) >= 64) splits *= 2;
+    const int splits = made_splits(Bp);
     float* splitk = reinterpret_cast<float*>(c.buf("made_splitk", (size_t)splits * Np * Hp * sizeof(float)));
     // dW2 = dl^T h (masked)
     e = GemmEpiArgs{};
@@ -790,6 +859,20 @@ struct rbmhip_ens {
     // sector's total S_z (set_total_z_symmetry); fast_sweep the multi-flip move
     bool fast_sweep = false;
     uint32_t fast_tries = 1;
+    // persistent chains (opt-in): each call after the first starts from the
+    // configurations the previous call ended in and skips thermalisation;
+    // state_* identify the call that left chain_state
+    bool persistent = false;
+    DevBuf chain_state;
+    int state_device = -1, state_N = -1, state_move = -1, state_sector = 0;
+    int64_t state_chains = -1;
+    // FP32 chains recompute their angles every resync_sweeps sweeps (0: never)
+    int resync_sweeps = MC_RESYNC_SWEEPS;
+    // final angles of the last call's chains (drift diagnostics)
+    bool record_angles = false;
+    DevBuf angles;
+    int angles_M = 0, angles_precision = 0;
+    int64_t angles_chains = 0;
     // records of the last Monte Carlo call
     int rec_device = -1;
     int64_t rec_n = 0;
@@ -799,7 +882,7 @@ struct rbmhip_ens {
     DevBuf prop_spins, prop_lq, prop_lp, run_mult, sample_run;
     ~rbmhip_ens() {
         for (auto& kv : configs_dev) kv.second.release();
-        spins.release(); lp.release(); el.release();
+        spins.release(); lp.release(); el.release(); chain_state.release(); angles.release();
         prop_spins.release(); prop_lq.release(); prop_lp.release(); run_mult.release(); sample_run.release();
     }
     int64_t num_steps() const { return kind == 0 ? num_configs : (int64_t)num_samples; }
@@ -913,6 +996,8 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
             mp.move = MC_FAST_SWEEP;
             mp.tries = (int)std::max<uint32_t>(1, e->fast_tries);
         }
+        if (e->kind == 1 && (e->persistent || e->record_angles) && psi->kind != 0)
+            throw Error(RBMHIP_ERR_UNSUPPORTED, "persistent chains and angle records are implemented for RBM wavefunctions");
         uint32_t* counts = reinterpret_cast<uint32_t*>(c.buf("counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
         const OpDev od = op ? op_dev(c, op) : empty_op();
         if (e->kind == 2) {
@@ -973,10 +1058,35 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
             const RbmDev<T> md = rbm_dev<T>(c, psi);
             const size_t lds = rbm_w_bytes<T>(psi);
             const bool wlds = lds + 4096 <= c.max_lds;
+            mp.resync = e->resync_sweeps * psi->N;
+            if (e->persistent) {
+                const bool same = e->state_device == c.device && e->state_chains == mp.chains &&
+                                  e->state_N == psi->N && e->state_move == mp.move &&
+                                  (mp.move != MC_EXCHANGE || e->state_sector == e->sector);
+                uint64_t* st = reinterpret_cast<uint64_t*>(e->chain_state.ensure((size_t)mp.chains * 2 * sizeof(uint64_t)));
+                if (same) {
+                    mp.start = st;
+                    mp.therm = 0;
+                }
+                mp.final_out = st;
+            }
+            if (e->record_angles) {
+                mp.angles_out = e->angles.ensure((size_t)mp.chains * psi->M * sizeof(cx<T>));
+                e->angles_M = psi->M;
+                e->angles_chains = mp.chains;
+                e->angles_precision = sizeof(T) == 4 ? RBMHIP_FP32 : RBMHIP_FP64;
+            }
             if (mp.move == MC_SINGLE_FLIP)
                 c.launch("mc", [&] { return launch_mc<T>(md, mp, s.spins, counts, wlds, lds, c.stream); });
             else
                 c.launch("mc", [&] { return launch_mc_moves<T>(md, mp, s.spins, counts, c.stream); });
+            if (e->persistent) {
+                e->state_device = c.device;
+                e->state_chains = mp.chains;
+                e->state_N = psi->N;
+                e->state_move = mp.move;
+                e->state_sector = e->sector;
+            }
             Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
             eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
         } else if (psi->precision == RBMHIP_BF16) {
@@ -1046,7 +1156,12 @@ static std::vector<cd> ok_contract(Ctx& c, int N, int M, int64_t n, const void* 
         nch = (n + chunk - 1) / chunk;
         float* part = reinterpret_cast<float*>(c.buf("mfpart", (size_t)(nch * per_chunk) * sizeof(float)));
         double* psum = reinterpret_cast<double*>(c.buf("mfsum", (size_t)per_chunk * MFMA_SUM_SPLIT * sizeof(double)));
-        c.launch("ok_reduce", [&] { return launch_ok_mfma(n, N, M, tanh, spins, coef, chunk, (int)nch, rows_p, cols_p, part, c.stream); });
+        uint32_t* planes = ok_mfma_needs_planes(N)
+                               ? reinterpret_cast<uint32_t*>(c.buf("okplanes", (size_t)((n + 31) / 32) * N * sizeof(uint32_t)))
+                               : nullptr;
+        c.launch("ok_reduce", [&] {
+            return launch_ok_mfma(n, N, M, tanh, spins, coef, chunk, (int)nch, rows_p, cols_p, part, planes, c.stream);
+        });
         c.launch("partials_sum", [&] { return launch_mfma_finish((int)nch, N, M, rows_p, cols_p, part, psum, out, c.stream); });
     } else {
         int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (256ll << 20) / (plen * 16)));
@@ -1376,8 +1491,26 @@ int rbmhip_device_count(int* count) {
     });
 }
 
-int rbmhip_start_profiling(void) { return guard([&] { (void)ctx(); }); }
-int rbmhip_stop_profiling(void) { return guard([&] { Ctx& c = ctx(); c.sync(); }); }
+// cudaProfilerStart / cudaProfilerStop (source/RBMonGPU.cu:11-17) become the
+// ROCm tools' collection switches: rocprofv3 --selected-regions collects only
+// between roctxProfilerResume and roctxProfilerPause, and the named range
+// marks the region in traces.
+int rbmhip_start_profiling(void) {
+    return guard([&] {
+        Ctx& c = ctx();
+        c.sync();
+        roctxProfilerResume(0);
+        roctxRangePush("rbmhip profiled region");
+    });
+}
+int rbmhip_stop_profiling(void) {
+    return guard([&] {
+        Ctx& c = ctx();
+        c.sync();
+        roctxRangePop();
+        roctxProfilerPause(0);
+    });
+}
 
 int rbmhip_activation(const double* z, int64_t n, int derivative, double* out) {
     return guard([&] {
@@ -1603,6 +1736,8 @@ int rbmhip_mc_copy(const rbmhip_ens* other, rbmhip_ens** out) {
         e->num_chains = other->num_chains; e->seed = other->seed;
         e->has_sector = other->has_sector; e->sector = other->sector;
         e->fast_sweep = other->fast_sweep; e->fast_tries = other->fast_tries;
+        e->persistent = other->persistent; e->record_angles = other->record_angles;
+        e->resync_sweeps = other->resync_sweeps;
         *out = e.release();
     });
 }
@@ -1622,6 +1757,62 @@ int rbmhip_mc_set_fast_sweep(rbmhip_ens* ens, uint32_t num_tries) {
         RBM_REQUIRE(ens->kind == 1, "set_fast_sweep applies to a MonteCarloLoop");
         ens->fast_sweep = true;
         ens->fast_tries = num_tries;
+    });
+}
+
+int rbmhip_mc_set_persistent(rbmhip_ens* ens, int on) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 1, "persistent chains are a MonteCarloLoop setting");
+        ens->persistent = on != 0;
+        ens->state_device = -1;   // the next call starts afresh either way
+    });
+}
+
+int rbmhip_mc_get_persistent(const rbmhip_ens* ens, int* on) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(on, "null output");
+        *on = ens->persistent ? 1 : 0;
+    });
+}
+
+int rbmhip_mc_set_angle_resync(rbmhip_ens* ens, int sweeps) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 1, "angle re-synchronisation is a MonteCarloLoop setting");
+        RBM_REQUIRE(sweeps >= 0 && sweeps <= (1 << 20), "sweeps must be in [0, 2^20]");
+        ens->resync_sweeps = sweeps;
+    });
+}
+
+int rbmhip_mc_record_angles(rbmhip_ens* ens, int on) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(ens->kind == 1, "angle records are a MonteCarloLoop setting");
+        ens->record_angles = on != 0;
+        ens->angles_chains = 0;
+    });
+}
+
+int rbmhip_mc_last_angles(const rbmhip_ens* ens, int64_t* chains, int* M, double* out) {
+    return guard([&] {
+        check_ens(ens);
+        RBM_REQUIRE(chains && M, "null output");
+        *chains = ens->angles_chains;
+        *M = ens->angles_M;
+        if (!out || ens->angles_chains == 0) return;
+        Ctx& c = ctx();
+        const size_t n = (size_t)ens->angles_chains * ens->angles_M;
+        if (ens->angles_precision == RBMHIP_FP64) {
+            c.d2h(out, ens->angles.p, n * sizeof(cx<double>));
+            c.sync();
+        } else {
+            std::vector<cx<float>> h(n);
+            c.d2h(h.data(), ens->angles.p, n * sizeof(cx<float>));
+            c.sync();
+            for (size_t k = 0; k < n; k++) { out[2 * k] = h[k].re; out[2 * k + 1] = h[k].im; }
+        }
     });
 }
 
@@ -2230,6 +2421,42 @@ static int hs_distance_impl(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* o
             Records rec{spins2, lp2, nullptr, tanh2};
             eval_model<T>(c, psi_prime, nullptr, s.n, configs, 0, rec);
         });
+        if (psi->kind == 0 && psi_prime->kind == 0 && psi->M == psi_prime->M && psi->precision == psi_prime->precision) {
+            // nearby RBMs: log psi' - log psi from the angle difference, unit by
+            // unit, instead of the difference of two sums of M log-cosh values
+            with_precision(psi->precision, [&](auto tag) {
+                using T = decltype(tag);
+                const RbmDev<T> md = rbm_dev<T>(c, psi);
+                const int Nn = psi->N, Mm = psi->M, ld = md.ld;
+                std::vector<cx<T>> hw((size_t)Nn * ld, mk<T>(0, 0)), hb((size_t)ld, mk<T>(0, 0));
+                for (int i = 0; i < Nn; i++)
+                    for (int j = 0; j < Mm; j++) {
+                        const cd d = psi_prime->W[(size_t)i * Mm + j] - psi->W[(size_t)i * Mm + j];
+                        hw[(size_t)i * ld + j] = mk<T>((T)d.real(), (T)d.imag());
+                    }
+                for (int j = 0; j < Mm; j++) {
+                    const cd d = psi_prime->b[j] - psi->b[j];
+                    hb[j] = mk<T>((T)d.real(), (T)d.imag());
+                }
+                void* dW = c.buf("hs_dW", hw.size() * sizeof(cx<T>));
+                void* db = c.buf("hs_db", hb.size() * sizeof(cx<T>));
+                RBM_HIPCHK(hipMemcpy(dW, hw.data(), hw.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+                RBM_HIPCHK(hipMemcpy(db, hb.data(), hb.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+                cx<double>* da = nullptr;
+                if (psi->has_a || psi_prime->has_a) {
+                    std::vector<cx<double>> ha((size_t)Nn);
+                    for (int i = 0; i < Nn; i++) {
+                        const cd d = (psi_prime->has_a ? psi_prime->a[i] : cd(0, 0)) - (psi->has_a ? psi->a[i] : cd(0, 0));
+                        ha[i] = mk(d.real(), d.imag());
+                    }
+                    da = reinterpret_cast<cx<double>*>(c.buf("hs_da", ha.size() * sizeof(cx<double>)));
+                    RBM_HIPCHK(hipMemcpy(da, ha.data(), ha.size() * sizeof(cx<double>), hipMemcpyHostToDevice));
+                }
+                c.launch("hs_log_ratio", [&] {
+                    return launch_rbm_log_ratio<T>(md, dW, db, da, s.n, configs, 0, s.lp, lp2, c.num_cus * 8, c.stream);
+                });
+            });
+        }
         const uint64_t* spins_for_grad = s.spins ? s.spins : spins2;
         // free quantum axis: log psi' on the N single-flip neighbours of every
         // sample (chunks of at most 2^26 configurations), ratios r_i and the
@@ -2418,6 +2645,14 @@ int rbmhip_made_set_params(rbmhip_made* made, const float* params) {
     });
 }
 
+int rbmhip_made_check_launches(int N, int H, int64_t n, int batch) {
+    return guard([&] {
+        RBM_REQUIRE(N >= 1 && N <= 128 && H >= 1 && made_pad(H) <= 512 && n > 0 && batch > 0 && batch <= (1 << 24),
+                    "bad arguments");
+        made_check_extents(made_plan(N, H, n, batch));
+    });
+}
+
 int rbmhip_made_reset_optimizer(rbmhip_made* made) {
     return guard([&] {
         RBM_REQUIRE(made, "null handle");
@@ -2437,6 +2672,7 @@ int rbmhip_made_loss_grad(rbmhip_made* made, const uint64_t* spins, int64_t n, d
         Ctx& c = ctx();
         made_check_device(c, made);
         const int B = (int)n, Bp = made_pad(B);
+        made_check_extents(made_plan(made->s.N, made->s.H, n, B));
         uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("made_in", (size_t)n * 2 * sizeof(uint64_t)));
         c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));
         float* X = reinterpret_cast<float*>(c.buf("made_x", (size_t)Bp * made->s.Np * sizeof(float)));
@@ -2468,6 +2704,7 @@ int rbmhip_made_train(rbmhip_made* made, const uint64_t* spins, int64_t n, int s
         made_check_device(c, made);
         if (steps == 0) return;
         const int B = batch, Bp = made_pad(B);
+        made_check_extents(made_plan(made->s.N, made->s.H, n, B));
         const bool full = (int64_t)batch == n;   // the whole sample set, in order, every step
         uint64_t* ds = reinterpret_cast<uint64_t*>(c.buf("made_train_in", (size_t)n * 2 * sizeof(uint64_t)));
         c.h2d(ds, spins, (size_t)n * 2 * sizeof(uint64_t));

@@ -694,6 +694,27 @@ PYBIND11_MODULE(_pyRBMonGPU, m) {
         }))
         .def("set_total_z_symmetry", [](MonteCarloLoop& e, int sector) { check(rbmhip_mc_set_total_z(e.h, sector)); })
         .def("set_fast_sweep", [](MonteCarloLoop& e, unsigned n) { check(rbmhip_mc_set_fast_sweep(e.h, n)); })
+        .def_property("persistent_chains",
+                      [](const MonteCarloLoop& e) {
+                          int on = 0;
+                          check(rbmhip_mc_get_persistent(e.h, &on));
+                          return on != 0;
+                      },
+                      [](MonteCarloLoop& e, bool on) { check(rbmhip_mc_set_persistent(e.h, on ? 1 : 0)); },
+                      "opt-in: each call after the first continues the chains from where the previous call "
+                      "left them, without thermalisation (RBM wavefunctions)")
+        .def("set_angle_resync", [](MonteCarloLoop& e, int sweeps) { check(rbmhip_mc_set_angle_resync(e.h, sweeps)); },
+             "complex64 chains recompute theta from the configuration every `sweeps` sweeps (default 16, 0: never)")
+        .def("record_angles", [](MonteCarloLoop& e, bool on) { check(rbmhip_mc_record_angles(e.h, on ? 1 : 0)); },
+             "keep each chain's final angles theta (diagnostics of the incremental updates)")
+        .def("last_angles", [](const MonteCarloLoop& e) {
+            int64_t chains = 0;
+            int M = 0;
+            check(rbmhip_mc_last_angles(e.h, &chains, &M, nullptr));
+            carr out({(py::ssize_t)chains, (py::ssize_t)M});
+            check(rbmhip_mc_last_angles(e.h, &chains, &M, reinterpret_cast<double*>(out.mutable_data())));
+            return out;
+        })
         .def_property_readonly("num_steps", &MonteCarloLoop::num_steps)
         .def_property_readonly("acceptance_rate", [](const MonteCarloLoop& e) {
             uint64_t a = 0, r = 0;

@@ -129,6 +129,34 @@ template <typename T> RBM_DI cx<T> logcosh(cx<T> z) {
     return mk(lr, z.re > T(0) ? li : -li);
 }
 
+// my_logcosh(z) - L0 without cancellation, L0 = n0/a0 - c = my_logcosh(0).
+// The direct form adds k w, n0 / a0 + O(w) and -c, which cancel down to
+// about w^2 / 2: at |z| ~ 0.1 an FP32 evaluation keeps only ~3e-8 absolute.
+// Over the common denominator the constant and linear parts cancel in the
+// coefficients instead (computed in FP64 from the reference's constants):
+//   my_logcosh(z) = L0 + w (e0 + e1 w + e2 w^2 + e3 w^3 + e4 w^4) / D_+(w),
+//   e_i = (k a0 [a0, a1, a2, a3, 1]_i - [a0 n1 + n0 a1, n0 a2, n0 a3, n0, 0]_i) / a0,
+// with e0 ~ 3e-11 and e1 ~ 4.55, so a small w keeps its relative accuracy.
+// Used where a sum of my_logcosh values is itself the result (log psi).
+constexpr double LC_L0 = LC_N0 / LC_A0 - LC_C;
+constexpr double LC_E0 = (LC_K * LC_A0 * LC_A0 - (LC_A0 * LC_N1 + LC_N0 * LC_A1)) / LC_A0;
+constexpr double LC_E1 = (LC_K * LC_A0 * LC_A1 - LC_N0 * LC_A2) / LC_A0;
+constexpr double LC_E2 = (LC_K * LC_A0 * LC_A2 - LC_N0 * LC_A3) / LC_A0;
+constexpr double LC_E3 = (LC_K * LC_A0 * LC_A3 - LC_N0) / LC_A0;
+constexpr double LC_E4 = LC_K;
+
+template <typename T> RBM_DI cx<T> logcosh_shifted(cx<T> z) {
+    const cx<T> v = mk(fab(z.re), z.im);
+    T dr, di;
+    lc_den_w(v.re, v.im, dr, di);
+    cx<T> p = T(LC_E4) * v + mk(T(LC_E3), T(0));
+    p = v * p + mk(T(LC_E2), T(0));
+    p = v * p + mk(T(LC_E1), T(0));
+    p = v * p + mk(T(LC_E0), T(0));
+    const cx<T> q = cdiv(v * p, mk(dr, di));
+    return mk(q.re, z.re > T(0) ? q.im : -q.im);
+}
+
 // Real part of my_logcosh only (the Metropolis test needs Re log psi).
 template <typename T> RBM_DI T logcosh_re(cx<T> z) {
     const T wr = fab(z.re), wi = z.im;
@@ -138,6 +166,29 @@ template <typename T> RBM_DI T logcosh_re(cx<T> z) {
     const T num = fm(nr, dr, T(-LC_N1) * (wi * di));
     const T r = rcp(fm(dr, dr, di * di));
     return fm(num, r, fm(T(LC_K), wr, T(-LC_C)));
+}
+
+// my_logcosh(z + h) - my_logcosh(z) without subtracting two evaluations.
+// When z and z + h lie on the same side of Re = 0, w = sg z, w' = w + sg h and
+// L(w') - L(w) = sg h [k - (n0 Q + n1 R) / (D(w) D(w'))] with the divided
+// differences of the rational part, written in s = w + w', p = w w':
+//   Q = (D(w') - D(w)) / (w' - w) = s (s^2 - 2p) + a3 (s^2 - p) + a2 s + a1,
+//   R = (w' D(w) - w D(w')) / (w' - w) = a0 - p (a2 + a3 s + s^2 - p),
+// so a small h keeps its relative accuracy.  Across Re = 0 (the reference's
+// branch jump) the two sides are evaluated separately.
+template <typename T> RBM_DI cx<T> logcosh_diff(cx<T> z, cx<T> h) {
+    const cx<T> z2 = z + h;
+    const bool up = z.re > T(0);
+    if (up != (z2.re > T(0))) return logcosh(z2) - logcosh(z);
+    const T sg = up ? T(1) : T(-1);
+    const cx<T> w = sg * z, w2 = sg * z2, hh = sg * h;
+    const cx<T> d1 = lc_den(w, T(1), T(LC_A1)), d2 = lc_den(w2, T(1), T(LC_A1));
+    const cx<T> sm = w + w2, pr = w * w2;
+    const cx<T> s2 = sm * sm;
+    const cx<T> q = sm * (s2 - T(2) * pr) + T(LC_A3) * (s2 - pr) + T(LC_A2) * sm + mk(T(LC_A1), T(0));
+    const cx<T> r = mk(T(LC_A0), T(0)) - pr * ((s2 - pr) + T(LC_A3) * sm + mk(T(LC_A2), T(0)));
+    const cx<T> num = T(LC_N0) * q + T(LC_N1) * r;
+    return hh * (mk(T(LC_K), T(0)) - cdiv(num, d1 * d2));
 }
 
 // my_tanh: include/quantum_state/psi_functions.hpp:97-114.

@@ -113,24 +113,29 @@ RBM_DI void measure(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const O
                     uint64_t s0, uint64_t s1, const cx<T> (&th)[JPL], cx<T> vis, int lane, int64_t sample,
                     const Records& rec, bool valid = true) {
     const int M = md.M;
-    cx<T> lsum = mk(T(0), T(0));   // hidden units of the model
-    cx<T> lall = logcosh(th[0]);   // all slots, padded ones included
-    lsum = lane < M ? lall : lsum;
+    // log psi from the cancellation-free form (logcosh_shifted) over the
+    // model's hidden units, stored at once so that nothing of it stays live
+    // through the E_loc loop
+    {
+        cx<T> lsum = lane < M ? logcosh_shifted(th[0]) : mk(T(0), T(0));
 #pragma unroll
-    for (int u = 1; u < JPL; u++) {
-        const cx<T> lc = logcosh(th[u]);
-        lall = lall + lc;
-        lsum = lsum + (lane + 16 * u < M ? lc : mk(T(0), T(0)));
+        for (int u = 1; u < JPL; u++)
+            if (lane + 16 * u < M) lsum = lsum + logcosh_shifted(th[u]);
+        const cx<double> lp = row_sum16(to_d(lsum)) + mk((double)M * LC_L0, 0.0) + to_d(vis);
+        if (valid && lane == 0) {
+            if (rec.spins) { rec.spins[2 * sample] = s0; rec.spins[2 * sample + 1] = s1; }
+            if (rec.log_psi) rec.log_psi[sample] = lp;
+        }
     }
-    const cx<double> lp = row_sum16(to_d(lsum)) + to_d(vis);
+    // E_loc's differences use the direct form summed over all slots (padded
+    // ones included: they cancel in every difference)
+    cx<T> lall = logcosh(th[0]);
+#pragma unroll
+    for (int u = 1; u < JPL; u++) lall = lall + logcosh(th[u]);
     cx<double> E = mk(0.0, 0.0);
     if (has_op) E = local_energy_rbm<T, JPL>(md, Wsrc, op, s0, s1, th, lall, lane);
     if (!valid) return;
-    if (lane == 0) {
-        if (rec.spins) { rec.spins[2 * sample] = s0; rec.spins[2 * sample + 1] = s1; }
-        if (rec.log_psi) rec.log_psi[sample] = lp;
-        if (rec.e_loc) rec.e_loc[sample] = E;
-    }
+    if (lane == 0 && rec.e_loc) rec.e_loc[sample] = E;
     if (rec.tanh) {
         cx<T>* Tr = reinterpret_cast<cx<T>*>(rec.tanh) + sample * (int64_t)M;
 #pragma unroll
@@ -172,6 +177,23 @@ RBM_DI uint32_t pack_u(uint32_t y, double) { return y; }
 RBM_DI float unpack_u(uint32_t v, float) { return __uint_as_float(v); }
 RBM_DI double unpack_u(uint32_t v, double) { return ((double)v + 0.5) * (1.0 / 4294967296.0); }
 
+// End of a chain: its last configuration (persistent chains) and its angles
+// (drift diagnostics), when asked for.
+template <typename T, int JPL>
+RBM_DI void mc_chain_outputs(const RbmDev<T>& md, const McParams& mp, int chain, int lane, uint64_t s0, uint64_t s1,
+                             const cx<T> (&th)[JPL]) {
+    if (mp.final_out && lane == 0) {
+        mp.final_out[2 * chain] = s0;
+        mp.final_out[2 * chain + 1] = s1;
+    }
+    if (mp.angles_out) {
+        cx<T>* out = reinterpret_cast<cx<T>*>(mp.angles_out) + (int64_t)chain * md.M;
+#pragma unroll
+        for (int u = 0; u < JPL; u++)
+            if (lane + 16 * u < md.M) out[lane + 16 * u] = th[u];
+    }
+}
+
 // NARROW: N <= 32, the configuration is one 32-bit word.
 template <typename T, int JPL, bool WLDS, bool NARROW>
 __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint64_t* __restrict__ spins_out,
@@ -192,9 +214,15 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
     const int chain = valid ? chain_raw : mp.chains - 1;
     __syncthreads();
 
-    const u32x4 r0 = philox((uint32_t)chain, 0u, mp.call, 1u, mp.seed_lo, mp.seed_hi);
-    uint64_t s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
-    uint64_t s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
+    uint64_t s0, s1;
+    if (mp.start) {
+        s0 = mp.start[2 * chain];
+        s1 = mp.start[2 * chain + 1];
+    } else {
+        const u32x4 r0 = philox((uint32_t)chain, 0u, mp.call, 1u, mp.seed_lo, mp.seed_hi);
+        s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
+        s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
+    }
     mask_words(s0, s1, N);
     uint32_t w32 = (uint32_t)s0;
 
@@ -212,6 +240,9 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
     const uint64_t n_sweep = (uint64_t)mp.sweeps * N;
     const uint64_t total = n_therm + (uint64_t)mp.steps_per_chain * n_sweep;
     uint64_t next_measure = n_therm + n_sweep;
+    // FP32 angle re-synchronisation point (checked at draw-block ends)
+    const bool resync = sizeof(T) == 4 && mp.resync > 0;
+    uint64_t next_resync = resync ? (uint64_t)mp.resync : ~(uint64_t)0;
     int sample = 0;
     uint32_t nacc = 0;
 
@@ -303,11 +334,21 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
             if (n_sweep == 0 && kb + (uint64_t)i == n_therm)
                 while (sample < mp.steps_per_chain) measure_now();
         }
+        if (resync && kb + (uint64_t)nk >= next_resync) {
+            // recompute the angles from the configuration: the incremental
+            // FP32 updates since the last re-sync leave their rounding behind
+            init_angles<T, JPL>(md, Wsrc, s0, s1, lane, th);
+            lane_lc = logcosh_re(th[0]);
+#pragma unroll
+            for (int u = 1; u < JPL; u++) lane_lc += logcosh_re(th[u]);
+            next_resync += (uint64_t)mp.resync;
+        }
     }
     if (lane == 0 && counts && valid) {
         counts[2 * chain] = nacc;
         counts[2 * chain + 1] = (uint32_t)(total - nacc);
     }
+    if (valid) mc_chain_outputs<T, JPL>(md, mp, chain, lane, NARROW ? (uint64_t)w32 : s0, NARROW ? 0u : s1, th);
 }
 
 // ---------------------------------------------------------------------------
@@ -677,17 +718,68 @@ RBM_DI void split3(float x0, float x1, uint32_t& hi, uint32_t& mid, uint32_t& lo
 }
 
 
-// Rows past N + 1 and columns past 2(M+1) are padding and contribute zero.
-// Workgroups are numbered so that all column groups of one sample chunk land
-// on the same XCD (workgroup b runs on XCD b mod 8) and start together: they
-// read the same my_tanh records at about the same time, so the repeats come
-// from that XCD's L2 instead of HBM.
+// Spin bit-planes of 32-sample blocks: planes[blk * N + i] has bit k set iff
+// spin i of sample 32 blk + k is up (samples past n read as down).  One
+// wavefront covers two blocks: a ballot per spin index collects the 64
+// samples' bits, and lane k keeps the ballot of index base + k, so the stores
+// are coalesced.
+__global__ __launch_bounds__(256) void k_spin_planes(int64_t n, int N, int64_t nblk,
+                                                     const uint64_t* __restrict__ spins,
+                                                     uint32_t* __restrict__ planes) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t s = wave * 64 + lane;
+    uint64_t w0 = 0, w1 = 0;
+    if (s < n) { w0 = spins[2 * s]; w1 = spins[2 * s + 1]; }
+    const int64_t blk = 2 * wave;
+    for (int base = 0; base < N; base += 64) {
+        uint64_t mine = 0;
+        const int cnt = N - base < 64 ? N - base : 64;
+        for (int k = 0; k < cnt; k++) {
+            const int i = base + k;
+            const uint64_t m = __ballot((int)((i < 64 ? w0 >> i : w1 >> (i - 64)) & 1u));
+            mine = lane == k ? m : mine;
+        }
+        if (lane < cnt) {
+            if (blk < nblk) planes[blk * N + base + lane] = (uint32_t)mine;
+            if (blk + 1 < nblk) planes[(blk + 1) * N + base + lane] = (uint32_t)(mine >> 32);
+        }
+    }
+}
+
+// The O_k contraction on bfloat16 MFMA (v_mfma_f32_16x16x32_bf16, 32-sample
+// K-steps) for models of at least 32 spins.  Operands of a K-step:
+//   A (rows {1, s_0..s_{N-1}}, padding rows zero): exact +-1 bfloat16.  Lane
+//     (ii, kk) of row tile a supplies row r = 16 a + ii for samples 8 kk..8 kk+7,
+//     which is byte kk of the block's bit-plane of spin r - 1; a 256-entry
+//     table in LDS turns that byte into the four packed bfloat16 pairs;
+//   B (columns {Re, Im} x {coef conj T_j, coef}): every FP32 value split
+//     exactly into three bfloat16 parts, so the three products are exact and
+//     the MFMA sums them in FP32.
+// RT row tiles (all rows) and CT column tiles per wavefront.  Workgroups are
+// numbered so that all column groups of one sample chunk land on the same XCD
+// (workgroup b runs on XCD b mod 8) and start together: they read the same
+// my_tanh records at about the same time, so the repeats come from that XCD's
+// L2 instead of HBM.  Chunks are whole 32-sample blocks.
 template <int RT, int CT>
 __global__ __launch_bounds__(64) void k_ok_mfma_bf3(int64_t n, int N, int M, const float* __restrict__ Tf,
-                                                const uint64_t* __restrict__ spins,
+                                                const uint32_t* __restrict__ planes,
                                                 const cx<double>* __restrict__ coef, int64_t chunk, int nchunks,
                                                 int ry, int cz, int rows_p, int cols_p, float* __restrict__ partials) {
+    __shared__ uint4 lut[256];
     const int lane = threadIdx.x;
+    // byte -> four bfloat16 pairs: sample 2q (low half) and 2q + 1 (high half)
+    // of the byte are +1 (0x3F80) where their bit is set and -1 (0xBF80) else
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint32_t byte = (uint32_t)(lane + 64 * e);
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            w[q] = 0xBF80BF80u ^ (((byte >> (2 * q)) & 1u) << 15) ^ (((byte >> (2 * q + 1)) & 1u) << 31);
+        lut[byte] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    __syncthreads();
     const int kk = lane >> 4, ii = lane & 15;
     const int G = ry * cz;
     const int slot = blockIdx.x >> 3;
@@ -711,12 +803,20 @@ __global__ __launch_bounds__(64) void k_ok_mfma_bf3(int64_t n, int N, int M, con
         cload[b] = j < M ? 1.f : 0.f;                    // take the loaded value
         cfix[b] = j == M ? 1.f : 0.f;                    // the constant column
     }
+    // row kind of this lane in each row tile: 0 constant, 1 spin, 2 padding
+    int rkind[RT], rspin[RT];
+#pragma unroll
+    for (int a = 0; a < RT; a++) {
+        const int r = 16 * a + ii;
+        rkind[a] = r == 0 ? 0 : (r <= N ? 1 : 2);
+        rspin[a] = r >= 1 && r <= N ? r - 1 : 0;
+    }
     const int64_t s_begin = (int64_t)chunk_id * chunk;
     const int64_t s_end = s_begin + chunk < n ? s_begin + chunk : n;
     const float2* T2 = reinterpret_cast<const float2*>(Tf);
     for (int64_t s0 = s_begin; s0 < s_end; s0 += 32) {
+        const uint32_t* pl = planes + (s0 >> 5) * N;
         uint32_t bh[CT][4], bm[CT][4], bl[CT][4];
-        uint64_t w0[8], w1[8];
 #pragma unroll
         for (int q2 = 0; q2 < 4; q2++) {
             float bv[2][CT];
@@ -726,8 +826,6 @@ __global__ __launch_bounds__(64) void k_ok_mfma_bf3(int64_t n, int N, int M, con
                 const bool ok = s < s_end;
                 const int64_t sc = ok ? s : s_begin;
                 const cx<double> c = coef[sc];
-                w0[2 * q2 + h] = spins[2 * sc];
-                w1[2 * q2 + h] = spins[2 * sc + 1];
                 const float cre = ok ? (float)c.re : 0.f, cim = ok ? (float)c.im : 0.f;
                 // Re(c conj t) = cre tr + cim ti,  Im(c conj t) = cim tr - cre ti
                 const float p = part ? cim : cre, qq = part ? -cre : cim;
@@ -743,20 +841,10 @@ __global__ __launch_bounds__(64) void k_ok_mfma_bf3(int64_t n, int N, int M, con
         }
 #pragma unroll
         for (int a = 0; a < RT; a++) {
-            const int r = 16 * a + ii;
-            uint32_t av[4];
-#pragma unroll
-            for (int q2 = 0; q2 < 4; q2++) {
-                uint32_t e[2];
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int q = 2 * q2 + h;
-                    e[h] = r == 0 ? 0x3F80u
-                                  : (r <= N ? (bit_of(w0[q], w1[q], (r - 1) & 127) ? 0x3F80u : 0xBF80u) : 0u);
-                }
-                av[q2] = e[0] | (e[1] << 16);
-            }
-            const bf16x8_t A = __builtin_bit_cast(bf16x8_t, make_uint4(av[0], av[1], av[2], av[3]));
+            const uint32_t byte = rkind[a] == 0 ? 0xFFu : ((pl[rspin[a]] >> (8 * kk)) & 0xFFu);
+            uint4 av = lut[byte];
+            if (rkind[a] == 2) av = make_uint4(0u, 0u, 0u, 0u);
+            const bf16x8_t A = __builtin_bit_cast(bf16x8_t, av);
 #pragma unroll
             for (int b = 0; b < CT; b++) {
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -895,8 +983,17 @@ static bool ok_uses_bf3(int N) { return (N + 1 + 15) / 16 > 2; }
 static void mfma_tiling(int N, int M, int& RT, int& CT, int& rows_p, int& cols_p) {
     const int rt = (N + 1 + 15) / 16, ct = (2 * (M + 1) + 15) / 16;
     if (ok_uses_bf3(N)) {
+        // column tiles per wavefront: the per-group work (row operands, the
+        // sample loop) against the padding tiles, per 32 samples and in VALU
+        // issue slots, with one MFMA counted as two
         RT = rt;
         CT = 2;
+        int best = -1;
+        for (int c = 2; c <= 4; c++) {
+            const int groups = (ct + c - 1) / c;
+            const int cost = groups * (4 * rt + 40) + groups * c * (76 + 6 * rt);
+            if (best < 0 || cost <= best) { best = cost; CT = c; }
+        }
     } else {
         RT = rt;
         static const int cts[] = {4, 8, 11, 13};
@@ -914,28 +1011,41 @@ void ok_mfma_shape(int N, int M, int* rows_p, int* cols_p) {
     mfma_tiling(N, M, RT, CT, *rows_p, *cols_p);
 }
 
+bool ok_mfma_needs_planes(int N) { return ok_uses_bf3(N); }
+
 hipError_t launch_ok_mfma(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
-                          int64_t chunk, int nchunks, int rows_p, int cols_p, float* partials, hipStream_t stream) {
+                          int64_t chunk, int nchunks, int rows_p, int cols_p, float* partials, uint32_t* planes,
+                          hipStream_t stream) {
     int RT, CT, rp, cp;
     mfma_tiling(N, M, RT, CT, rp, cp);
     if (rp != rows_p || cp != cols_p) return hipErrorInvalidValue;
     const int ry = rows_p / (16 * RT), cz = cols_p / (16 * CT);
     const dim3 grid((unsigned)(8 * ((nchunks + 7) / 8) * ry * cz));
     const float* Tf = reinterpret_cast<const float*>(Trec);
-#define RBM_MF_CASE(K_, R_, C_)                                                                                   \
+#define RBM_MF_CASE(K_, R_, C_, SP_)                                                                              \
     if (RT == R_ && CT == C_) {                                                                                    \
-        hipLaunchKernelGGL((K_<R_, C_>), grid, dim3(64), 0, stream, n, N, M, Tf, spins, coef, chunk, nchunks, ry, cz, \
+        hipLaunchKernelGGL((K_<R_, C_>), grid, dim3(64), 0, stream, n, N, M, Tf, SP_, coef, chunk, nchunks, ry, cz,   \
                            rows_p, cols_p, partials);                                                              \
         return hipGetLastError();                                                                                  \
     }
     if (ok_uses_bf3(N)) {
-#define RBM_MF_ROW(R_) RBM_MF_CASE(k_ok_mfma_bf3, R_, 2)
+        // chunks are whole 32-sample blocks, and the bit-planes cover them
+        if (!planes || chunk % 32) return hipErrorInvalidValue;
+        const int64_t nblk = (n + 31) / 32;
+        const int64_t waves = (nblk + 1) / 2;
+        hipLaunchKernelGGL(k_spin_planes, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, stream, n, N, nblk, spins,
+                           planes);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+#define RBM_MF_ROW(R_)                                                                                             \
+    RBM_MF_CASE(k_ok_mfma_bf3, R_, 2, planes) RBM_MF_CASE(k_ok_mfma_bf3, R_, 3, planes)                            \
+    RBM_MF_CASE(k_ok_mfma_bf3, R_, 4, planes)
         RBM_MF_ROW(3) RBM_MF_ROW(4) RBM_MF_ROW(5) RBM_MF_ROW(6) RBM_MF_ROW(7) RBM_MF_ROW(8) RBM_MF_ROW(9)
 #undef RBM_MF_ROW
     } else {
 #define RBM_MF_ROW(R_)                                                                                             \
-    RBM_MF_CASE(k_ok_mfma_f32, R_, 4) RBM_MF_CASE(k_ok_mfma_f32, R_, 8) RBM_MF_CASE(k_ok_mfma_f32, R_, 11)           \
-    RBM_MF_CASE(k_ok_mfma_f32, R_, 13)
+    RBM_MF_CASE(k_ok_mfma_f32, R_, 4, spins) RBM_MF_CASE(k_ok_mfma_f32, R_, 8, spins)                              \
+    RBM_MF_CASE(k_ok_mfma_f32, R_, 11, spins) RBM_MF_CASE(k_ok_mfma_f32, R_, 13, spins)
         RBM_MF_ROW(1) RBM_MF_ROW(2)
 #undef RBM_MF_ROW
     }
@@ -1028,11 +1138,20 @@ __global__ __launch_bounds__(256) void k_mc_rbm_moves(RbmDev<T> md, McParams mp,
     const int chain_raw = blockIdx.x * 16 + row;
     const bool valid = chain_raw < mp.chains;
     const uint32_t chain = (uint32_t)(valid ? chain_raw : mp.chains - 1);
-    const u32x4 r0 = philox(chain, 0u, mp.call, 1u, mp.seed_lo, mp.seed_hi);
-    uint64_t s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
-    uint64_t s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
-    mask_words(s0, s1, N);
-    if (mp.move == MC_EXCHANGE) sector_start(N, mp.k_up, chain, mp.call, mp.seed_lo, mp.seed_hi, s0, s1);
+    uint64_t s0, s1;
+    if (mp.start) {
+        // a persistent chain continues from its last configuration (already in
+        // its sector for the exchange move)
+        s0 = mp.start[2 * chain];
+        s1 = mp.start[2 * chain + 1];
+        mask_words(s0, s1, N);
+    } else {
+        const u32x4 r0 = philox(chain, 0u, mp.call, 1u, mp.seed_lo, mp.seed_hi);
+        s0 = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
+        s1 = (uint64_t)r0.z | ((uint64_t)r0.w << 32);
+        mask_words(s0, s1, N);
+        if (mp.move == MC_EXCHANGE) sector_start(N, mp.k_up, chain, mp.call, mp.seed_lo, mp.seed_hi, s0, s1);
+    }
     cx<T> th[JPL];
     init_angles<T, JPL>(md, md.W, s0, s1, lane, th);
     T lane_lc = logcosh_re(th[0]);
@@ -1085,12 +1204,88 @@ __global__ __launch_bounds__(256) void k_mc_rbm_moves(RbmDev<T> md, McParams mp,
             nacc++;
         }
         while (step < mp.steps_per_chain && k + 1 == next_rec) record();
+        if (sizeof(T) == 4 && mp.resync > 0 && (k + 1) % (uint64_t)mp.resync == 0) {
+            init_angles<T, JPL>(md, md.W, s0, s1, lane, th);
+            lane_lc = logcosh_re(th[0]);
+#pragma unroll
+            for (int u = 1; u < JPL; u++) lane_lc += logcosh_re(th[u]);
+        }
     }
     if (lane == 0 && counts && valid) {
         counts[2 * chain] = nacc;
         counts[2 * chain + 1] = (uint32_t)(total - nacc);
     }
+    if (valid) mc_chain_outputs<T, JPL>(md, mp, (int)chain, lane, s0, s1, th);
 }
+
+// log psi'(s) - log psi(s) for two RBMs of the same shape from the angle
+// difference delta = (b' - b) + (W' - W)^T s, unit by unit with
+// logcosh_diff, plus sum_i (a' - a)_i s_i in FP64; lp2 = lp + that difference.
+// Subtracting two complex64 sums of M log-cosh values would lose the small
+// difference of nearby wavefunctions (HilbertSpaceDistance.cu:40-44, 83-105).
+template <typename T, int JPL>
+__global__ __launch_bounds__(256) void k_rbm_log_ratio(RbmDev<T> md, const cx<T>* __restrict__ dW,
+                                                       const cx<T>* __restrict__ db, const cx<double>* __restrict__ da,
+                                                       int64_t n, const uint64_t* __restrict__ configs,
+                                                       uint64_t index_base, const cx<double>* __restrict__ lp,
+                                                       cx<double>* __restrict__ lp2) {
+    const int N = md.N, ld = md.ld;
+    const int lane = threadIdx.x & 15;
+    const int groups = blockDim.x >> 4;
+    for (int64_t idx = (int64_t)blockIdx.x * groups + (threadIdx.x >> 4); idx < n;
+         idx += (int64_t)gridDim.x * groups) {
+        uint64_t s0, s1;
+        if (configs) { s0 = configs[2 * idx]; s1 = configs[2 * idx + 1]; }
+        else { s0 = index_base + (uint64_t)idx; s1 = 0; }
+        mask_words(s0, s1, N);
+        cx<T> th[JPL], dl[JPL];
+        init_angles<T, JPL>(md, md.W, s0, s1, lane, th);
+#pragma unroll
+        for (int u = 0; u < JPL; u++) dl[u] = db[lane + 16 * u];
+        for (int i = 0; i < N; i++) {
+            const T si = spin_of<T>(s0, s1, i);
+#pragma unroll
+            for (int u = 0; u < JPL; u++) dl[u] = dl[u] + si * dW[i * ld + lane + 16 * u];
+        }
+        cx<T> acc = logcosh_diff(th[0], dl[0]);
+#pragma unroll
+        for (int u = 1; u < JPL; u++) acc = acc + logcosh_diff(th[u], dl[u]);
+        cx<double> d = row_sum16(to_d(acc));
+        if (lane == 0) {
+            if (da)
+                for (int i = 0; i < N; i++) d = d + spin_of<double>(s0, s1, i) * da[i];
+            lp2[idx] = lp[idx] + d;
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_rbm_log_ratio(const RbmDev<T>& md, const void* dW, const void* db, const cx<double>* da, int64_t n,
+                                const uint64_t* configs, uint64_t index_base, const cx<double>* lp, cx<double>* lp2,
+                                int max_blocks, hipStream_t stream) {
+    const int jpl = pick_jpl(md.M);
+    int64_t blocks = (n + 15) / 16;
+    if (blocks > max_blocks) blocks = max_blocks;
+    if (blocks < 1) blocks = 1;
+    const cx<T>* w = reinterpret_cast<const cx<T>*>(dW);
+    const cx<T>* b = reinterpret_cast<const cx<T>*>(db);
+#define RBM_RATIO_CASE(J)                                                                                           \
+    case J:                                                                                                         \
+        hipLaunchKernelGGL((k_rbm_log_ratio<T, J>), dim3((unsigned)blocks), dim3(256), 0, stream, md, w, b, da, n, \
+                           configs, index_base, lp, lp2);                                                            \
+        break;
+    switch (jpl) {
+        RBM_JPL_LIST(RBM_RATIO_CASE)
+        default: return hipErrorInvalidValue;
+    }
+#undef RBM_RATIO_CASE
+    return hipGetLastError();
+}
+template hipError_t launch_rbm_log_ratio<float>(const RbmDev<float>&, const void*, const void*, const cx<double>*, int64_t,
+                                                const uint64_t*, uint64_t, const cx<double>*, cx<double>*, int, hipStream_t);
+template hipError_t launch_rbm_log_ratio<double>(const RbmDev<double>&, const void*, const void*, const cx<double>*,
+                                                 int64_t, const uint64_t*, uint64_t, const cx<double>*, cx<double>*, int,
+                                                 hipStream_t);
 
 int max_hidden_units() { return 16 * 16; }
 int rbm_padded_width(int M) { return 16 * pick_jpl(M); }
@@ -1353,7 +1548,8 @@ __global__ __launch_bounds__(512) void k_theta_logpsi(RbmDev<float> md, int64_t 
                 for (int r = 0; r < TL_TILES; r++)
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
-                        const cx<float> lc = logcosh(mk(acc[r][c][0][v] + bj.re, acc[r][c][1][v] + bj.im));
+                        // the cancellation-free form: log psi is the result here
+                        const cx<float> lc = logcosh_shifted(mk(acc[r][c][0][v] + bj.re, acc[r][c][1][v] + bj.im));
                         lre[r][v] += lc.re;
                         lim[r][v] += lc.im;
                     }
@@ -1367,7 +1563,7 @@ __global__ __launch_bounds__(512) void k_theta_logpsi(RbmDev<float> md, int64_t 
             const float sr = row_sum16(lre[r][v]);
             const float si = row_sum16(lim[r][v]);
             const int64_t p = base + 16 * r + 4 * g + v;
-            if ((lane & 15) == 0 && p < n) lp[p] = mk((double)sr, (double)si);
+            if ((lane & 15) == 0 && p < n) lp[p] = mk((double)sr + (double)md.M * LC_L0, (double)si);
         }
 }
 

@@ -39,7 +39,26 @@ struct McParams {
     int move = MC_SINGLE_FLIP;
     int tries = 1;
     int k_up = 0;
+    // RBM samplers only:
+    //   start      chain start configurations (2 words per chain) instead of
+    //              the Philox random start (persistent chains), or null;
+    //   final_out  where each chain leaves its last configuration, or null
+    //              (may alias start: a chain reads its slot before writing it);
+    //   angles_out each chain's final angles (chains x M complex of the
+    //              kernel's type), or null;
+    //   resync     FP32 chains recompute theta = b + W^T s from the
+    //              configuration at the first draw-block end after every
+    //              `resync` proposals (0: never), which bounds the drift of
+    //              the incremental updates.
+    const uint64_t* start = nullptr;
+    uint64_t* final_out = nullptr;
+    void* angles_out = nullptr;
+    int resync = 0;
 };
+
+// FP32 Metropolis chains recompute their angles from the configuration every
+// MC_RESYNC_SWEEPS sweeps (McParams::resync = MC_RESYNC_SWEEPS * N proposals)
+constexpr int MC_RESYNC_SWEEPS = 16;
 
 constexpr int SC_STRIDE = 4;
 constexpr int OK_ROWS_PER_THREAD = 8;
@@ -79,6 +98,13 @@ template <typename T>
 hipError_t launch_eval(const RbmDev<T>& md, const OpDev& op, int has_op, int64_t n, const uint64_t* configs,
                        uint64_t index_base, const Records& rec, bool wlds, size_t lds_bytes, int max_blocks,
                        hipStream_t stream);
+// lp2 = lp + log psi'(s) - log psi(s) for two RBMs of one shape, from the
+// angle difference (dW = W' - W as N x ld, db = b' - b as ld, both cx<T>;
+// da = a' - a as N complex128 or null)
+template <typename T>
+hipError_t launch_rbm_log_ratio(const RbmDev<T>& md, const void* dW, const void* db, const cx<double>* da, int64_t n,
+                                const uint64_t* configs, uint64_t index_base, const cx<double>* lp, cx<double>* lp2,
+                                int max_blocks, hipStream_t stream);
 template <typename T>
 hipError_t launch_ok_reduce(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
                             int nc, int64_t chunk, int nblocks, cx<double>* partials, hipStream_t stream);
@@ -88,8 +114,13 @@ hipError_t launch_ok_reduce(int64_t n, int N, int M, const void* Trec, const uin
 // padded row / column counts of the MFMA contraction for an N x M RBM
 void ok_mfma_shape(int N, int M, int* rows_p, int* cols_p);
 constexpr int MFMA_SUM_SPLIT = 16;   // FP64 partial-sum groups (Psum holds MFMA_SUM_SPLIT x rows_p x cols_p)
+// N >= 32 uses the bfloat16 kernel, which reads the spins as 32-sample
+// bit-planes: planes must hold ceil(n / 32) x N uint32 and chunk be a multiple
+// of 32; smaller N ignore planes (may be null).
+bool ok_mfma_needs_planes(int N);
 hipError_t launch_ok_mfma(int64_t n, int N, int M, const void* Trec, const uint64_t* spins, const cx<double>* coef,
-                          int64_t chunk, int nchunks, int rows_p, int cols_p, float* partials, hipStream_t stream);
+                          int64_t chunk, int nchunks, int rows_p, int cols_p, float* partials, uint32_t* planes,
+                          hipStream_t stream);
 // FP64 sum of FP32 partials over chunks, then R[r][j] (complex) from P.
 hipError_t launch_mfma_finish(int nchunks, int N, int M, int rows_p, int cols_p, const float* partials, double* Psum,
                               cx<double>* R, hipStream_t stream);

@@ -12,6 +12,14 @@ c1_tfim_rbm.npz: 1-D TFIM N=10 (J=h=1, periodic), RBM M=20 with parameters
   Hilbert-space distance (+ gradient) to a copy with W += N(0, 1e-3^2).
 deep_small.npz: PsiDeep N=6, layers [12, 6] (connectivity [3, 12]) on a
   random operator: log psi, E_loc, O_k at three configurations, gradient.
+c2_tfim_rbm.npz: 1-D TFIM N=20 (J=h=1, periodic), RBM M=80 with parameters
+  drawn i.i.d. N(0, 0.01^2) (seed 1236): the normalising prefactor, energy
+  and the 1720-entry gradient under ExactSummation over all 2^20 states
+  (the centred estimator, equal to the reference's E_k form for this
+  Hermitian operator), and the Hilbert-space distance (identity operator,
+  unitary form, nu = 1) and its gradient to a copy with W += N(0, 1e-3^2).
+  `python tests/golden/make_golden.py c2` rewrites only this file (about two
+  minutes of oracle time).
 """
 import os
 import sys
@@ -74,7 +82,25 @@ def deep_small():
     return out
 
 
+def c2():
+    rng = np.random.default_rng(1236)
+    N, M = 20, 80
+    cn = lambda *s: 0.01 * (rng.normal(size=s) + 1j * rng.normal(size=s))  # noqa: E731
+    b, W = cn(M), cn(N, M)
+    W2 = W + 1e-3 * (rng.normal(size=(N, M)) + 1j * rng.normal(size=(N, M)))
+    op = O.Operator(*HAM.tfim_1d(N, 1.0, 1.0))
+    psi = O.RBM(b, W)
+    psi.prefactor = 1.0 / O.psi_norm(psi, O.Exact(N))
+    grad, energy = O.gradient(psi, op, O.Exact(N), mode=1)
+    ident = O.Operator(np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    hs_grad, hs_d = O.hs_distance(psi, O.RBM(b, W2), ident, True, O.Exact(N), want_grad=True, nu=1.0)
+    return {"b": b, "W": W, "W2": W2, "prefactor": np.array(psi.prefactor), "energy": np.array(energy),
+            "gradient": grad, "hs_distance": np.array(hs_d), "hs_gradient": hs_grad}
+
+
 if __name__ == "__main__":
-    np.savez_compressed(os.path.join(HERE, "c1_tfim_rbm.npz"), **c1(False), **c1(True))
-    np.savez_compressed(os.path.join(HERE, "deep_small.npz"), **deep_small())
+    if sys.argv[1:] != ["c2"]:
+        np.savez_compressed(os.path.join(HERE, "c1_tfim_rbm.npz"), **c1(False), **c1(True))
+        np.savez_compressed(os.path.join(HERE, "deep_small.npz"), **deep_small())
+    np.savez_compressed(os.path.join(HERE, "c2_tfim_rbm.npz"), **c2())
     print("wrote", os.listdir(HERE))

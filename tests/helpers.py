@@ -140,3 +140,101 @@ def deep_bf16_log_psi(N, biases, conns, weights, final, words):
         y = O.logcosh(a)
         x = bf16c(y) if l + 1 < len(dense) else y
     return x @ np.asarray(final, np.complex64).astype(complex)
+
+
+# ------------------------------------------------------------------ branch disagreements
+# my_logcosh is sign-branched on Re theta > 0 (psi_functions.hpp:49-57) and jumps
+# across Re theta = 0; a complex64 angle on the other side of 0 than the FP64
+# angle is evaluated on the other branch.  These helpers replay the kernels'
+# FP32 angle arithmetic to find such configurations exactly.
+def spins_pm(words, N):
+    s = np.where((words[:, :1] >> np.arange(min(N, 64), dtype=np.uint64)) & np.uint64(1), 1.0, -1.0)
+    if N > 64:
+        s = np.concatenate([s, np.where((words[:, 1:] >> np.arange(N - 64, dtype=np.uint64)) & np.uint64(1), 1.0, -1.0)],
+                           axis=1)
+    return s
+
+
+def flip_groups(tab):
+    """Distinct flip sets of the table's strings (X and Y positions, sorted
+    ascending like the canonical operator form) with the sum of |coefficient|
+    of the strings that share each set."""
+    c, t, idx = tab
+    groups = {}
+    for n in range(len(c)):
+        fl = tuple(sorted({int(idx[n, k]) for k in range(t.shape[1]) if idx[n, k] != -1 and t[n, k] in (1, 2)}))
+        if fl:
+            groups[fl] = groups.get(fl, 0.0) + abs(c[n])
+    return groups
+
+
+def _jump(y):
+    """|my_logcosh(+0 + i y) - my_logcosh(-0 + i y)|, the jump across Re = 0."""
+    return np.abs(O.logcosh(1e-300 + 1j * y) - O.logcosh(-1e-300 + 1j * y))
+
+
+def branch_analysis(b, W, s, tab):
+    """Replays the kernel's FP32 angles.  Returns per configuration:
+    own   -- an angle of s itself is on the other side of Re = 0 than in FP64
+             (log psi and every ratio move by the jump);
+    any   -- own, or an angle of a flipped neighbour s' disagrees (that ratio
+             moves);
+    bound -- for flagged configurations, an upper bound of the E_loc change
+             the disagreements can cause: sum over flip sets of
+             |coefficients| |psi(s')/psi(s)| x (the largest jump involved)."""
+    W32 = W.astype(np.complex64)
+    th32 = np.tile(b.astype(np.complex64), (len(s), 1))
+    for i in range(W.shape[0]):
+        th32 = th32 + s[:, i:i + 1].astype(np.float32) * W32[i][None, :]
+    th64 = b[None, :] + s @ W
+
+    def worst_jump(v32, v64):
+        dis = (v32.real > 0) != (v64.real > 0)
+        out = np.zeros(len(s))
+        if dis.any():
+            rows, cols = np.nonzero(dis)
+            np.maximum.at(out, rows, _jump(np.abs(v64.imag[rows, cols])))
+        return out
+
+    j_own = worst_jump(th32, th64)
+    groups = flip_groups(tab)
+    j_term = np.zeros((len(s), len(groups)))
+
+    def neighbour(th, fl, rows, dtype):
+        v = th[rows].copy()
+        for p in fl:
+            v = v + (-2.0 * s[rows, p:p + 1]).astype(dtype) * W.astype(v.dtype)[p][None, :]
+        return v
+
+    every = np.arange(len(s))
+    for g, fl in enumerate(groups):
+        j_term[:, g] = worst_jump(neighbour(th32, fl, every, np.float32), neighbour(th64, fl, every, np.float64))
+    own = j_own > 0
+    anyd = own | np.any(j_term > 0, axis=1)
+    bound = np.zeros(len(s))
+    rows = np.nonzero(anyd)[0]
+    if len(rows):
+        lc0 = O.logcosh(th64[rows]).sum(axis=1)
+        for g, (fl, coef) in enumerate(groups.items()):
+            ratio = np.abs(np.exp(O.logcosh(neighbour(th64, fl, rows, np.float64)).sum(axis=1) - lc0))
+            bound[rows] += coef * ratio * np.maximum(j_own[rows], j_term[rows, g])
+    return own, anyd, bound
+
+
+def check_local_energies(label, el, El, lp, lpr, b, W, s, tab):
+    own, anyd, bound = branch_analysis(b, W, s, tab)
+    good = ~anyd
+    scale = np.abs(El).max()
+    err_good = float(np.abs(el[good] - El[good]).max() / scale)
+    err_lp = float(np.abs(lp[~own] - lpr[~own]).max() / np.abs(lpr).max())
+    print(f"[{label}] n={len(El)}  branch-disagreeing configurations: {int(anyd.sum())} ({int(own.sum())} at s "
+          f"itself)  E_loc rel err (others) {err_good:.2e}  log psi rel err {err_lp:.2e}", flush=True)
+    assert err_good < 1e-5
+    assert err_lp < 1e-5
+    if anyd.any():
+        dev = np.abs(el[anyd] - El[anyd])
+        print(f"  disagreeing: |E_loc error| {dev.max():.2e} against the jump bound {bound[anyd].min():.2e}.."
+              f"{bound[anyd].max():.2e}", flush=True)
+        assert np.all(dev <= 1.05 * bound[anyd] + 1e-5 * scale)
+        assert anyd.mean() < 0.02
+    return good

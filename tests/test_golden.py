@@ -40,3 +40,16 @@ def test_oracle_reproduces_deep_fixture():
     assert rel_err(O.local_energy(psi, op, basis_words(6)), d["e_loc"]) < 1e-13
     ok = np.stack([O.o_k(psi, w) for w in O.spins_words(d["ok_configs"])])
     assert rel_err(ok, d["o_k"]) < 1e-14
+
+
+def test_oracle_reproduces_c2_fixture_norm_and_distance():
+    """The c2 fixture (N = 20, 2^20 states): the normalisation and the
+    Hilbert-space distance recomputed by the oracle (about 10 s; the energy
+    and gradients take minutes and are regenerated with make_golden.py)."""
+    d = np.load(os.path.join(GOLDEN, "c2_tfim_rbm.npz"))
+    psi = O.RBM(d["b"], d["W"])
+    assert abs(1.0 / O.psi_norm(psi, O.Exact(20)) - d["prefactor"]) < 1e-13 * d["prefactor"]
+    psi.prefactor = float(d["prefactor"])
+    ident = O.Operator(np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32))
+    hd = O.hs_distance(psi, O.RBM(d["b"], d["W2"]), ident, True, O.Exact(20))
+    assert abs(hd - d["hs_distance"]) < 1e-12 * d["hs_distance"]

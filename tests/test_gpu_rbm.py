@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import O, basis_words, dense_hamiltonian, random_operator, random_rbm, rel_err
+from helpers import O, basis_words, check_local_energies, dense_hamiltonian, random_operator, random_rbm, rel_err, spins_pm
 
 import rbmongpu_amd as R
 from rbmongpu_amd import hamiltonians as HAM
@@ -33,6 +33,16 @@ def words_to_int(w):
     return [int(x[0]) | (int(x[1]) << 64) for x in w]
 
 
+def fluctuation_bar(tol, f, E):
+    """The fluctuation is sqrt(<|E_loc|^2> - |<E_loc>|^2) (ExpectationValue.cu:
+    130-175).  With <|E_loc|^2> = f^2 + |E|^2 held to tol and <E_loc> to tol
+    (so |<E_loc>|^2 to 2 tol), f^2 carries at most tol (f^2 + 3 |E|^2) and f
+    half of that relative to f^2: the bar is tol (f^2 + 3 |E|^2) / (2 f),
+    about tol f / 2 where the variance does not cancel and growing as
+    |E|^2 / f^2 where it does."""
+    return tol * (f * f + 3 * abs(E) ** 2) / (2 * f)
+
+
 # ------------------------------------------------------------------ golden
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("pre", ["", "vis_"])
@@ -45,6 +55,9 @@ def test_c1_golden(precision, pre):
     op = gpu_op(HAM.tfim_1d(10))
     ex = R.ExactSummation(10, True)
     assert rel_err(psi.vector, d[pre + "psi"]) < tol
+    el = R.local_energy_batch(psi, op, basis_words(10))
+    assert rel_err(el, d[pre + "e_loc"]) < tol                 # per state, all 1024 of them
+    assert np.abs(el - d[pre + "e_loc"]).max() < tol * np.abs(d[pre + "e_loc"]).max()
     psi.normalize(ex)
     assert abs(psi.prefactor - d[pre + "prefactor"]) < tol * d[pre + "prefactor"]
     psi.prefactor = float(d[pre + "prefactor"])
@@ -53,7 +66,8 @@ def test_c1_golden(precision, pre):
     assert abs(E - d[pre + "energy"]) < tol * abs(d[pre + "energy"])
     assert rel_err(g, d[pre + "gradient"]) < tol
     f, E2 = ev.fluctuation(psi, op, ex)
-    assert abs(f - d[pre + "fluctuation"]) < tol * abs(d[pre + "fluctuation"]) * 10
+    fr = float(d[pre + "fluctuation"])
+    assert abs(f - fr) < fluctuation_bar(tol, fr, d[pre + "energy"]), (f, fr)
     psi2 = gpu_psi(d[pre + "b"], d[pre + "W2"], a, precision=precision)
     hs = R.HilbertSpaceDistance(10, psi2.num_params, True)
     hg, hd = hs.gradient(psi, psi2, op, True, ex, 1.0)
@@ -171,9 +185,9 @@ def test_exact_functions(precision, sector):
     assert abs(El[1] - O.expectation(ref_psi, O.Operator(*tab2), oex)) < tol * abs(El[1])
     assert abs(El[0] - E) < 1e-12 * abs(E)
     f, E2 = ev.fluctuation(psi, gpu_op(tab), ex)
-    fr, _ = O.fluctuation(ref_psi, O.Operator(*tab), oex)
+    fr, Er = O.fluctuation(ref_psi, O.Operator(*tab), oex)
     if np.isfinite(fr):
-        assert abs(f - fr) < tol * 100 * abs(fr)
+        assert abs(f - fr) < fluctuation_bar(tol, fr, Er), (f, fr)
     g, E3 = ev.gradient(psi, gpu_op(tab), ex)
     gr, _ = O.gradient(ref_psi, O.Operator(*tab), oex, mode=0)
     assert rel_err(g, gr) < tol
@@ -355,40 +369,83 @@ def test_mc_gradient_matches_exact_statistically():
 
 
 # ------------------------------------------------------------------ full size: 1-D TFIM N = 20, M = 80
-def test_c2_full_size_properties():
-    """N=20, M=80 TFIM with N(0, 0.01^2) parameters: ExactSummation over 2^20
-    states in complex64 agrees with complex128 to 1e-5; the 4096-chain
-    sampler agrees with the exact energy within 3 standard errors; the
-    distance to the W + N(0, 1e-3^2) copy agrees between precisions."""
-    print("[TFIM N=20 M=80 full size]", flush=True)
-    rng = np.random.default_rng(1236)
-    N, M = 20, 80
-    cn = lambda *s: 0.01 * (rng.normal(size=s) + 1j * rng.normal(size=s))  # noqa: E731
-    b, W = cn(M), cn(N, M)
-    W2 = W + 1e-3 * (rng.normal(size=(N, M)) + 1j * rng.normal(size=(N, M)))
+@pytest.fixture(scope="module")
+def c2_golden():
+    return np.load(os.path.join(GOLDEN, "c2_tfim_rbm.npz"))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_c2_exact_summation_matches_golden(c2_golden, precision):
+    """N = 20, M = 80 TFIM with N(0, 0.01^2) parameters, ExactSummation over
+    all 2^20 states: the normalising prefactor, energy, 1720-entry gradient,
+    and the Hilbert-space distance (and its gradient) to the W + N(0, 1e-3^2)
+    copy, against the oracle's values (tests/golden/c2_tfim_rbm.npz): 1e-10
+    in complex128, 1e-5 in complex64.  The complex64 distance is formed from
+    the per-unit log-cosh differences of the two angle sets
+    (k_rbm_log_ratio), which keeps the small log psi' - log psi accurate.
+    In complex128 the distance d = sqrt(1 - F) of the nearby states
+    (F = |<psi|psi'>|^2 / (<psi|psi><psi'|psi'>) = 1 - 1.2e-5) turns relative
+    errors of F's 2^20-term sums into (1 - d^2) / (2 d^2) = 4e4 times larger
+    ones of d, so there the 1e-10 bar holds F and the gradient of F,
+    -2 d grad d, which carry no such factor."""
+    print(f"[c2 golden] precision={precision}", flush=True)
+    d = c2_golden
+    tol = TOL[precision]
+    N = 20
     op = gpu_op(HAM.tfim_1d(N))
     ident = R.Operator(np.ones(1, complex), np.zeros((1, 1), np.int32), -np.ones((1, 1), np.int32), True)
     ex = R.ExactSummation(N, True)
+    psi = gpu_psi(d["b"], d["W"], precision=precision)
+    psi.normalize(ex)
+    errs = {"prefactor": abs(psi.prefactor - d["prefactor"]) / d["prefactor"]}
+    psi.prefactor = float(d["prefactor"])
+    g, E = R.ExpectationValue(True).gradient(psi, op, ex)
+    errs["energy"] = abs(E - d["energy"]) / abs(d["energy"])
+    errs["gradient"] = rel_err(g, d["gradient"])
+    hs = R.HilbertSpaceDistance(N, psi.num_params, True)
+    psi2 = gpu_psi(d["b"], d["W2"], precision=precision)
+    hg, hd = hs.gradient(psi, psi2, ident, True, ex, 1.0)
+    d_ref, g_ref = float(d["hs_distance"]), d["hs_gradient"]
+    if precision == 64:
+        errs["hs_fidelity"] = abs((1 - hd ** 2) - (1 - d_ref ** 2)) / (1 - d_ref ** 2)
+        errs["hs_fidelity_gradient"] = rel_err(-2 * hd * hg, -2 * d_ref * g_ref)
+    else:
+        errs["hs_distance"] = abs(hd - d_ref) / d_ref
+        errs["hs_gradient"] = rel_err(hg, g_ref)
+    print("  " + "  ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v < tol, (k, v)
+
+
+def test_c2_sampled_energy_and_gradient_against_exact(c2_golden):
+    """The plain GPU chain at the full configuration (4096 chains, 200
+    thermalisation and 1000 measurement sweeps of 20 flips) against the exact
+    2^20 values: the energy within 3 standard errors (per-chain means), and a
+    z-test of the 1720-entry gradient over six independent seeds (median
+    |z| < 1.5, 95 % of |z| < 4 on the entries above 5 % of the largest)."""
+    print("[c2 sampled vs exact]", flush=True)
+    d = c2_golden
+    N, chains, per = 20, 4096, 1000
+    psi = gpu_psi(d["b"], d["W"], prefactor=float(d["prefactor"]), precision=32)
+    op = gpu_op(HAM.tfim_1d(N))
     ev = R.ExpectationValue(True)
-    res = {}
-    for precision in (64, 32):
-        psi = gpu_psi(b, W, precision=precision)
-        psi.normalize(ex)
-        g, E = ev.gradient(psi, op, ex)
-        hs = R.HilbertSpaceDistance(N, psi.num_params, True)
-        d = hs(psi, gpu_psi(b, W2, precision=precision), ident, True, ex)
-        res[precision] = (psi.prefactor, E, g, d)
-    assert abs(res[32][0] - res[64][0]) < 1e-5 * res[64][0]
-    assert abs(res[32][1] - res[64][1]) < 1e-5 * abs(res[64][1])
-    assert rel_err(res[32][2], res[64][2]) < 1e-5
-    assert abs(res[32][3] - res[64][3]) < 1e-3 * res[64][3]
-    psi = gpu_psi(b, W, precision=32)
-    psi.prefactor = res[64][0]
-    mc = R.MonteCarloLoop(4096 * 100, 1, 50, 4096, True, seed=2)
-    E_mc = ev(psi, op, mc)
-    _, _, el = mc.last_samples()
-    m, se = _chain_mean_se(el.real, 4096)
-    assert abs(E_mc.real - res[64][1].real) < 3 * se, (E_mc, res[64][1], se)
+    grads = []
+    for rep in range(6):
+        mc = R.MonteCarloLoop(chains * per, 1, 200, chains, True, seed=2 + rep)
+        g, E_mc = ev.gradient(psi, op, mc)
+        grads.append(g)
+        if rep == 0:
+            _, _, el = mc.last_samples()
+            m, se = _chain_mean_se(el.real, chains)
+            assert abs(E_mc.real - d["energy"].real) < 3 * se, (E_mc, d["energy"], se)
+    grads = np.array(grads)
+    mean, se = grads.mean(axis=0), grads.std(axis=0, ddof=1) / np.sqrt(len(grads))
+    g_ex = d["gradient"]
+    big = np.abs(g_ex) > 0.05 * np.abs(g_ex).max()
+    z = np.abs(mean - g_ex)[big] / np.maximum(np.abs(se[big]), 1e-300)
+    print(f"  gradient z over {int(big.sum())} entries: median {np.median(z):.2f}, "
+          f"fraction below 4: {np.mean(z < 4):.3f}", flush=True)
+    assert np.median(z) < 1.5 and np.mean(z < 4) > 0.95
 
 
 @pytest.mark.parametrize("N,M", [(100, 200), (36, 144), (128, 16), (50, 7), (20, 80)])
@@ -397,8 +454,11 @@ def test_mc_gradient_contraction_shapes(N, M):
     column tiles): the complex64 gradient of recorded Metropolis samples against
     the FP64 centred estimator 2/n sum_s (E_s - <E>) conj(O_k(s)) on the same
     samples and local energies, O_k from the oracle's my_tanh.  The complex64
-    local energies of these larger models (sums of up to 100 ratios over 200
-    units) are checked against the oracle at 5e-5."""
+    local energies and log psi of these larger models (sums of up to 100 ratios
+    over 200 units) are held to 1e-5 against the oracle, apart from the
+    configurations whose FP32 and FP64 angles lie on different sides of the
+    my_logcosh branch cut (counted and bounded by the jump,
+    helpers.check_local_energies)."""
     print(f"[gradient shapes] N={N} M={M}", flush=True)
     b, W, _ = random_rbm(N, M, seed=3 * N + M, scale=0.05)
     psi = gpu_psi(b, W, precision=32)
@@ -406,16 +466,27 @@ def test_mc_gradient_contraction_shapes(N, M):
     chains, per = 512, 6
     mc = R.MonteCarloLoop(chains * per, 1, 5, chains, True, seed=N)
     g, E = R.ExpectationValue(True).gradient(psi, gpu_op(tab), mc)
-    spins, _, el = mc.last_samples()
-    s = np.where((spins[:, :1] >> np.arange(min(N, 64), dtype=np.uint64)) & np.uint64(1), 1.0, -1.0)
-    if N > 64:
-        s = np.concatenate([s, np.where((spins[:, 1:] >> np.arange(N - 64, dtype=np.uint64)) & np.uint64(1), 1.0, -1.0)],
-                           axis=1)
+    spins, lp, el = mc.last_samples()
+    s = spins_pm(spins, N)
     T = O.tanh(b[None, :] + s @ W)
     Ok = np.concatenate([T, (s[:, :, None] * T[:, None, :]).reshape(len(s), N * M)], axis=1)
     El = O.local_energy(O.RBM(b, W), O.Operator(*tab), spins)
-    assert rel_err(el, El) < 5e-5
+    check_local_energies(f"TFIM N={N} M={M}", el, El, lp, O.log_psi(O.RBM(b, W), spins), b, W, s, tab)
     ref = (2.0 / len(s)) * ((el - el.mean())[:, None] * Ok.conj()).sum(axis=0)
     err = rel_err(g[2 * N:], ref)
     print(f"  gradient rel err {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+def test_profiling_switches_bracket_gpu_work():
+    """start_profiling / stop_profiling (source/RBMonGPU.cu:11-17) as roctx
+    collection switches: they bracket device work and leave results intact."""
+    b, W, _ = random_rbm(8, 8, seed=91, scale=0.1)
+    psi = gpu_psi(b, W, precision=64)
+    ex = R.ExactSummation(8, True)
+    op = gpu_op(HAM.tfim_1d(8))
+    E0 = R.ExpectationValue(True)(psi, op, ex)
+    R.start_profiling()
+    E1 = R.ExpectationValue(True)(psi, op, ex)
+    R.stop_profiling()
+    assert E1 == E0
