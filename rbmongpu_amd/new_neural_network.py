@@ -32,18 +32,38 @@ def new_neural_network(N, M, initial_value=(0.01 + 1j * math.pi / 4), alpha=0, b
     return Psi(alpha, beta, b, W, 1, free_quantum_axis, gpu, **kwargs)
 
 
-def deep_connections(N, M, C):
-    """Connection tables (j * delta + i) % n of new_neural_network.py:116-127 (1-D)."""
-    def delta_func(n, m, c):
-        if m > n:
-            return 1
-        elif n % m == 0:
-            return n // m
-        return c
+def _stride(n, m, c):
+    """Input stride between neighbouring units of a layer of m units over n
+    inputs with c connections each (new_neural_network.py:108-114)."""
+    if m > n:
+        return 1
+    if n % m == 0:
+        return n // m
+    return c
 
+
+def deep_connections(N, M, C):
+    """Connection tables of the reference's deep constructor
+    (new_neural_network.py:116-145).  1-D: unit j of a layer reads inputs
+    (j * stride + i) % n for i < c.  2-D (N = (n0, n1), layers and
+    connection counts as pairs): unit (j0, j1) reads the c0 x c1 window of
+    inputs ((j0 * stride0 + i0) % n0, (j1 * stride1 + i1) % n1), units and
+    window positions in row-major order, inputs numbered row * n1 + column.
+    (The reference numbers them row * n0 + column, which is the same on the
+    square lattices it is used with and addresses past the layer otherwise.)"""
     tables = []
+    if isinstance(N, (list, tuple)):
+        for n, m, c in zip([tuple(N)] + [tuple(x) for x in M[:-1]], [tuple(x) for x in M], [tuple(x) for x in C]):
+            d0, d1 = _stride(n[0], m[0], c[0]), _stride(n[1], m[1], c[1])
+            rows = []
+            for i0 in range(c[0]):
+                for i1 in range(c[1]):
+                    rows.append([((j0 * d0 + i0) % n[0]) * n[1] + (j1 * d1 + i1) % n[1]
+                                 for j0 in range(m[0]) for j1 in range(m[1])])
+            tables.append(np.array(rows, dtype=np.uint32))
+        return tables
     for n, m, c in zip([N] + M[:-1], M, C):
-        d = delta_func(n, m, c)
+        d = _stride(n, m, c)
         tables.append(np.array([[(j * d + i) % n for j in range(m)] for i in range(c)], dtype=np.uint32))
     return tables
 
@@ -51,11 +71,21 @@ def deep_connections(N, M, C):
 def new_deep_neural_network(N, M, C, initial_value=(0.01 + 1j * math.pi / 4), alpha=0, beta=0,
                             free_quantum_axis=False, noise=1e-4, gpu=True, noise_modulation="auto", final_weights=10,
                             rng=None, **kwargs):
-    """PsiDeep with the reference's initialisation (new_neural_network.py:53-153), 1-D layouts."""
+    """PsiDeep with the reference's initialisation (new_neural_network.py:53-153):
+    1-D layouts (N, M and C integers) or 2-D ones (N a pair, M and C lists of
+    pairs; the layer sizes are the products)."""
     from . import PsiDeep
-    for n, m, c in zip([N] + M[:-1], M, C):
-        assert (m * c) % n == 0
-        assert c <= n
+    two_d = isinstance(N, (list, tuple))
+    for n, m, c in zip([N] + list(M[:-1]), M, C):
+        if two_d:
+            assert (m[0] * c[0]) % n[0] == 0 and (m[1] * c[1]) % n[1] == 0
+            assert c[0] <= n[0] and c[1] <= n[1]
+        else:
+            assert (m * c) % n == 0
+            assert c <= n
+    connections = deep_connections(N, list(M), list(C))
+    if two_d:
+        N, M, C = N[0] * N[1], [m[0] * m[1] for m in M], [c[0] * c[1] for c in C]
     if isinstance(alpha, (float, int)):
         alpha = alpha * np.ones(N)
     if isinstance(beta, (float, int)):
@@ -68,7 +98,6 @@ def new_deep_neural_network(N, M, C, initial_value=(0.01 + 1j * math.pi / 4), al
         noise_modulation = [math.sqrt(6 / (c + next_c)) for c, m, next_c in zip(C[1:], M[1:], C[2:] + [1])]
     for c, m, next_c, nm in zip(C[1:], M[1:], C[2:] + [1], noise_modulation):
         W.append(nm * math.sqrt(6 / (c + next_c)) * real_noise((c, m), rng) + noise * complex_noise((c, m), rng))
-    connections = deep_connections(N, M, C)
     if isinstance(final_weights, (float, int)):
         final_weights = final_weights * np.ones(M[-1])
     assert len(final_weights) == M[-1]

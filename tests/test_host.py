@@ -189,6 +189,61 @@ def test_json_roundtrip():
     assert all(np.array_equal(a, b) for a, b in zip(deep2.connections, deep.connections))
 
 
+# Documents in the reference's on-disk format, written out by hand from
+# pyRBMonGPU/json_numpy.py:5-38 (array objects), Psi.py:8-21 and
+# PsiDeep.py:8-23 (the model records): a renamed key, another array encoding
+# or another dtype string fails the equality below.
+PSI_DOC = {
+    "type": "Psi",
+    "alpha": {"type": "ndarray", "dtype": "float64", "complex": False, "data": [0.5, 0.25]},
+    "beta": {"type": "ndarray", "dtype": "float64", "complex": False, "data": [0.0, 1.0]},
+    "b": {"type": "ndarray", "dtype": "complex128", "complex": True, "real": [1.0], "imag": [2.0]},
+    "W": {"type": "ndarray", "dtype": "complex128", "complex": True, "real": [[0.5], [0.0]], "imag": [[-1.0], [2.0]]},
+    "prefactor": 0.5,
+}
+DEEP_DOC = {
+    "type": "PsiDeep",
+    "alpha": {"type": "ndarray", "dtype": "float64", "complex": False, "data": [0.0, 0.0]},
+    "beta": {"type": "ndarray", "dtype": "float64", "complex": False, "data": [0.0, 0.0]},
+    "b": [{"type": "ndarray", "dtype": "complex128", "complex": True, "real": [0.25, -0.5], "imag": [0.0, 1.5]}],
+    "connections": [{"type": "ndarray", "dtype": "uint32", "complex": False, "data": [[0, 1], [1, 0]]}],
+    "W": [{"type": "ndarray", "dtype": "complex128", "complex": True, "real": [[1.0, 2.0], [3.0, 4.0]],
+           "imag": [[0.5, 0.0], [0.0, -0.5]]}],
+    "final_weights": {"type": "ndarray", "dtype": "complex128", "complex": True, "real": [10.0, 10.0],
+                      "imag": [0.0, 0.0]},
+    "prefactor": 1.0,
+    "free_quantum_axis": False,
+}
+
+
+def test_json_documents_have_the_reference_layout():
+    psi = R.Psi(np.array([0.5, 0.25]), np.array([0.0, 1.0]), np.array([1 + 2j]), np.array([[0.5 - 1j], [2j]]), 0.5,
+                False, True)
+    assert psi.to_json() == PSI_DOC
+    # the text round trip keeps key order and value types
+    assert json.loads(json.dumps(psi.to_json())) == PSI_DOC
+    deep = R.PsiDeep(np.zeros(2), np.zeros(2), [np.array([0.25, -0.5 + 1.5j])],
+                     [np.array([[0, 1], [1, 0]], dtype=np.uint32)], [np.array([[1 + 0.5j, 2], [3, 4 - 0.5j]])],
+                     np.array([10.0, 10.0], complex), 1.0, False, True)
+    assert deep.to_json() == DEEP_DOC
+
+
+def test_json_documents_in_the_reference_layout_load():
+    """A document in the reference's layout loads into the same model."""
+    psi = R.Psi.from_json(json.loads(json.dumps(PSI_DOC)), True)
+    assert np.array_equal(psi.alpha, [0.5, 0.25]) and np.array_equal(psi.beta, [0.0, 1.0])
+    assert np.array_equal(psi.b, [1 + 2j]) and np.array_equal(psi.W, [[0.5 - 1j], [2j]])
+    assert psi.prefactor == 0.5
+    deep = R.PsiDeep.from_json(json.loads(json.dumps(DEEP_DOC)), True)
+    assert np.array_equal(deep.W[0], [[1 + 0.5j, 2], [3, 4 - 0.5j]])
+    assert np.array_equal(deep.b[0], [0.25, -0.5 + 1.5j])
+    assert np.array_equal(deep.connections[0], [[0, 1], [1, 0]]) and deep.connections[0].dtype == np.uint32
+    assert np.array_equal(deep.final_weights, [10.0, 10.0])
+    # the encoder's array objects decode with the reference's rule
+    arr = json.loads(json.dumps({"x": np.arange(3, dtype=np.int64)}, cls=R.NumpyEncoder), cls=R.NumpyDecoder)["x"]
+    assert arr.dtype == np.int64 and list(arr) == [0, 1, 2]
+
+
 def test_new_neural_network_init_matches_reference_rule():
     """W = noise * U(-1,1)(1+i) + initial_value on the M // N diagonal blocks."""
     psi = R.new_neural_network(4, 8, noise=0.0, gpu=True)
@@ -196,3 +251,46 @@ def test_new_neural_network_init_matches_reference_rule():
     iv = 0.01 + 1j * np.pi / 4
     assert np.allclose(W[:, :4], iv * np.eye(4)) and np.allclose(W[:, 4:], iv * np.eye(4))
     assert np.allclose(psi.b, 0)
+
+
+def test_made_training_launch_extents_without_gpu():
+    """Every kernel launch of a proposal-network training step stays within
+    its buffer (host-side extent check of the launch arguments against the
+    allocations), over network shapes up to the 100 -> 256 -> 100 learned
+    proposal and batches around the 64-row padding and split-K boundaries."""
+    lib = ctypes.CDLL(LIB)
+    lib.rbmhip_made_check_launches.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+    lib.rbmhip_last_error.restype = ctypes.c_char_p
+    for N in (1, 2, 7, 20, 63, 64, 65, 100, 128):
+        for H in (1, 16, 64, 65, 256, 512):
+            for batch in (1, 63, 64, 65, 128, 1000, 4096, 4097, 65536):
+                rc = lib.rbmhip_made_check_launches(N, H, 1_007_616, batch)
+                assert rc == 0, (N, H, batch, lib.rbmhip_last_error())
+    assert lib.rbmhip_made_check_launches(0, 256, 10, 10) == -1
+
+
+def test_deep_connections_2d_layouts():
+    """2-D connection tables (pyRBMonGPU/new_neural_network.py:128-145):
+    a lattice of width 1 reduces to the 1-D rule; every unit reads a c0 x c1
+    window of distinct inputs; with stride 1 the window of unit (j0 + 1, j1)
+    is that of (j0, j1) moved down one row (periodically)."""
+    from rbmongpu_amd.new_neural_network import deep_connections
+    one_d = deep_connections(8, [8, 4], [3, 8])
+    assert all(np.array_equal(a, b) for a, b in zip(deep_connections((8, 1), [(8, 1), (4, 1)], [(3, 1), (8, 1)]), one_d))
+    assert all(np.array_equal(a, b) for a, b in zip(deep_connections((1, 8), [(1, 8), (1, 4)], [(1, 3), (1, 8)]), one_d))
+    (t,) = deep_connections((6, 6), [(6, 6)], [(3, 2)])
+    assert t.shape == (6, 36)
+    for unit in range(36):
+        assert len(set(t[:, unit])) == 6
+    for j0 in range(6):
+        for j1 in range(6):
+            here = {(int(x) // 6, int(x) % 6) for x in t[:, j0 * 6 + j1]}
+            below = {(int(x) // 6, int(x) % 6) for x in t[:, ((j0 + 1) % 6) * 6 + j1]}
+            assert below == {((r + 1) % 6, c) for r, c in here}
+    # a layer with fewer units strides over the inputs: 3x3 units over 6x6 inputs
+    (t2,) = deep_connections((6, 6), [(3, 3)], [(2, 2)])
+    assert sorted({int(x) for x in t2.ravel()}) == list(range(36))
+    deep = R.new_deep_neural_network((4, 4), [(4, 4), (2, 2)], [(2, 2), (4, 4)], gpu=True,
+                                     rng=np.random.default_rng(3))
+    assert deep.N == 16
+    assert [w.shape for w in deep.W] == [(4, 16), (16, 4)]
