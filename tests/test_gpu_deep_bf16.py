@@ -183,7 +183,7 @@ def test_bf16_mc_energy_within_three_standard_errors():
 def test_bf16_deep_tier_shape_sampler_agrees_with_fp32():
     """The 40 -> 160 -> 80 network runs a compile-time-shaped, software-pipelined
     forward pass: its chain's energy agrees with the complex64 kernels' chain
-    on the same model within 4 combined standard errors (the models differ by
+    on the same model within 3 combined standard errors (the models differ by
     the bfloat16 rounding, well below the statistical error here)."""
     N, M, C = 40, [160, 80], [40, 160]
     biases, conns, weights, final = layers(N, M, C, seed=44, re=0.25, im=0.1, bias=0.2)
@@ -199,8 +199,9 @@ def test_bf16_deep_tier_shape_sampler_agrees_with_fp32():
         per_chain = el.real.reshape(per, chains).mean(axis=0)
         res.append((E, per_chain.std(ddof=1) / np.sqrt(chains), mc.acceptance_rate))
     (e16, s16, a16), (e32, s32, a32) = res
-    print(f"  bf16 {e16:.5f} +- {s16:.5f} (acc {a16:.3f}), fp32 {e32:.5f} +- {s32:.5f} (acc {a32:.3f})", flush=True)
-    assert abs(e16 - e32) < 4 * np.hypot(s16, s32)
+    print(f"  bf16 {e16:.5f} +- {s16:.5f} (acc {a16:.3f}), fp32 {e32:.5f} +- {s32:.5f} (acc {a32:.3f}): "
+          f"{abs(e16 - e32) / np.hypot(s16, s32):.2f} combined standard errors apart", flush=True)
+    assert abs(e16 - e32) < 3 * np.hypot(s16, s32)
     assert abs(a16 - a32) < 0.02
 
 
@@ -256,11 +257,16 @@ def test_bf16_hs_distance_and_learned_ensemble():
     psi.normalize(ex)
     ev = R.ExpectationValue(True)
     E_ex = ev(psi, R.Operator(*tab, True), ex).real
+    # an untrained proposal network: independent-MH acceptance is low, so the
+    # chains thermalise over 300 proposals (the per-chain standard error
+    # covers the autocorrelation of the kept samples, not a start-up bias)
     chains, per = 512, 60
     md = R.MADE(N, 32, seed=2)
-    ens = R.LearnedMonteCarloLoop(md, chains * per, 5, chains, True, seed=7)
+    ens = R.LearnedMonteCarloLoop(md, chains * per, 300, chains, True, seed=7)
     E = ev(psi, R.Operator(*tab, True), ens).real
     _, _, el = ens.last_samples()
     per_chain = el.real.reshape(per, chains).mean(axis=0)
     se = per_chain.std(ddof=1) / np.sqrt(chains)
-    assert abs(E - E_ex) < 4 * se, (E, E_ex, se)
+    print(f"  learned-proposal chain on the bfloat16 model: E {E:.6f} vs exact {E_ex:.6f}, "
+          f"{abs(E - E_ex) / se:.2f} standard errors, acceptance {ens.acceptance_rate:.3f}", flush=True)
+    assert abs(E - E_ex) < 3 * se, (E, E_ex, se)

@@ -2,9 +2,12 @@
 gradient, `corrected` and `difference` against the FP64 oracle, on the exact
 basis and on Monte Carlo chains that the complex128 kernels reproduce bit for
 bit (tolerance 1e-10 there).  complex64 kernels feed FP64 sums here, so their
-error is the FP32 rounding of log psi differences and tanh records; the
-E_k estimator is not centred and loses up to two further digits to
-cancellation (1e-4)."""
+error is the FP32 rounding of log psi differences and tanh records, held to
+1e-5.  The E_k estimator is the difference of three sums,
+<E_loc conj O_k> + conj<E_k(s')> - 2 <E_loc> conj<O_k> (ExpectationValue.cu:
+246-250), which is not centred: its complex64 error is held to 1e-5 of the
+three sums' magnitudes, computed from the oracle's E_loc and O_k on the exact
+basis (a dropped or doubled term exceeds that by orders of magnitude)."""
 import numpy as np
 import pytest
 
@@ -14,7 +17,7 @@ import rbmongpu_amd as R
 from rbmongpu_amd import hamiltonians as HAM
 
 pytestmark = pytest.mark.gpu
-TOL = {64: 1e-10, 32: 1e-4}
+TOL = {64: 1e-10, 32: 1e-5}
 
 
 def gpu_psi(b, W, a=None, precision=64, prefactor=1.0):
@@ -32,6 +35,19 @@ def non_hermitian(N, seed):
     H = dense_hamiltonian(N, *tab)
     assert not np.allclose(H, H.conj().T)
     return tab
+
+
+def ek_term_scale(ref, op, N, g_ref):
+    """|<E conj O>| + |conj<E_k>| + 2 |<E>| |<O>| per parameter on the exact
+    basis: the magnitudes of the estimator's three sums (averages are weighted
+    sums over the 2^N states divided by 2^N, as ExactSummation's).  <E_k> is
+    what g_ref leaves after the other two."""
+    spins, _, el, w = O.records(ref, op, O.Exact(N))
+    w = w / len(w)
+    Ok = np.array([O.o_k(ref, s) for s in spins])
+    e_o = (w * el) @ Ok.conj()
+    two_e_o = 2 * (w @ el) * (w @ Ok).conj()
+    return np.abs(e_o) + np.abs(g_ref - e_o + two_e_o) + np.abs(two_e_o)
 
 
 @pytest.mark.parametrize("precision", [64, 32])
@@ -54,8 +70,11 @@ def test_ek_gradient_exact(precision, model):
     for tab, est in ((HAM.tfim_1d(N, 1.0, 0.7), "E_k"), (non_hermitian(N, 3), "auto")):
         g, E = ev.gradient(psi, R.Operator(*tab, True), ex, estimator=est)
         gr, Er = O.gradient(ref, O.Operator(*tab), O.Exact(N), mode=0)
+        scale = ek_term_scale(ref, O.Operator(*tab), N, gr)
+        err = np.linalg.norm(g - gr) / np.linalg.norm(scale)
+        print(f"  {est}: |g - g_ref| / |terms| = {err:.2e} (relative to |g_ref|: {rel_err(g, gr):.2e})", flush=True)
         assert abs(E - Er) < TOL[precision] * abs(Er)
-        assert rel_err(g, gr) < TOL[precision], (est, rel_err(g, gr))
+        assert err < TOL[precision], (est, err)
     # a non-Hermitian operator cannot use the Hermitian estimator
     with pytest.raises(ValueError):
         ev.gradient(psi, R.Operator(*non_hermitian(N, 3), True), ex, estimator="hermitian")
