@@ -51,6 +51,15 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::rbmhip::Error(RBMHIP_ERR_ARG, std::string(msg)); \
     } while (0)
 
+// A roctx range around a compute entry point, so that a profiler's trace
+// (rocprofv3 --marker-trace) groups the kernels of each call under its name.
+struct ApiRange {
+    explicit ApiRange(const char* name) { roctxRangePush(name); }
+    ~ApiRange() { roctxRangePop(); }
+    ApiRange(const ApiRange&) = delete;
+    ApiRange& operator=(const ApiRange&) = delete;
+};
+
 template <class F>
 int guard(F&& f) {
     try {
@@ -1866,6 +1875,7 @@ int rbmhip_ens_destroy(rbmhip_ens* ens) {
 
 // ---- batched evaluation ---------------------------------------------------------
 int rbmhip_log_psi(rbmhip_psi* psi, const uint64_t* spins, int64_t n, double* out) {
+    const ApiRange range_("rbmhip_log_psi");
     return guard([&] {
         check_psi(psi);
         RBM_REQUIRE(n >= 0 && (n == 0 || (spins && out)), "bad arguments");
@@ -1885,6 +1895,7 @@ int rbmhip_log_psi(rbmhip_psi* psi, const uint64_t* spins, int64_t n, double* ou
 }
 
 int rbmhip_local_energy(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* spins, int64_t n, double* out) {
+    const ApiRange range_("rbmhip_local_energy");
     return guard([&] {
         check_psi(psi);
         RBM_REQUIRE(op, "null operator");
@@ -1928,6 +1939,7 @@ int rbmhip_local_energy_noalloc(rbmhip_psi* psi, rbmhip_op* op, const uint64_t* 
 }
 
 int rbmhip_o_k_vector(rbmhip_psi* psi, const uint64_t* spins, double* out) {
+    const ApiRange range_("rbmhip_o_k_vector");
     return guard([&] {
         check_psi(psi);
         RBM_REQUIRE(spins && out, "bad arguments");
@@ -1967,6 +1979,7 @@ int rbmhip_o_k_vector(rbmhip_psi* psi, const uint64_t* spins, double* out) {
 }
 
 int rbmhip_psi_vector(rbmhip_psi* psi, double* out) {
+    const ApiRange range_("rbmhip_psi_vector");
     return guard([&] {
         check_psi(psi);
         RBM_REQUIRE(out, "null output");
@@ -1988,6 +2001,7 @@ int rbmhip_psi_vector(rbmhip_psi* psi, double* out) {
 }
 
 int rbmhip_psi_norm(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
+    const ApiRange range_("rbmhip_psi_norm");
     return guard([&] {
         check_psi(psi);
         check_ens(ens);
@@ -2016,6 +2030,7 @@ int rbmhip_psi_norm(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
 
 // ---- network functions ------------------------------------------------------------
 int rbmhip_expectation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* out) {
+    const ApiRange range_("rbmhip_expectation");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(op && out, "bad arguments");
@@ -2032,6 +2047,7 @@ int rbmhip_expectation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* 
 }
 
 int rbmhip_expectation_list(rbmhip_psi* psi, rbmhip_op* const* ops, int n_ops, rbmhip_ens* ens, double* out) {
+    const ApiRange range_("rbmhip_expectation_list");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(n_ops >= 0 && (n_ops == 0 || (ops && out)), "bad arguments");
@@ -2061,6 +2077,7 @@ int rbmhip_expectation_list(rbmhip_psi* psi, rbmhip_op* const* ops, int n_ops, r
 }
 
 int rbmhip_fluctuation(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* fluctuation, double* expectation) {
+    const ApiRange range_("rbmhip_fluctuation");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(op && fluctuation && expectation, "bad arguments");
@@ -2178,15 +2195,18 @@ static int gradient_impl(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, int es
 extern "C" {
 
 int rbmhip_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad, double* expectation) {
+    const ApiRange range_("rbmhip_gradient");
     return gradient_impl(psi, op, ens, RBMHIP_GRAD_AUTO, grad, expectation);
 }
 
 int rbmhip_gradient_estimator(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, int estimator, double* grad,
                               double* expectation) {
+    const ApiRange range_("rbmhip_gradient_estimator");
     return gradient_impl(psi, op, ens, estimator, grad, expectation);
 }
 
 int rbmhip_fluctuation_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double* grad) {
+    const ApiRange range_("rbmhip_fluctuation_gradient");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(op && grad, "bad arguments");
@@ -2214,6 +2234,7 @@ int rbmhip_fluctuation_gradient(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens,
 }
 
 int rbmhip_corrected(rbmhip_psi* psi, rbmhip_op* op, rbmhip_op* op2, rbmhip_ens* ens, double* value, double* norm) {
+    const ApiRange range_("rbmhip_corrected");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(op && value && norm, "bad arguments");
@@ -2242,6 +2263,7 @@ int rbmhip_corrected(rbmhip_psi* psi, rbmhip_op* op, rbmhip_op* op2, rbmhip_ens*
 
 int rbmhip_difference(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* const* ops, int n_ops, rbmhip_ens* ens,
                       double* out) {
+    const ApiRange range_("rbmhip_difference");
     return guard([&] {
         check_psi(psi); check_psi(psi_prime); check_ens(ens);
         RBM_REQUIRE(ops && n_ops >= 1 && out, "bad arguments");
@@ -2297,6 +2319,7 @@ int rbmhip_difference(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* const* 
 }
 
 int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
+    const ApiRange range_("rbmhip_s_matrix");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(out, "null output");
@@ -2317,6 +2340,7 @@ int rbmhip_s_matrix(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
 }
 
 int rbmhip_s_matrix_centred(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
+    const ApiRange range_("rbmhip_s_matrix_centred");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(out, "null output");
@@ -2334,6 +2358,7 @@ int rbmhip_s_matrix_centred(rbmhip_psi* psi, rbmhip_ens* ens, double* out) {
 }
 
 int rbmhip_cholesky_solve(int n, const double* A, const double* b, double* x) {
+    const ApiRange range_("rbmhip_cholesky_solve");
     return guard([&] {
         RBM_REQUIRE(n > 0 && A && b && x, "bad arguments");
         Ctx& c = ctx();
@@ -2348,6 +2373,7 @@ int rbmhip_cholesky_solve(int n, const double* A, const double* b, double* x) {
 }
 
 int rbmhip_sr_update(rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* ens, double eps, double* update, double* expectation) {
+    const ApiRange range_("rbmhip_sr_update");
     return guard([&] {
         check_psi(psi); check_ens(ens);
         RBM_REQUIRE(op && update && expectation, "bad arguments");
@@ -2539,12 +2565,14 @@ static int hs_distance_impl(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* o
 
 int rbmhip_hs_distance(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
                        int want_grad, double nu, double* distance, double* grad) {
+    const ApiRange range_("rbmhip_hs_distance");
     return hs_distance_impl(psi, psi_prime, op, is_unitary, ens, want_grad, nu, nullptr, distance, grad);
 }
 
 int rbmhip_hs_distance_axis(rbmhip_psi* psi, rbmhip_psi* psi_prime, rbmhip_op* op, int is_unitary, rbmhip_ens* ens,
                             int want_grad, double nu, const double* alpha, const double* beta, const double* alpha_prime,
                             const double* beta_prime, double* distance, double* grad) {
+    const ApiRange range_("rbmhip_hs_distance_axis");
     int rc = guard([&] {
         check_psi(psi);
         RBM_REQUIRE(alpha && beta && alpha_prime && beta_prime, "null quantum-axis angles");
@@ -2666,6 +2694,7 @@ int rbmhip_made_reset_optimizer(rbmhip_made* made) {
 }
 
 int rbmhip_made_loss_grad(rbmhip_made* made, const uint64_t* spins, int64_t n, double* loss, float* grad) {
+    const ApiRange range_("rbmhip_made_loss_grad");
     return guard([&] {
         RBM_REQUIRE(made && spins && n > 0, "bad arguments");
         RBM_REQUIRE(n <= (1 << 24), "batch too large");
@@ -2697,6 +2726,7 @@ int rbmhip_made_loss_grad(rbmhip_made* made, const uint64_t* spins, int64_t n, d
 
 int rbmhip_made_train(rbmhip_made* made, const uint64_t* spins, int64_t n, int steps, int batch, double lr,
                       uint64_t seed, double* losses) {
+    const ApiRange range_("rbmhip_made_train");
     return guard([&] {
         RBM_REQUIRE(made && spins && n > 0 && steps >= 0 && batch > 0, "bad arguments");
         RBM_REQUIRE(batch <= (1 << 20), "batch too large");
@@ -2741,6 +2771,7 @@ int rbmhip_made_train(rbmhip_made* made, const uint64_t* spins, int64_t n, int s
 }
 
 int rbmhip_made_log_q(rbmhip_made* made, const uint64_t* spins, int64_t n, double* out) {
+    const ApiRange range_("rbmhip_made_log_q");
     return guard([&] {
         RBM_REQUIRE(made && n >= 0 && (n == 0 || (spins && out)), "bad arguments");
         if (n == 0) return;
@@ -2756,6 +2787,7 @@ int rbmhip_made_log_q(rbmhip_made* made, const uint64_t* spins, int64_t n, doubl
 }
 
 int rbmhip_made_sample(rbmhip_made* made, int64_t n, uint64_t seed, uint64_t* spins, double* log_q) {
+    const ApiRange range_("rbmhip_made_sample");
     return guard([&] {
         RBM_REQUIRE(made && n >= 0 && (n == 0 || spins), "bad arguments");
         if (n == 0) return;
