@@ -35,9 +35,23 @@ RBM_DI RowLds<T> row_lds(unsigned char* smem, int row, int width, int units) {
     return g;
 }
 
+// my_logcosh(a) as a value (not a difference): the cancellation-free form
+// plus its constant, so that a small activation (~a^2 / 2) keeps its relative
+// accuracy.  The gradient records (activations as inputs of the next layer's
+// weight derivatives, and the final-weight derivatives) use it; log psi
+// differences of E_loc and the sampler keep the direct form, whose rounding
+// is common to both sides of a difference.
+template <typename T>
+RBM_DI cx<T> logcosh_value(cx<T> a) {
+    const cx<T> s = logcosh_shifted(a);
+    return mk(s.re + T(LC_L0), s.im);
+}
+
 // forward_pass (PsiDeep.hpp:89-133) on the configuration rotated left by
 // `shift` (Spins.h:280-285: new bit k = old bit (k - shift) mod N).  Returns
-// this lane's share of sum_j x_j final_j; stores pre-activations if asked.
+// this lane's share of sum_j x_j final_j; stores pre-activations if asked
+// (the gradient records' forward pass, which then evaluates the activations
+// with logcosh_value).
 template <typename T>
 RBM_DI cx<T> deep_forward(const DeepDev<T>& md, uint64_t s0, uint64_t s1, int shift, const RowLds<T>& g,
                           bool store_ang, int lane) {
@@ -62,7 +76,7 @@ RBM_DI cx<T> deep_forward(const DeepDev<T>& md, uint64_t s0, uint64_t s1, int sh
             for (int i = 0; i < C; i++) a = a + w[i * size + j] * x[cn[i * size + j]];
             a = a + b[j];
             if (store_ang) g.ang[abegin + j] = a;
-            y[j] = logcosh(a);
+            y[j] = store_ang ? logcosh_value(a) : logcosh(a);
         }
         __syncthreads();
         cx<T>* t = x; x = y; y = t;
@@ -144,14 +158,14 @@ RBM_DI void deep_o_k(const DeepDev<T>& md, uint64_t s0, uint64_t s1, const RowLd
             Orow[bp[l] + j] = D[j];
             for (int c = 0; c < C; c++) {
                 const uint32_t u = cn[c * size + j];
-                const cx<T> xp = l == 0 ? mk(spin_of<T>(s0, s1, (int)u), T(0)) : logcosh(g.ang[ab[l - 1] + u]);
+                const cx<T> xp = l == 0 ? mk(spin_of<T>(s0, s1, (int)u), T(0)) : logcosh_value(g.ang[ab[l - 1] + u]);
                 Orow[bp[l] + size + (int64_t)c * size + j] = D[j] * xp;
             }
         }
     }
     const int last = md.size[L - 1];
     for (int j = lane; j < last; j += 16) {
-        const cx<T> x = logcosh(g.ang[ab[L - 1] + j]);
+        const cx<T> x = logcosh_value(g.ang[ab[L - 1] + j]);
         Orow[p + j] = md.ref_compat_final ? lp * x : x;
     }
     __syncthreads();

@@ -150,3 +150,82 @@ def test_tfim10x10_local_energies(scale):
     lpr = O.log_psi(O.RBM(b, W), spins)
     good = check_local_energies(f"10x10 TFIM M=200 scale {scale}", el, El, lp, lpr, b, W, spins_pm(spins, N), tab)
     assert abs(E - El.mean()) < BAR32 * abs(El.mean()) + np.abs(el[~good] - El[~good]).sum() / len(El)
+
+
+# ------------------------------------------------------------------ J1-J2 N = 40, PsiDeep 40 -> 160 -> 80
+def j1j2_deep_tier_model(scale=1.0, seed=1237):
+    """The deep-network tier's model: dense 40 -> 160 -> 80 connections, complex
+    weights N(0, (0.05 scale)^2 / fan_in) in real and imaginary part, zero
+    biases, final weights 1."""
+    from rbmongpu_amd.new_neural_network import deep_connections
+    N, M, C = 40, [160, 80], [40, 160]
+    rng = np.random.default_rng(seed)
+    conns = [np.asarray(c, np.uint32) for c in deep_connections(N, M, C)]
+    weights = [(0.05 * scale / np.sqrt(c)) * (rng.normal(size=(c, m)) + 1j * rng.normal(size=(c, m)))
+               for c, m in zip(C, M)]
+    biases = [np.zeros(m, complex) for m in M]
+    final = np.ones(M[-1], complex)
+    return N, biases, conns, weights, final
+
+
+def random_words40(n, seed):
+    rng = np.random.default_rng(seed)
+    w = np.zeros((n, 2), np.uint64)
+    w[:, 0] = rng.integers(0, 2 ** 40, size=n, dtype=np.uint64)
+    return w
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_j1j2_deep_tier_shape_against_oracle(precision):
+    """log psi, E_loc and O_k of the generic complex64 / complex128 PsiDeep
+    kernels at the deep-network tier's shape (N = 40, 40 -> 160 -> 80, 240
+    Pauli strings) on 512 random configurations, against the FP64 oracle
+    (PsiDeep.hpp:89-169, 214-293; Operator.hpp:90-115).  At the specified
+    initialisation log psi is ~1e-5, so its error is held in absolute terms
+    (the relative error of psi): 1e-5 in complex64, 1e-12 in complex128."""
+    N, biases, conns, weights, final = j1j2_deep_tier_model()
+    tab = HAM.j1j2_chain(N)
+    assert len(tab[0]) == 240
+    psi = R.PsiDeep(np.zeros(N), np.zeros(N), biases, conns, weights, final, 1.0, False, True, precision=precision)
+    ref = O.Deep(N, biases, conns, weights, final)
+    words = random_words40(512, seed=40)
+    lp = R.log_psi_batch(psi, words)
+    lpr = O.log_psi(ref, words)
+    el = R.local_energy_batch(psi, R.Operator(*tab, True), words)
+    elr = O.local_energy(ref, O.Operator(*tab), words)
+    e_lp, e_el = np.abs(lp - lpr).max(), rel_err(el, elr)
+    oks = [(R.psi_O_k_vector(psi, R.Spins(int(w), N)), O.o_k(ref, np.array([w, 0], np.uint64))[2 * N:])
+           for w in words[:8, 0]]
+    e_ok = max(rel_err(a, b) for a, b in oks)
+    print(f"[J1-J2 deep tier, precision {precision}] |d log psi| max {e_lp:.2e} (max |log psi| "
+          f"{np.abs(lpr).max():.2e}); E_loc rel {e_el:.2e}; O_k rel {e_ok:.2e}", flush=True)
+    bar = 1e-10 if precision == 64 else 1e-5
+    assert e_lp < (1e-12 if precision == 64 else 1e-5)
+    assert e_el < bar
+    assert e_ok < bar
+
+
+def test_j1j2_deep_tier_shape_fp64_chains_and_gradient():
+    """complex128 Metropolis chains of the deep-network tier's model (weights
+    x 12: acceptance ~0.5) replay the oracle's chains bit for bit, and the
+    Monte Carlo gradient matches the oracle's estimator on them to 1e-10 (the
+    centred estimator 2<E conj O> - 2<E> conj<O> that both use for a
+    Hermitian operator)."""
+    N, biases, conns, weights, final = j1j2_deep_tier_model(scale=12.0)
+    tab = HAM.j1j2_chain(N)
+    psi = R.PsiDeep(np.zeros(N), np.zeros(N), biases, conns, weights, final, 1.0, False, True, precision=64)
+    ref = O.Deep(N, biases, conns, weights, final)
+    chains, per, therm = 32, 4, 5
+    mc = R.MonteCarloLoop(chains * per, 1, therm, chains, True, seed=19)
+    mc_g = R.MonteCarloLoop(mc)
+    R.ExpectationValue(True)(psi, R.Operator(*tab, True), mc)
+    spins, lp, el = mc.last_samples()
+    rs, rlp, rel, _ = O.records(ref, O.Operator(*tab), O.MonteCarlo(N, chains * per, 1, therm, chains, seed=19))
+    assert np.array_equal(spins, rs)
+    assert rel_err(el, rel) < 1e-10
+    print(f"[J1-J2 deep tier chains] acceptance {mc.acceptance_rate:.3f}", flush=True)
+    assert 0.05 < mc.acceptance_rate < 0.95
+    g, E = R.ExpectationValue(True).gradient(psi, R.Operator(*tab, True), mc_g)
+    gr, Er = O.gradient(ref, O.Operator(*tab), O.MonteCarlo(N, chains * per, 1, therm, chains, seed=19), mode=1)
+    assert abs(E - Er) < 1e-10 * abs(Er)
+    assert rel_err(g, gr) < 1e-10
