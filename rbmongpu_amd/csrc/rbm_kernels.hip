@@ -312,21 +312,31 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
         while (i < nk) {
             // proposals up to the next measurement point or the end of the block;
             // the W row of the next proposal is loaded while the current one is
-            // evaluated, alternating between two register sets
+            // evaluated, alternating between two register sets, and the draw
+            // after that is read one proposal ahead, so that no row load waits
+            // on the LDS read that gives its address (at one wavefront per
+            // SIMD nothing else would hide that latency)
             const int stop = next_measure > kb && next_measure - kb < (uint64_t)nk ? (int)(next_measure - kb) : nk;
             uint2 d = draws[i];
+            uint2 dn = draws[i + 1 < nk ? i + 1 : i];
             cx<T> wa[JPL], wb[JPL];
             load_row(wa, d.x);
             while (i < stop) {
-                uint2 dn = draws[i + 1 < nk ? i + 1 : i];
+                // (the scheduling barriers keep these loads ahead of the
+                // proposal instead of next to their first use)
                 load_row(wb, dn.x);
+                uint2 dnn = draws[i + 2 < nk ? i + 2 : nk - 1];
+                __builtin_amdgcn_sched_barrier(0);
                 propose(d, wa);
                 d = dn;
+                dn = dnn;
                 if (++i >= stop) break;
-                dn = draws[i + 1 < nk ? i + 1 : i];
                 load_row(wa, dn.x);
+                dnn = draws[i + 2 < nk ? i + 2 : nk - 1];
+                __builtin_amdgcn_sched_barrier(0);
                 propose(d, wb);
                 d = dn;
+                dn = dnn;
                 ++i;
             }
             if (NARROW) s0 = w32;
