@@ -1445,10 +1445,11 @@ static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, co
     }
     cx<float>* dob = reinterpret_cast<cx<float>*>(c.buf("s_obar", (size_t)P * sizeof(cx<float>)));
     c.h2d(dob, obar.data(), obar.size() * sizeof(cx<float>));
-    const int ld2 = (2 * P + 127) / 128 * 128;   // k_gram_lower works on 128 x 128 tiles
-    int64_t nb = ((int64_t)(512ll << 20) / ((int64_t)ld2 * 4)) / 64 * 64;
+    const int ld2 = (2 * P + 127) / 128 * 128;   // k_gram_lower_bf3 works on 128 x 128 tiles
+    // rows of a batch: 6 bytes per value (three bfloat16 parts), at most 768 MiB
+    int64_t nb = ((int64_t)(768ll << 20) / ((int64_t)ld2 * 6)) / 64 * 64;
     nb = std::max<int64_t>(64, std::min<int64_t>(nb, (s.n + 63) / 64 * 64));
-    float* O = reinterpret_cast<float*>(c.buf("s_rows", (size_t)nb * ld2 * sizeof(float)));
+    uint32_t* O = reinterpret_cast<uint32_t*>(c.buf("s_rows", (size_t)nb * ld2 * 6));
     float* G = reinterpret_cast<float*>(c.buf("s_gram", (size_t)ld2 * ld2 * sizeof(float)));
     double* Gd = reinterpret_cast<double*>(c.buf("s_gram64", (size_t)ld2 * ld2 * sizeof(double)));
     c.launch("s_matrix", [&] { return launch_zero((int64_t)2 * ld2 * ld2, reinterpret_cast<float*>(Gd), c.stream); });
@@ -1456,7 +1457,7 @@ static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, co
     for (int64_t s0 = 0; s0 < s.n; s0 += nb) {
         const int64_t cur = std::min<int64_t>(nb, s.n - s0), curp = (cur + 63) / 64 * 64;
         c.launch("s_matrix", [&] {
-            return launch_o_realified<T>(s0, cur, curp, N, M, P, ld2, s.tanh, s.spins, w, dob, O, c.stream);
+            return launch_o_split3<T>(s0, cur, curp, N, M, P, ld2, s.tanh, s.spins, w, dob, O, c.stream);
         });
         c.launch("s_gram", [&] { return launch_gram_lower(ld2, (int)curp, O, G, c.stream); });
         c.launch("s_matrix", [&] { return launch_accum_f64((int64_t)ld2 * ld2, G, Gd, c.stream); });

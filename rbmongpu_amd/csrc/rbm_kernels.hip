@@ -1589,43 +1589,66 @@ hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_
 // ---------------------------------------------------------------------------
 // Centred, weighted S-matrix S_kl = (1/n) sum_s w (O_k - <O_k>)^* (O_l - <O_l>)
 // over the RBM parameters with an O_k (b, W, visible a; order of `params`
-// after alpha and beta).  Rows of a sample batch are materialised "realified"
-// as FP32 [Re O (P) | Im O (P) | 0 padding] scaled by sqrt(w), so that one
-// real GEMM G = O^T O gives S_kl = G[k][l] + G[P+k][P+l] + i (G[k][P+l] - G[P+k][l]).
+// after alpha and beta).  The rows of a sample batch are materialised
+// "realified", [Re O (P) | Im O (P) | 0 padding] scaled by sqrt(w), so that one
+// real product G = O^T O gives S_kl = G[k][l] + G[P+k][P+l] + i (G[k][P+l] - G[P+k][l]).
 // O_k(s): b_j -> T_j, W_ij -> s_i T_j, a_i -> s_i  (Psi.hpp:165-202).
+// Each FP32 value is split exactly into three bfloat16 parts here, once, for
+// the Gram kernel's matrix cores, and stored transposed per block of 32
+// samples: part q of column c and sample 32 b + k at
+// out[((b * 3 + q) * ld2 + c) * 32 + k].  A thread forms two neighbouring
+// samples of one column; 16 threads write a column's 32 samples (64 bytes per
+// part) together.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_o_realified(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2,
-                                                     const void* __restrict__ Trec, const uint64_t* __restrict__ spins,
-                                                     const cx<double>* __restrict__ w, const cx<float>* __restrict__ obar,
-                                                     float* __restrict__ out) {
-    const int64_t total = nbp * ld2;
+RBM_DI float o_realified_value(int64_t s, int c, int N, int M, int P, const cx<T>* __restrict__ Tr,
+                               const uint64_t* __restrict__ spins, const cx<double>* __restrict__ w,
+                               const cx<float>* __restrict__ obar) {
+    const int k = c < P ? c : c - P;
+    const uint64_t x0 = spins[2 * s], x1 = spins[2 * s + 1];
+    cx<float> o;
+    if (k < M) {
+        const cx<T> tj = Tr[s * M + k];
+        o = mk((float)tj.re, (float)tj.im);
+    } else if (k < M + N * M) {
+        const int i = (k - M) / M, j = (k - M) % M;
+        const cx<T> tj = Tr[s * M + j];
+        const float si = spin_of<float>(x0, x1, i);
+        o = mk(si * (float)tj.re, si * (float)tj.im);
+    } else {
+        o = mk(spin_of<float>(x0, x1, k - M - N * M), 0.0f);
+    }
+    const float sw = (float)sqrt(w[s].re);
+    const cx<float> d = o - obar[k];
+    return sw * (c < P ? d.re : d.im);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_o_split3(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2,
+                                                  const void* __restrict__ Trec, const uint64_t* __restrict__ spins,
+                                                  const cx<double>* __restrict__ w, const cx<float>* __restrict__ obar,
+                                                  uint32_t* __restrict__ out) {
+    // one work item per (block b, column c, sample pair kp): 16 pairs per column
+    const int64_t total = (nbp / 32) * (int64_t)ld2 * 16;
     const cx<T>* Tr = reinterpret_cast<const cx<T>*>(Trec);
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = t / ld2;
-        const int c = (int)(t - r * ld2);
-        float v = 0.0f;
-        if (r < nb && c < 2 * P) {
-            const int k = c < P ? c : c - P;
-            const int64_t s = s0 + r;
-            const uint64_t x0 = spins[2 * s], x1 = spins[2 * s + 1];
-            cx<float> o;
-            if (k < M) {
-                const cx<T> tj = Tr[s * M + k];
-                o = mk((float)tj.re, (float)tj.im);
-            } else if (k < M + N * M) {
-                const int i = (k - M) / M, j = (k - M) % M;
-                const cx<T> tj = Tr[s * M + j];
-                const float si = spin_of<float>(x0, x1, i);
-                o = mk(si * (float)tj.re, si * (float)tj.im);
-            } else {
-                o = mk(spin_of<float>(x0, x1, k - M - N * M), 0.0f);
-            }
-            const float sw = (float)sqrt(w[s].re);
-            const cx<float> d = o - obar[k];
-            v = sw * (c < P ? d.re : d.im);
+        const int kp = (int)(t & 15);
+        const int64_t bc = t >> 4;
+        const int c = (int)(bc % ld2);
+        const int64_t blk = bc / ld2;
+        const int64_t r = blk * 32 + 2 * kp;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (c < 2 * P) {
+            if (r < nb) v0 = o_realified_value<T>(s0 + r, c, N, M, P, Tr, spins, w, obar);
+            if (r + 1 < nb) v1 = o_realified_value<T>(s0 + r + 1, c, N, M, P, Tr, spins, w, obar);
         }
-        out[t] = v;
+        uint32_t hi, mid, lo;
+        split3(v0, v1, hi, mid, lo);
+        // 32-bit words: ((blk * 3 + q) * ld2 + c) * 16 + kp
+        const int64_t base = (blk * 3 * ld2 + c) * 16 + kp;
+        out[base] = hi;
+        out[base + (int64_t)ld2 * 16] = mid;
+        out[base + (int64_t)ld2 * 32] = lo;
     }
 }
 
@@ -1651,12 +1674,13 @@ __global__ __launch_bounds__(256) void k_s_assemble(int P, int ld2, double inv, 
 }
 
 template <typename T>
-hipError_t launch_o_realified(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2, const void* Trec,
-                              const uint64_t* spins, const cx<double>* w, const cx<float>* obar, float* out,
-                              hipStream_t stream) {
-    const int64_t total = nbp * ld2;
-    const unsigned blocks = (unsigned)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
-    hipLaunchKernelGGL((k_o_realified<T>), dim3(blocks), dim3(256), 0, stream, s0, nb, nbp, N, M, P, ld2, Trec, spins, w,
+hipError_t launch_o_split3(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2, const void* Trec,
+                           const uint64_t* spins, const cx<double>* w, const cx<float>* obar, uint32_t* out,
+                           hipStream_t stream) {
+    if (nbp % 32 || nb > nbp || ld2 < 2 * P) return hipErrorInvalidValue;
+    const int64_t total = (nbp / 32) * (int64_t)ld2 * 16;
+    const unsigned blocks = (unsigned)((total + 255) / 256 < 16384 ? (total + 255) / 256 : 16384);
+    hipLaunchKernelGGL((k_o_split3<T>), dim3(blocks), dim3(256), 0, stream, s0, nb, nbp, N, M, P, ld2, Trec, spins, w,
                        obar, out);
     return hipGetLastError();
 }
@@ -1674,10 +1698,10 @@ hipError_t launch_s_assemble(int P, int ld2, double inv, double eps, const doubl
     return hipGetLastError();
 }
 
-template hipError_t launch_o_realified<float>(int64_t, int64_t, int64_t, int, int, int, int, const void*, const uint64_t*,
-                                              const cx<double>*, const cx<float>*, float*, hipStream_t);
-template hipError_t launch_o_realified<double>(int64_t, int64_t, int64_t, int, int, int, int, const void*,
-                                               const uint64_t*, const cx<double>*, const cx<float>*, float*, hipStream_t);
+template hipError_t launch_o_split3<float>(int64_t, int64_t, int64_t, int, int, int, int, const void*, const uint64_t*,
+                                           const cx<double>*, const cx<float>*, uint32_t*, hipStream_t);
+template hipError_t launch_o_split3<double>(int64_t, int64_t, int64_t, int, int, int, int, const void*, const uint64_t*,
+                                            const cx<double>*, const cx<float>*, uint32_t*, hipStream_t);
 
 hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, int deriv, hipStream_t stream) {
     int64_t g = (n + 255) / 256;
