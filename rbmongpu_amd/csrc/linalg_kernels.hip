@@ -218,7 +218,9 @@ RBM_DI void tile_tn(const float* __restrict__ X, int ldx, int m0, const float* _
 // bfloat16, so that the 8-byte stores and the fragments' 8-byte loads spread
 // over the banks, while the next block is loaded into registers.  Within a
 // stage the six products of each output tile are issued across the four row
-// tiles, so consecutive MFMAs accumulate into different registers.
+// tiles, so consecutive MFMAs accumulate into different registers.  The batch's FP32
+// tile is added into the FP64 accumulator C, so the FP32 error does not grow
+// with the number of batches.
 // ---------------------------------------------------------------------------
 constexpr int GB_K = 32, GB_LDK = 36;
 typedef float gbf32x4 __attribute__((ext_vector_type(4)));
@@ -229,7 +231,7 @@ typedef __bf16 gbbf16x8 __attribute__((ext_vector_type(8)));
 // column stripes in its L2 was measured slower: 176-187 ms against 169 ms of
 // Gram product per SR step of the 6x6 Heisenberg RBM.)
 __global__ __launch_bounds__(256) void k_gram_lower_bf3(int ld, int K, const uint4* __restrict__ A,
-                                                        float* __restrict__ C) {
+                                                        double* __restrict__ C) {
     __shared__ __align__(16) uint16_t S[2][3][GR_T][GB_LDK];
     const int t = blockIdx.x;
     int bm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(256) void k_gram_lower_bf3(int ld, int K, const uin
             for (int v = 0; v < 4; v++) {
                 const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
                 const int col = n0 + wn + 16 * j + (lane & 15);
-                C[(int64_t)row * ld + col] = acc[i][j][v];
+                C[(int64_t)row * ld + col] += (double)acc[i][j][v];
             }
 }
 
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int ldx, int ldy, int Kper, con
     tile_tn(X, ldx, bm * GR_T, Y, ldy, bn * GR_T, k0, k0 + Kper, out + (int64_t)blockIdx.z * ldx * ldy, ldy);
 }
 
-hipError_t launch_gram_lower(int ld, int K, const uint32_t* A, float* C, hipStream_t stream) {
+hipError_t launch_gram_lower(int ld, int K, const uint32_t* A, double* C, hipStream_t stream) {
     if (ld <= 0 || ld % GR_T || K <= 0 || K % GB_K) return hipErrorInvalidValue;
     const int T = ld / GR_T;
     hipLaunchKernelGGL(k_gram_lower_bf3, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, stream, ld, K,

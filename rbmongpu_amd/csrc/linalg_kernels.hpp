@@ -13,12 +13,12 @@ namespace rbmhip {
 hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream);
 // x <- (L L^H)^{-1} x with the factor from launch_cholesky
 hipError_t launch_cholesky_solve(int n, const cx<double>* L, cx<double>* x, hipStream_t stream);
-// C = A^T A on the lower 128 x 128 tiles of C (ld x ld) for a K x ld FP32
-// matrix A given split into three bfloat16 parts per 32-row block
-// ([block][part][column][32 rows], as launch_o_split3 writes it), on the
-// bfloat16 matrix cores with FP32 accuracy; ld a multiple of 128, K of 32.
-// Tiles above the diagonal are not written.
-hipError_t launch_gram_lower(int ld, int K, const uint32_t* A, float* C, hipStream_t stream);
+// C += A^T A on the lower 128 x 128 tiles of the FP64 matrix C (ld x ld) for
+// a K x ld FP32 matrix A given split into three bfloat16 parts per 32-row
+// block ([block][part][column][32 rows], as launch_o_split3 writes it), the
+// product formed on the bfloat16 matrix cores with FP32 accuracy; ld a
+// multiple of 128, K of 32.  Tiles above the diagonal are not touched.
+hipError_t launch_gram_lower(int ld, int K, const uint32_t* A, double* C, hipStream_t stream);
 // out[z] = X_z^T Y_z (ldx x ldy) for the row ranges z = 0 .. splits - 1 of
 // length K / splits of row-major K x ldx and K x ldy operands (same MFMA
 // tiles); ldx and ldy multiples of 128, K a multiple of 16 splits.

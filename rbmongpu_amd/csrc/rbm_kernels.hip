@@ -1652,14 +1652,9 @@ __global__ __launch_bounds__(256) void k_o_split3(int64_t s0, int64_t nb, int64_
     }
 }
 
-__global__ __launch_bounds__(256) void k_accum_f64(int64_t n, const float* __restrict__ g, double* __restrict__ acc) {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-        acc[t] += (double)g[t];
-}
-
 __global__ __launch_bounds__(256) void k_s_assemble(int P, int ld2, double inv, double eps, const double* __restrict__ G,
                                                     cx<double>* __restrict__ S) {
-    // G = O^T O is symmetric and only its lower 64 x 64 tiles are computed:
+    // G = O^T O is symmetric and only its lower 128 x 128 tiles are computed:
     // every entry read here has row >= column (G[k][P+l] is read as G[P+l][k]);
     // the upper triangle of S is the conjugate of the lower one.
     const int64_t total = (int64_t)P * P;
@@ -1682,12 +1677,6 @@ hipError_t launch_o_split3(int64_t s0, int64_t nb, int64_t nbp, int N, int M, in
     const unsigned blocks = (unsigned)((total + 255) / 256 < 16384 ? (total + 255) / 256 : 16384);
     hipLaunchKernelGGL((k_o_split3<T>), dim3(blocks), dim3(256), 0, stream, s0, nb, nbp, N, M, P, ld2, Trec, spins, w,
                        obar, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_accum_f64(int64_t n, const float* g, double* acc, hipStream_t stream) {
-    const unsigned blocks = (unsigned)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
-    hipLaunchKernelGGL(k_accum_f64, dim3(blocks), dim3(256), 0, stream, n, g, acc);
     return hipGetLastError();
 }
 

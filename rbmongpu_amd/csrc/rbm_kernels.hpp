@@ -155,13 +155,12 @@ hipError_t launch_activation(int64_t n, const cx<double>* z, cx<double>* out, in
 // [s0, s0 + nb) (rows nb .. nbp zero; columns [Re (P) | Im (P) | 0 .. ld2)),
 // split into three bfloat16 parts and stored per 32-sample block as
 // [block][part][column][32 samples] (nbp a multiple of 32; out holds
-// nbp * ld2 * 3 bfloat16); FP64 accumulation of an FP32 Gram block; assembly
-// of S (P x P complex, scaled by inv, eps added on the diagonal).
+// nbp * ld2 * 3 bfloat16); assembly of S (P x P complex, scaled by inv, eps
+// added on the diagonal) from the lower triangle of the FP64 Gram matrix.
 template <typename T>
 hipError_t launch_o_split3(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2, const void* Trec,
                            const uint64_t* spins, const cx<double>* w, const cx<float>* obar, uint32_t* out,
                            hipStream_t stream);
-hipError_t launch_accum_f64(int64_t n, const float* g, double* acc, hipStream_t stream);
 hipError_t launch_s_assemble(int P, int ld2, double inv, double eps, const double* G, cx<double>* S, hipStream_t stream);
 // log psi of n configurations of an FP32 RBM without visible bias (N <= 128):
 // theta = b + W^T s on bfloat16 MFMA with W split exactly into three bfloat16
