@@ -20,7 +20,8 @@ BUILD = os.path.join(HERE, "_build")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librbmhip.so")
 
-HIP_SOURCES = ["rbm_kernels.hip", "deep_kernels.hip", "deep_mfma.hip", "made_kernels.hip", "linalg_kernels.hip", "noalloc.hip", "capi.cpp"]
+HIP_SOURCES = ["rbm_kernels.hip", "deep_kernels.hip", "deep_mfma.hip", "made_kernels.hip", "linalg_kernels.hip",
+               "gram_kernels.hip", "noalloc.hip", "capi.cpp"]
 # Packed FP32 instructions are kept out of the generated code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1451,13 +1451,19 @@ static cx<double>* s_matrix_centred(Ctx& c, rbmhip_psi* psi, rbmhip_ens* ens, co
     nb = std::max<int64_t>(64, std::min<int64_t>(nb, (s.n + 63) / 64 * 64));
     uint32_t* O = reinterpret_cast<uint32_t*>(c.buf("s_rows", (size_t)nb * ld2 * 6));
     double* Gd = reinterpret_cast<double*>(c.buf("s_gram64", (size_t)ld2 * ld2 * sizeof(double)));
+    std::vector<int2> order;
+    gram_tile_order(ld2, order);
+    int2* dorder = reinterpret_cast<int2*>(c.buf("s_gram_order", order.size() * sizeof(int2)));
+    c.h2d(dorder, order.data(), order.size() * sizeof(int2));
     c.launch("s_matrix", [&] { return launch_zero((int64_t)2 * ld2 * ld2, reinterpret_cast<float*>(Gd), c.stream); });
     for (int64_t s0 = 0; s0 < s.n; s0 += nb) {
         const int64_t cur = std::min<int64_t>(nb, s.n - s0), curp = (cur + 63) / 64 * 64;
         c.launch("s_matrix", [&] {
             return launch_o_split3<T>(s0, cur, curp, N, M, P, ld2, s.tanh, s.spins, w, dob, O, c.stream);
         });
-        c.launch("s_gram", [&] { return launch_gram_lower(ld2, (int)curp, O, Gd, c.stream); });
+        c.launch("s_gram", [&] {
+            return launch_gram_lower(ld2, (int)curp, O, Gd, dorder, (int)order.size(), c.stream);
+        });
     }
     cx<double>* S = reinterpret_cast<cx<double>*>(c.buf("s_centred", (size_t)P * P * sizeof(cx<double>)));
     c.launch("s_matrix", [&] { return launch_s_assemble(P, ld2, 1.0 / n, eps, Gd, S, c.stream); });

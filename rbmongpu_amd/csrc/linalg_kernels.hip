@@ -200,120 +200,6 @@ RBM_DI void tile_tn(const float* __restrict__ X, int ldx, int m0, const float* _
             }
 }
 
-// ---------------------------------------------------------------------------
-// Gram product G = A^T A of the S-matrix rows (lower 128 x 128 tiles) on the
-// bfloat16 matrix cores with FP32 accuracy.  Every FP32 value of A arrives
-// split exactly into three bfloat16 parts, x = hi + mid + lo (hi = bf16(x),
-// mid = bf16(x - hi), lo = x - hi - mid), stored per block of 32 samples as
-// [block][part][column][32 samples].  A product x y is the sum of the six part
-// products hi hi, hi mid, mid hi, hi lo, lo hi, mid mid, which leaves out
-// terms below 2^-24 |x y|, and v_mfma_f32_16x16x32_bf16 sums them in FP32
-// like an FP32 GEMM; six bfloat16 products cost 6 / 16 of one FP32 product on
-// the matrix cores.
-// A 256-thread workgroup owns a 128 x 128 tile as 2 x 2 waves of 64 x 64, each
-// 4 x 4 MFMA tiles of 16 x 16 (lane l supplies A[l % 16][8 (l / 16) + q] and
-// B[8 (l / 16) + q][l % 16]; accumulator v of lane l is row 4 (l / 16) + v,
-// column l % 16).  K advances one 32-sample block per stage: the block's
-// three parts of both column stripes (48 KB) are copied into LDS rows of 36
-// bfloat16, so that the 8-byte stores and the fragments' 8-byte loads spread
-// over the banks, while the next block is loaded into registers.  Within a
-// stage the six products of each output tile are issued across the four row
-// tiles, so consecutive MFMAs accumulate into different registers.  The batch's FP32
-// tile is added into the FP64 accumulator C, so the FP32 error does not grow
-// with the number of batches.
-// ---------------------------------------------------------------------------
-constexpr int GB_K = 32, GB_LDK = 36;
-typedef float gbf32x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 gbbf16x8 __attribute__((ext_vector_type(8)));
-
-// Gram product, lower tiles: workgroup b covers tile (bm, bn), b = bm (bm + 1) / 2 + bn.
-// (Grouping the tiles so that the workgroups an XCD runs at once share
-// column stripes in its L2 was measured slower: 176-187 ms against 169 ms of
-// Gram product per SR step of the 6x6 Heisenberg RBM.)
-__global__ __launch_bounds__(256) void k_gram_lower_bf3(int ld, int K, const uint4* __restrict__ A,
-                                                        double* __restrict__ C) {
-    __shared__ __align__(16) uint16_t S[2][3][GR_T][GB_LDK];
-    const int t = blockIdx.x;
-    int bm = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((bm + 1) * (bm + 2) / 2 <= t) bm++;
-    while (bm * (bm + 1) / 2 > t) bm--;
-    const int bn = t - bm * (bm + 1) / 2;
-    const int m0 = bm * GR_T, n0 = bn * GR_T;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    // staging: 16-byte chunk u = tid + 256 h (h = 0, 1) of each operand and
-    // part is column u / 4, samples 8 (u % 4) .. + 7
-    uint4 x[2][3][2];
-    auto fetch = [&](int blk) {
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const uint4* base = A + ((int64_t)(blk * 3 + q) * ld) * 4;   // 4 chunks per column
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int u = tid + 256 * h;
-                x[0][q][h] = base[(int64_t)m0 * 4 + u];
-                x[1][q][h] = base[(int64_t)n0 * 4 + u];
-            }
-        }
-    };
-    auto frag = [&](int op, int q, int col) {
-        const int k = 8 * (lane >> 4);
-        const uint2 u0 = *reinterpret_cast<const uint2*>(&S[op][q][col][k]);
-        const uint2 u1 = *reinterpret_cast<const uint2*>(&S[op][q][col][k + 4]);
-        return __builtin_bit_cast(gbbf16x8, make_uint4(u0.x, u0.y, u1.x, u1.y));
-    };
-    gbf32x4 acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = gbf32x4{0.f, 0.f, 0.f, 0.f};
-    fetch(0);
-    for (int k0 = 0; k0 < K; k0 += GB_K) {
-        __syncthreads();
-#pragma unroll
-        for (int op = 0; op < 2; op++)
-#pragma unroll
-            for (int q = 0; q < 3; q++)
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int u = tid + 256 * h, col = u >> 2, k = 8 * (u & 3);
-                    const uint4 v = x[op][q][h];
-                    *reinterpret_cast<uint2*>(&S[op][q][col][k]) = make_uint2(v.x, v.y);
-                    *reinterpret_cast<uint2*>(&S[op][q][col][k + 4]) = make_uint2(v.z, v.w);
-                }
-        __syncthreads();
-        if (k0 + GB_K < K) fetch(k0 / GB_K + 1);
-        gbbf16x8 a[4][3];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int q = 0; q < 3; q++) a[i][q] = frag(0, q, wm + 16 * i + (lane & 15));
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            gbbf16x8 b[3];
-#pragma unroll
-            for (int q = 0; q < 3; q++) b[q] = frag(1, q, wn + 16 * j + (lane & 15));
-            // (part of a, part of b): the small products first
-            constexpr int PA[6] = {1, 0, 2, 0, 1, 0}, PB[6] = {1, 2, 0, 1, 0, 0};
-#pragma unroll
-            for (int p = 0; p < 6; p++)
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][PA[p]], b[PB[p]], acc[i][j], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
-                const int col = n0 + wn + 16 * j + (lane & 15);
-                C[(int64_t)row * ld + col] += (double)acc[i][j][v];
-            }
-}
-
 // Split-K product: workgroup (x, z) covers tile x of the (ldx / 128) x (ldy / 128)
 // grid over rows [z Kper, (z + 1) Kper); out[z] is ldx x ldy.
 __global__ __launch_bounds__(256) void k_gemm_tn(int ldx, int ldy, int Kper, const float* __restrict__ X,
@@ -322,14 +208,6 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int ldx, int ldy, int Kper, con
     const int bm = blockIdx.x / tn, bn = blockIdx.x % tn;
     const int k0 = (int)blockIdx.z * Kper;
     tile_tn(X, ldx, bm * GR_T, Y, ldy, bn * GR_T, k0, k0 + Kper, out + (int64_t)blockIdx.z * ldx * ldy, ldy);
-}
-
-hipError_t launch_gram_lower(int ld, int K, const uint32_t* A, double* C, hipStream_t stream) {
-    if (ld <= 0 || ld % GR_T || K <= 0 || K % GB_K) return hipErrorInvalidValue;
-    const int T = ld / GR_T;
-    hipLaunchKernelGGL(k_gram_lower_bf3, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, stream, ld, K,
-                       reinterpret_cast<const uint4*>(A), C);
-    return hipGetLastError();
 }
 
 // acc[i] += sum_z part[z n + i], the split partials added in FP64 in a fixed order

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "rbm_device.hpp"
 
 namespace rbmhip {
@@ -17,8 +19,14 @@ hipError_t launch_cholesky_solve(int n, const cx<double>* L, cx<double>* x, hipS
 // a K x ld FP32 matrix A given split into three bfloat16 parts per 32-row
 // block ([block][part][column][32 rows], as launch_o_split3 writes it), the
 // product formed on the bfloat16 matrix cores with FP32 accuracy; ld a
-// multiple of 128, K of 32.  Tiles above the diagonal are not touched.
-hipError_t launch_gram_lower(int ld, int K, const uint32_t* A, double* C, hipStream_t stream);
+// multiple of 128, K of 64.  Tiles above the diagonal are not touched.
+// order (n_order entries, from gram_tile_order) assigns the tiles to
+// workgroups.
+hipError_t launch_gram_lower(int ld, int K, const uint32_t* A, double* C, const int2* order, int n_order,
+                             hipStream_t stream);
+// Fills order with the workgroup-to-tile map of launch_gram_lower for ld
+// (entries (bm, bn) with bn <= bm, or (-1, -1)); returns the number of tiles.
+int gram_tile_order(int ld, std::vector<int2>& order);
 // out[z] = X_z^T Y_z (ldx x ldy) for the row ranges z = 0 .. splits - 1 of
 // length K / splits of row-major K x ldx and K x ldy operands (same MFMA
 // tiles); ldx and ldy multiples of 128, K a multiple of 16 splits.
