@@ -368,6 +368,12 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
 // index_base + idx masked to N bits (Spins(index, N), Spins.h:230-237).
 // Grid-strides over groups of 16 configurations per workgroup.
 // ---------------------------------------------------------------------------
+// Rows of W staged per step when W does not fit in LDS (FP32 models): the
+// angles of a group's 16 configurations add W rows from LDS instead of each
+// lane loading its units of every row from memory (100 dependent round trips
+// per configuration at N = 100).
+constexpr int EV_ROWS = 16;
+
 template <typename T, int JPL, bool WLDS>
 __global__ __launch_bounds__(256) void k_eval_rbm(RbmDev<T> md, OpDev op, int has_op, int64_t n,
                                                   const uint64_t* __restrict__ configs, uint64_t index_base,
@@ -383,6 +389,40 @@ __global__ __launch_bounds__(256) void k_eval_rbm(RbmDev<T> md, OpDev op, int ha
     }
     const int lane = threadIdx.x & 15;
     const int groups = blockDim.x >> 4;
+    if constexpr (!WLDS && sizeof(T) == 4) {
+        // theta = b + sum_i s_i W_i in the order of init_angles (the same FP32
+        // values), the rows staged EV_ROWS at a time; every row of the
+        // workgroup runs the same group loop, and rows past n evaluate the
+        // last configuration and write nothing
+        cx<T>* Wc = reinterpret_cast<cx<T>*>(smem_raw);
+        const int row = threadIdx.x >> 4;
+        for (int64_t g0 = (int64_t)blockIdx.x * groups; g0 < n; g0 += (int64_t)gridDim.x * groups) {
+            const int64_t idx = g0 + row;
+            const bool valid = idx < n;
+            const int64_t ci = valid ? idx : n - 1;
+            uint64_t s0, s1;
+            if (configs) { s0 = configs[2 * ci]; s1 = configs[2 * ci + 1]; }
+            else { s0 = index_base + (uint64_t)ci; s1 = 0; }
+            mask_words(s0, s1, N);
+            cx<T> th[JPL];
+#pragma unroll
+            for (int u = 0; u < JPL; u++) th[u] = md.b[lane + 16 * u];
+            for (int i0 = 0; i0 < N; i0 += EV_ROWS) {
+                const int nr = N - i0 < EV_ROWS ? N - i0 : EV_ROWS;
+                __syncthreads();   // the previous rows have been used
+                for (int k = threadIdx.x; k < nr * ld; k += blockDim.x) Wc[k] = md.W[(int64_t)i0 * ld + k];
+                __syncthreads();
+                for (int r = 0; r < nr; r++) {
+                    const T si = spin_of<T>(s0, s1, i0 + r);
+#pragma unroll
+                    for (int u = 0; u < JPL; u++) th[u] = th[u] + si * Wc[r * ld + lane + 16 * u];
+                }
+            }
+            const cx<T> vis = visible_sum(md, s0, s1);
+            measure<T, JPL>(md, md.W, op, has_op, s0, s1, th, vis, lane, ci, rec, valid);
+        }
+        return;
+    }
     for (int64_t idx = (int64_t)blockIdx.x * groups + (threadIdx.x >> 4); idx < n;
          idx += (int64_t)gridDim.x * groups) {
         uint64_t s0, s1;
@@ -1340,6 +1380,8 @@ hipError_t launch_eval(const RbmDev<T>& md, const OpDev& op, int has_op, int64_t
     if (blocks < 1) blocks = 1;
     const dim3 block(256), grid((unsigned)blocks);
     const size_t shm = wlds ? lds_bytes : 0;
+    // FP32 models whose W is not in LDS stage EV_ROWS rows of it at a time
+    const size_t rows_shm = sizeof(T) == 4 ? (size_t)EV_ROWS * md.ld * sizeof(cx<T>) : 0;
 #define RBM_EVAL_CASE(J)                                                                                              \
     case J:                                                                                                           \
         if (wlds) {                                                                                                   \
@@ -1348,7 +1390,7 @@ hipError_t launch_eval(const RbmDev<T>& md, const OpDev& op, int has_op, int64_t
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                       \
             hipLaunchKernelGGL((k_eval_rbm<T, J, true>), grid, block, shm, stream, md, op, has_op, n, configs, index_base, rec); \
         }                                                                                                             \
-        else hipLaunchKernelGGL((k_eval_rbm<T, J, false>), grid, block, 0, stream, md, op, has_op, n, configs, index_base, rec);       \
+        else hipLaunchKernelGGL((k_eval_rbm<T, J, false>), grid, block, rows_shm, stream, md, op, has_op, n, configs, index_base, rec); \
         break;
     switch (jpl) {
         RBM_JPL_LIST(RBM_EVAL_CASE)
