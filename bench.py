@@ -77,7 +77,7 @@ def kernel_source_sha():
     import hashlib
     h = hashlib.sha256()
     for f in ("rbm_kernels.hip", "rbm_device.hpp", "rbm_kernels.hpp", "made_kernels.hip", "made_kernels.hpp",
-              "deep_mfma.hip", "deep_kernels.hpp"):
+              "deep_mfma.hip", "deep_kernels.hpp", "gram_kernels.hip", "linalg_kernels.hpp"):
         h.update(open(os.path.join(ROOT, "rbmongpu_amd", "csrc", f), "rb").read())
     return h.hexdigest()[:16]
 
@@ -671,7 +671,9 @@ def run_heisenberg6x6_sr(args, R, HAM, F):
         "tau_int": tau, "energy": E.real, "energy_se": se,
         "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in kms.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_BF16_TFLOPS,
+                     # HBM bytes of one Gram dispatch (one batch of 12 480 samples)
+                     "traffic": stored_traffic("k_gram_lower_bf3@heisenberg6x6-sr"),
                      "kernel": "k_gram_lower_bf3 (S Gram, lower 128x128 tiles, FP32 products as six bfloat16 part "
                                "products)",
                      "flop_per_step": 6 * flop, "fp32_flop_per_step": flop, "fp32_equivalent_tflops": fp32_equiv,
