@@ -72,13 +72,17 @@ def tau_int(series, c=5.0):
 
 
 def kernel_source_sha():
-    """Hash of the kernels' sources: a stored traffic measurement is used only
-    for the kernels it was taken on."""
+    """Hash of the kernels' sources with comments and blank space removed: a
+    stored traffic measurement is used only for the code it was taken on."""
     import hashlib
+    import re
     h = hashlib.sha256()
     for f in ("rbm_kernels.hip", "rbm_device.hpp", "rbm_kernels.hpp", "made_kernels.hip", "made_kernels.hpp",
               "deep_mfma.hip", "deep_kernels.hpp", "gram_kernels.hip", "linalg_kernels.hpp"):
-        h.update(open(os.path.join(ROOT, "rbmongpu_amd", "csrc", f), "rb").read())
+        text = open(os.path.join(ROOT, "rbmongpu_amd", "csrc", f)).read()
+        text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", " ", text)
+        h.update(" ".join(text.split()).encode())
     return h.hexdigest()[:16]
 
 

@@ -23,8 +23,8 @@ namespace rbmhip {
 // the matrix cores.  The batch's FP32 tile is added into the FP64 accumulator
 // C, so the FP32 error does not grow with the number of batches.
 //
-// A 128-thread workgroup owns the lower tile (bm, bn), t = bm (bm + 1) / 2 + bn,
-// as two waves of 128 x 64 (8 x 4 MFMA tiles; lane l supplies A[l % 16][8 (l /
+// A 128-thread workgroup owns the lower tile (bm, bn) that order[blockIdx.x]
+// names (gram_tile_order below), as two waves of 128 x 64 (8 x 4 MFMA tiles; lane l supplies A[l % 16][8 (l /
 // 16) + q] and B[8 (l / 16) + q][l % 16], accumulator v of lane l is row 4 (l /
 // 16) + v, column l % 16).  The stored layout is the MFMA operand layout: the
 // fragment of lane l for a column tile is the 16 bytes of column c0 + l % 16 at
@@ -34,10 +34,11 @@ namespace rbmhip {
 // at its end; the A fragments of row tile i are reloaded for the next block as
 // soon as row tile i's 24 MFMAs of this block are issued.  With 128 FP32
 // accumulators the wave runs alone on its SIMD.
-// Measured against other forms on the 6x6 Heisenberg SR step (22 batches of
-// 12 480 samples, ld 10 752): 150 ms of Gram product per step; staging both
-// column stripes through LDS for 2 x 2 waves of 64 x 64, 165-171 ms; 64 x 64
-// per wave loaded straight from memory (two waves per SIMD), 244 ms.
+// Measured against other forms on the 6x6 Heisenberg SR step (21 batches of
+// 12 480 samples and one of 64, ld 10 752), in the row-major tile order:
+// 150 ms of Gram product per step; staging both column stripes through LDS
+// for 2 x 2 waves of 64 x 64, 165-171 ms; 64 x 64 per wave loaded straight
+// from memory (two waves per SIMD), 244 ms.  The grouped order below: 147 ms.
 // ---------------------------------------------------------------------------
 constexpr int GM_T = 128, GB_K = 32;
 typedef float gbf32x4 __attribute__((ext_vector_type(4)));
