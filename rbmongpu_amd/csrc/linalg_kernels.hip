@@ -6,7 +6,7 @@
 //
 // Blocked right-looking factorisation with NB = 32:
 //   k_potrf_diag   factor the diagonal block in LDS (one 1024-thread workgroup)
-//   k_trsm_panel   rows below: X L_kk^H = A_ik (one thread per row)
+//   k_trsm_panel   rows below: X L_kk^H = A_ik (32 lanes per row)
 //   k_herk_update  trailing lower triangle: A_il -= X_i X_l^H (32 x 32 tiles)
 // then blocked forward and backward substitution (k_trsv_block, k_trsv_update).
 #include <hip/hip_runtime.h>
@@ -58,7 +58,21 @@ __global__ __launch_bounds__(1024) void k_potrf_diag(int n, int k0, cx<double>* 
     if (threadIdx.x == 0 && bad && *info == 0) *info = bad;
 }
 
-// Rows r >= k0 + nb: solve X L_kk^H = A[r, k0:k0+nb] in place.
+// Lane j's value of a complex double, broadcast to the whole wavefront (j is
+// wave-uniform).
+RBM_DI cx<double> lane_bcast(cx<double> v, int j) {
+    const int2 r = *reinterpret_cast<int2*>(&v.re), i = *reinterpret_cast<int2*>(&v.im);
+    int2 rr, ii;
+    rr.x = __builtin_amdgcn_readlane(r.x, j); rr.y = __builtin_amdgcn_readlane(r.y, j);
+    ii.x = __builtin_amdgcn_readlane(i.x, j); ii.y = __builtin_amdgcn_readlane(i.y, j);
+    return mk(*reinterpret_cast<double*>(&rr), *reinterpret_cast<double*>(&ii));
+}
+
+// Rows r >= k0 + nb: solve X L_kk^H = A[r, k0:k0+nb] in place, i.e. for each
+// row x_j = (a_j - sum_{m<j} x_m conj(L_jm)) / L_jj.  32 lanes per row (two
+// rows per wavefront), lane j holding x_j: at step m the row's x_m is final and
+// lanes j > m subtract x_m conj(L_jm), so each x_j receives its terms in the
+// order m = 0, 1, ... as in a sequential solve.
 __global__ __launch_bounds__(256) void k_trsm_panel(int n, int k0, cx<double>* __restrict__ A) {
     __shared__ cx<double> L[NB][NB + 1];
     const int nb = n - k0 < NB ? n - k0 : NB;
@@ -67,30 +81,24 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int k0, cx<double>* _
         L[i][l] = (i < nb && l <= i) ? A[(int64_t)(k0 + i) * n + k0 + l] : mk(0.0, 0.0);
     }
     __syncthreads();
-    const int64_t r = (int64_t)k0 + nb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    cx<double> x[NB];
-    cx<double>* row = A + r * n + k0;
-#pragma unroll
-    for (int j = 0; j < NB; j++) x[j] = j < nb ? row[j] : mk(0.0, 0.0);
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-        if (j < nb) {
-            cx<double> a = x[j];
-#pragma unroll
-            for (int m = 0; m < NB; m++) {
-                if (m < j) {
-                    const cx<double> u = cmul_conj(x[m], L[j][m]);
-                    a = mk(a.re - u.re, a.im - u.im);
-                }
-            }
-            const double r_ = 1.0 / L[j][j].re;
-            x[j] = mk(a.re * r_, a.im * r_);
+    const int j = threadIdx.x & 31, half = (threadIdx.x >> 5) & 1;
+    const int64_t r = (int64_t)k0 + nb + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 32;
+    const bool live = r < n && j < nb;
+    cx<double>* row = A + (r < n ? r : n - 1) * n + k0;
+    cx<double> x = live ? row[j] : mk(0.0, 0.0);
+    for (int m = 0; m < nb; m++) {
+        if (j == m) {
+            const double rd = 1.0 / L[m][m].re;
+            x = mk(x.re * rd, x.im * rd);
+        }
+        const cx<double> x0 = lane_bcast(x, m), x1 = lane_bcast(x, 32 + m);
+        const cx<double> xm = half ? x1 : x0;
+        if (j > m) {
+            const cx<double> u = cmul_conj(xm, L[j][m]);
+            x = mk(x.re - u.re, x.im - u.im);
         }
     }
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-        if (j < nb) row[j] = x[j];
+    if (live) row[j] = x;
 }
 
 // Trailing update of the lower triangle below the panel: tile (bi, bj) with
@@ -244,7 +252,7 @@ hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream) 
         hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(NB * NB), 0, stream, n, k0, A, info);
         const int rest = n - k0 - nb;
         if (rest <= 0) break;
-        hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, stream, n, k0, A);
+        hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)((rest + 7) / 8)), dim3(256), 0, stream, n, k0, A);
         const int tiles = (rest + NB - 1) / NB;
         hipLaunchKernelGGL(k_herk_update, dim3((unsigned)(tiles * (tiles + 1) / 2)), dim3(256), 0, stream, n, k0, A);
     }
@@ -252,7 +260,8 @@ hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream) 
 }
 
 // Blocked substitution over all CUs.  Per diagonal block k0 (NB rows):
-//   k_trsv_block   one wavefront solves the block with its 32 x 32 factor
+//   k_trsv_block   one wavefront solves the block with its 32 x 32 factor,
+//                  one lane per row
 //                  (forward: y_b = L_bb^{-1} x_b; backward: x_b = L_bb^{-H} y_b)
 //   k_trsv_update  the rows outside the block subtract the block's products,
 //                  one wavefront per row pair with coalesced row reads
@@ -260,40 +269,44 @@ hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream) 
 //                  x_i -= sum_m conj(L_mi) x_m over the block's rows m).
 __global__ __launch_bounds__(64) void k_trsv_block(int n, int k0, int backward, const cx<double>* __restrict__ A,
                                                    cx<double>* __restrict__ x) {
+    // lane j < nb holds x_j; at step m the finished x_m is broadcast and the
+    // lanes still to finish subtract its term (forward: L_jm x_m for j > m;
+    // backward: conj(L_mj) x_m for j < m)
     __shared__ cx<double> Lb[NB][NB + 1];
-    __shared__ cx<double> xb[NB];
     const int nb = n - k0 < NB ? n - k0 : NB;
     for (int t = threadIdx.x; t < NB * NB; t += 64) {
         const int i = t / NB, l = t % NB;
         Lb[i][l] = (i < nb && l < nb && l <= i) ? A[(int64_t)(k0 + i) * n + k0 + l] : mk(0.0, 0.0);
     }
-    if (threadIdx.x < NB) xb[threadIdx.x] = threadIdx.x < nb ? x[k0 + threadIdx.x] : mk(0.0, 0.0);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (!backward) {
-            for (int j = 0; j < nb; j++) {
-                cx<double> a = xb[j];
-                for (int m = 0; m < j; m++) {
-                    const cx<double> L = Lb[j][m], y = xb[m];
-                    a = mk(a.re - (L.re * y.re - L.im * y.im), a.im - (L.re * y.im + L.im * y.re));
-                }
-                const double r = 1.0 / Lb[j][j].re;
-                xb[j] = mk(a.re * r, a.im * r);
+    const int j = threadIdx.x;
+    cx<double> v = j < nb ? x[k0 + j] : mk(0.0, 0.0);
+    if (!backward) {
+        for (int m = 0; m < nb; m++) {
+            if (j == m) {
+                const double r = 1.0 / Lb[m][m].re;
+                v = mk(v.re * r, v.im * r);
             }
-        } else {
-            for (int j = nb - 1; j >= 0; j--) {
-                cx<double> a = xb[j];
-                for (int m = j + 1; m < nb; m++) {
-                    const cx<double> u = cmul_conj(xb[m], Lb[m][j]);
-                    a = mk(a.re - u.re, a.im - u.im);
-                }
-                const double r = 1.0 / Lb[j][j].re;
-                xb[j] = mk(a.re * r, a.im * r);
+            const cx<double> y = lane_bcast(v, m);
+            if (j > m && j < nb) {
+                const cx<double> L = Lb[j][m];
+                v = mk(v.re - (L.re * y.re - L.im * y.im), v.im - (L.re * y.im + L.im * y.re));
+            }
+        }
+    } else {
+        for (int m = nb - 1; m >= 0; m--) {
+            if (j == m) {
+                const double r = 1.0 / Lb[m][m].re;
+                v = mk(v.re * r, v.im * r);
+            }
+            const cx<double> y = lane_bcast(v, m);
+            if (j < m) {
+                const cx<double> u = cmul_conj(y, Lb[m][j]);
+                v = mk(v.re - u.re, v.im - u.im);
             }
         }
     }
-    __syncthreads();
-    if (threadIdx.x < nb) x[k0 + threadIdx.x] = xb[threadIdx.x];
+    if (j < nb) x[k0 + j] = v;
 }
 
 __global__ __launch_bounds__(256) void k_trsv_update(int n, int k0, int backward, const cx<double>* __restrict__ A,
