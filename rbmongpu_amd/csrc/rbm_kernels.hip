@@ -1638,56 +1638,77 @@ hipError_t launch_theta_logpsi(const RbmDev<float>& md, int64_t n, const uint64_
 // Each FP32 value is split exactly into three bfloat16 parts here, once, for
 // the Gram kernel's matrix cores, and stored transposed per block of 32
 // samples: part q of column c and sample 32 b + k at
-// out[((b * 3 + q) * ld2 + c) * 32 + k].  A thread forms two neighbouring
-// samples of one column; 16 threads write a column's 32 samples (64 bytes per
-// part) together.
+// out[((b * 3 + q) * ld2 + c) * 32 + k].
 // ---------------------------------------------------------------------------
+// One workgroup per (32-sample block blockIdx.y, 128 columns from
+// 128 blockIdx.x).  The per-sample factors (sqrt w, spin words) are staged in
+// LDS once and each thread decodes its column once; the values are formed
+// with consecutive threads on consecutive columns (the my_tanh records of a
+// sample are read along its row) into LDS, then split and written with
+// consecutive threads on the 16 sample pairs of consecutive columns, so both
+// the reads and the 3 x 8 KB of writes are contiguous.
 template <typename T>
-RBM_DI float o_realified_value(int64_t s, int c, int N, int M, int P, const cx<T>* __restrict__ Tr,
-                               const uint64_t* __restrict__ spins, const cx<double>* __restrict__ w,
-                               const cx<float>* __restrict__ obar) {
-    const int k = c < P ? c : c - P;
-    const uint64_t x0 = spins[2 * s], x1 = spins[2 * s + 1];
-    cx<float> o;
-    if (k < M) {
-        const cx<T> tj = Tr[s * M + k];
-        o = mk((float)tj.re, (float)tj.im);
-    } else if (k < M + N * M) {
-        const int i = (k - M) / M, j = (k - M) % M;
-        const cx<T> tj = Tr[s * M + j];
-        const float si = spin_of<float>(x0, x1, i);
-        o = mk(si * (float)tj.re, si * (float)tj.im);
-    } else {
-        o = mk(spin_of<float>(x0, x1, k - M - N * M), 0.0f);
-    }
-    const float sw = (float)sqrt(w[s].re);
-    const cx<float> d = o - obar[k];
-    return sw * (c < P ? d.re : d.im);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_o_split3(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2,
+__global__ __launch_bounds__(256) void k_o_split3(int64_t s0, int64_t nb, int N, int M, int P, int ld2,
                                                   const void* __restrict__ Trec, const uint64_t* __restrict__ spins,
                                                   const cx<double>* __restrict__ w, const cx<float>* __restrict__ obar,
                                                   uint32_t* __restrict__ out) {
-    // one work item per (block b, column c, sample pair kp): 16 pairs per column
-    const int64_t total = (nbp / 32) * (int64_t)ld2 * 16;
+    __shared__ float vals[32][129];
+    __shared__ float rsw[32];
+    __shared__ uint64_t rsp[32][2];
     const cx<T>* Tr = reinterpret_cast<const cx<T>*>(Trec);
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int kp = (int)(t & 15);
-        const int64_t bc = t >> 4;
-        const int c = (int)(bc % ld2);
-        const int64_t blk = bc / ld2;
-        const int64_t r = blk * 32 + 2 * kp;
-        float v0 = 0.0f, v1 = 0.0f;
-        if (c < 2 * P) {
-            if (r < nb) v0 = o_realified_value<T>(s0 + r, c, N, M, P, Tr, spins, w, obar);
-            if (r + 1 < nb) v1 = o_realified_value<T>(s0 + r + 1, c, N, M, P, Tr, spins, w, obar);
+    const int64_t blk = blockIdx.y;
+    if (threadIdx.x < 32) {
+        const int64_t sr = blk * 32 + threadIdx.x;
+        const bool in = sr < nb;
+        rsw[threadIdx.x] = in ? (float)sqrt(w[s0 + sr].re) : 0.0f;
+        rsp[threadIdx.x][0] = in ? spins[2 * (s0 + sr)] : 0;
+        rsp[threadIdx.x][1] = in ? spins[2 * (s0 + sr) + 1] : 0;
+    }
+    const int c0 = blockIdx.x * 128;
+    const int cc = threadIdx.x & 127;
+    const int c = c0 + cc;
+    // column c: O_k with k = c mod P, real part for c < P, imaginary part after
+    const bool live = c < 2 * P;
+    const bool re_part = c < P;
+    const int k = live ? (re_part ? c : c - P) : 0;
+    int j = 0, si_idx = -1;   // T_j factor (absent for the visible biases), spin factor s_i
+    bool has_t = true;
+    if (k < M) {
+        j = k;
+    } else if (k < M + N * M) {
+        si_idx = (k - M) / M;
+        j = (k - M) % M;
+    } else {
+        has_t = false;
+        si_idx = k - M - N * M;
+    }
+    const cx<float> ob = obar[k];
+    __syncthreads();
+    for (int r = threadIdx.x >> 7; r < 32; r += 2) {
+        const int64_t sr = blk * 32 + r;
+        float v = 0.0f;
+        if (live && sr < nb) {
+            cx<float> o = mk(1.0f, 0.0f);
+            if (has_t) {
+                const cx<T> tj = Tr[(s0 + sr) * M + j];
+                o = mk((float)tj.re, (float)tj.im);
+            }
+            if (si_idx >= 0) {
+                const float si = spin_of<float>(rsp[r][0], rsp[r][1], si_idx);
+                o = mk(si * o.re, si * o.im);
+            }
+            const cx<float> d = o - ob;
+            v = rsw[r] * (re_part ? d.re : d.im);
         }
+        vals[r][cc] = v;
+    }
+    __syncthreads();
+    const int kp = threadIdx.x & 15;
+    for (int cw = threadIdx.x >> 4; cw < 128; cw += 16) {
         uint32_t hi, mid, lo;
-        split3(v0, v1, hi, mid, lo);
+        split3(vals[2 * kp][cw], vals[2 * kp + 1][cw], hi, mid, lo);
         // 32-bit words: ((blk * 3 + q) * ld2 + c) * 16 + kp
-        const int64_t base = (blk * 3 * ld2 + c) * 16 + kp;
+        const int64_t base = (blk * 3 * ld2 + c0 + cw) * 16 + kp;
         out[base] = hi;
         out[base + (int64_t)ld2 * 16] = mid;
         out[base + (int64_t)ld2 * 32] = lo;
@@ -1714,11 +1735,9 @@ template <typename T>
 hipError_t launch_o_split3(int64_t s0, int64_t nb, int64_t nbp, int N, int M, int P, int ld2, const void* Trec,
                            const uint64_t* spins, const cx<double>* w, const cx<float>* obar, uint32_t* out,
                            hipStream_t stream) {
-    if (nbp % 32 || nb > nbp || ld2 < 2 * P) return hipErrorInvalidValue;
-    const int64_t total = (nbp / 32) * (int64_t)ld2 * 16;
-    const unsigned blocks = (unsigned)((total + 255) / 256 < 16384 ? (total + 255) / 256 : 16384);
-    hipLaunchKernelGGL((k_o_split3<T>), dim3(blocks), dim3(256), 0, stream, s0, nb, nbp, N, M, P, ld2, Trec, spins, w,
-                       obar, out);
+    if (nbp % 32 || nb > nbp || ld2 < 2 * P || ld2 % 128 || nbp / 32 > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_o_split3<T>), dim3((unsigned)(ld2 / 128), (unsigned)(nbp / 32)), dim3(256), 0, stream, s0,
+                       nb, N, M, P, ld2, Trec, spins, w, obar, out);
     return hipGetLastError();
 }
 
