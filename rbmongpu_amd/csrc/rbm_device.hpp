@@ -104,8 +104,25 @@ constexpr double LC_A3 = 3.746646023906276, LC_A2 = 7.771429504240965, LC_A1 = 1
                  LC_A0 = 9.19376335670885, LC_N0 = 5.49914721954, LC_N1 = 2.16564366435,
                  LC_K = 0.9003320053750442, LC_C = 0.598139;
 
-// D_+(w) = (((w + a3) w + a2) w + a1) w + a0 at w = wr + i wi
+// D_+(w) = (((w + a3) w + a2) w + a1) w + a0 at w = wr + i wi.
+// FP32: D_+ has real coefficients and w is a root of x^2 - s x + t with
+// s = 2 wr, t = |w|^2, so D_+(w) is the remainder R1 w + R0 of D_+ divided by
+// that quadratic: with c3 = a3 + s, c2 = a2 - t + s c3,
+// R1 = a1 + s c2 - t c3 and R0 = a0 - t c2 (11 instructions against 13 for
+// the complex Horner form; the same accuracy for |w| below 1 and within 2x
+// of it up to |w| ~ 10).  FP64 keeps the Horner form.
 template <typename T> RBM_DI void lc_den_w(T wr, T wi, T& dr, T& di) {
+    if constexpr (sizeof(T) == 4) {
+        const T s = wr + wr;
+        const T t = fm(wr, wr, wi * wi);
+        const T c3 = s + T(LC_A3);
+        const T c2 = fm(s, c3, T(LC_A2) - t);
+        const T r1 = fm(s, c2, fm(-t, c3, T(LC_A1)));
+        const T r0 = fm(-t, c2, T(LC_A0));
+        dr = fm(r1, wr, r0);
+        di = r1 * wi;
+        return;
+    }
     const T ur = wr + T(LC_A3);
     const T vr = fm(wr, ur, fm(-wi, wi, T(LC_A2)));
     const T vi = wi * (wr + ur);
