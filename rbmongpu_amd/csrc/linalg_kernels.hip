@@ -4,10 +4,13 @@
 // builds S).  Matrices are row-major, n x n complex<double>; the factor
 // overwrites the lower triangle (A = L L^H).
 //
-// Blocked right-looking factorisation with NB = 32:
+// Blocked right-looking factorisation, panels of NB = 32 in block columns of
+// NB2 = 128:
 //   k_potrf_diag   factor the diagonal block in LDS (one 1024-thread workgroup)
 //   k_trsm_panel   rows below: X L_kk^H = A_ik (32 lanes per row)
-//   k_herk_update  trailing lower triangle: A_il -= X_i X_l^H (32 x 32 tiles)
+//   k_herk_update  lower-triangle update A_il -= X_i X_l^H (64 x 64 tiles):
+//                  per panel within its block column, per block column
+//                  (rank 128) on the trailing matrix
 // then blocked forward and backward substitution (k_trsv_block, k_trsv_update).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -101,40 +104,77 @@ __global__ __launch_bounds__(256) void k_trsm_panel(int n, int k0, cx<double>* _
     if (live) row[j] = x;
 }
 
-// Trailing update of the lower triangle below the panel: tile (bi, bj) with
-// bi >= bj of 32 x 32 entries, rows/cols offset by k0 + nb.
-__global__ __launch_bounds__(256) void k_herk_update(int n, int k0, cx<double>* __restrict__ A) {
-    __shared__ cx<double> Pi[NB][NB + 1];
-    __shared__ cx<double> Pl[NB][NB + 1];
-    const int nb = n - k0 < NB ? n - k0 : NB;
-    const int base = k0 + nb;
-    // blockIdx.x enumerates lower-triangle tiles: t = bi (bi + 1) / 2 + bj
-    const int t = blockIdx.x;
-    int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
-    while (bi * (bi + 1) / 2 > t) bi--;
-    const int bj = t - bi * (bi + 1) / 2;
-    const int r0 = base + bi * NB, c0 = base + bj * NB;
-    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
-        const int a = q / NB, m = q % NB;
-        const int ri = r0 + a, rl = c0 + a;
-        Pi[a][m] = (ri < n && m < nb) ? A[(int64_t)ri * n + k0 + m] : mk(0.0, 0.0);
-        Pl[a][m] = (rl < n && m < nb) ? A[(int64_t)rl * n + k0 + m] : mk(0.0, 0.0);
+// Trailing update A_il -= sum_m A_im conj(A_lm) over the panel columns
+// m = kc .. kc + kw - 1, for entries with row i >= column l >= base and
+// l < col_end.  One 256-thread workgroup per 64 x 64 tile: thread
+// (tx, ty) = (t % 16, t / 16) owns rows ty + 16 a and columns tx + 16 b
+// (a, b < 4), so each panel value read from LDS serves four products.  The
+// panel is staged 32 columns at a time.  ctiles == 0: the lower-triangle tiles
+// of [base, n)^2, t = bi (bi + 1) / 2 + bj; ctiles > 0: a 2-D grid over a band
+// of ctiles column tiles (bj = blockIdx.x, bi = blockIdx.y), tiles above the
+// diagonal exit.
+constexpr int HT = 64;
+__global__ __launch_bounds__(256) void k_herk_update(int n, int kc, int kw, int base, int col_end, int ctiles,
+                                                     cx<double>* __restrict__ A) {
+    __shared__ cx<double> Pi[HT][NB + 1];
+    __shared__ cx<double> Pl[HT][NB + 1];
+    int bi, bj;
+    if (ctiles == 0) {
+        const int t = blockIdx.x;
+        bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
+        while (bi * (bi + 1) / 2 > t) bi--;
+        bj = t - bi * (bi + 1) / 2;
+    } else {
+        bj = blockIdx.x;
+        bi = blockIdx.y;
+        if (bj > bi) return;
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < NB * NB; q += blockDim.x) {
-        const int a = q / NB, b = q % NB;
-        const int ri = r0 + a, cl = c0 + b;
-        if (ri >= n || cl >= n || cl > ri) continue;
-        double sr = 0.0, si = 0.0;
-#pragma unroll 8
-        for (int m = 0; m < NB; m++) {
-            const cx<double> u = cmul_conj(Pi[a][m], Pl[b][m]);
-            sr += u.re;
-            si += u.im;
+    const int r0 = base + bi * HT, c0 = base + bj * HT;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double accr[4][4], acci[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) accr[a][b] = acci[a][b] = 0.0;
+    for (int k0 = 0; k0 < kw; k0 += NB) {
+        for (int q = threadIdx.x; q < HT * NB; q += 256) {
+            const int a = q / NB, m = q % NB;
+            const int ri = r0 + a, rl = c0 + a;
+            const bool live = k0 + m < kw;
+            Pi[a][m] = (ri < n && live) ? A[(int64_t)ri * n + kc + k0 + m] : mk(0.0, 0.0);
+            Pl[a][m] = (rl < n && live) ? A[(int64_t)rl * n + kc + k0 + m] : mk(0.0, 0.0);
         }
-        cx<double>& e = A[(int64_t)ri * n + cl];
-        e = mk(e.re - sr, e.im - si);
+        __syncthreads();
+#pragma unroll 2
+        for (int m = 0; m < NB; m++) {
+            cx<double> pi[4], pl[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) pi[a] = Pi[ty + 16 * a][m];
+#pragma unroll
+            for (int b = 0; b < 4; b++) pl[b] = Pl[tx + 16 * b][m];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    // pi conj(pl)
+                    accr[a][b] = __builtin_fma(pi[a].re, pl[b].re, __builtin_fma(pi[a].im, pl[b].im, accr[a][b]));
+                    acci[a][b] = __builtin_fma(pi[a].im, pl[b].re, __builtin_fma(-pi[a].re, pl[b].im, acci[a][b]));
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const int ri = r0 + ty + 16 * a;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int cl = c0 + tx + 16 * b;
+            if (ri < n && cl <= ri && cl < col_end) {
+                cx<double>& e = A[(int64_t)ri * n + cl];
+                e = mk(e.re - accr[a][b], e.im - acci[a][b]);
+            }
+        }
     }
 }
 
@@ -246,15 +286,32 @@ hipError_t launch_gemm_tn_splitk(int ldx, int ldy, int64_t K, int splits, const 
     return hipGetLastError();
 }
 
+// Two-level blocking: block columns of NB2 = 128 are factored panel by panel
+// (NB = 32), each panel's update confined to the rest of its block column;
+// then one rank-128 update covers the trailing matrix, which is read and
+// written once per 128 columns instead of once per 32.
+constexpr int NB2 = 128;
 hipError_t launch_cholesky(int n, cx<double>* A, int* info, hipStream_t stream) {
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        const int nb = n - k0 < NB ? n - k0 : NB;
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(NB * NB), 0, stream, n, k0, A, info);
-        const int rest = n - k0 - nb;
+    for (int K0 = 0; K0 < n; K0 += NB2) {
+        const int bend = n - K0 < NB2 ? n : K0 + NB2;
+        for (int c = K0; c < bend; c += NB) {
+            const int nb = n - c < NB ? n - c : NB;
+            hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(NB * NB), 0, stream, n, c, A, info);
+            const int rest = n - c - nb;
+            if (rest <= 0) break;
+            hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)((rest + 7) / 8)), dim3(256), 0, stream, n, c, A);
+            const int base = c + nb;
+            if (base < bend) {
+                const int ct = (bend - base + HT - 1) / HT, rt = (n - base + HT - 1) / HT;
+                hipLaunchKernelGGL(k_herk_update, dim3((unsigned)ct, (unsigned)rt), dim3(256), 0, stream, n, c, nb,
+                                   base, bend, ct, A);
+            }
+        }
+        const int rest = n - bend;
         if (rest <= 0) break;
-        hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)((rest + 7) / 8)), dim3(256), 0, stream, n, k0, A);
-        const int tiles = (rest + NB - 1) / NB;
-        hipLaunchKernelGGL(k_herk_update, dim3((unsigned)(tiles * (tiles + 1) / 2)), dim3(256), 0, stream, n, k0, A);
+        const int tiles = (rest + HT - 1) / HT;
+        hipLaunchKernelGGL(k_herk_update, dim3((unsigned)(tiles * (tiles + 1) / 2)), dim3(256), 0, stream, n, K0,
+                           bend - K0, bend, n, 0, A);
     }
     return hipGetLastError();
 }

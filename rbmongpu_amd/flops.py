@@ -81,6 +81,7 @@ def s_matrix_flops(P, samples):
 
 
 def cholesky_flops(n):
-    """Real FLOP of a complex Cholesky factorisation: n^3 / 3 complex
-    multiply-adds of 8 real FLOP."""
-    return 8.0 * n ** 3 / 3.0
+    """Real FLOP of a complex Hermitian Cholesky factorisation A = L L^H:
+    the updates of the lower triangle are n^3 / 6 complex multiply-adds of
+    8 real FLOP (LAPACK's zpotrf count, 4 n^3 / 3 to leading order)."""
+    return 4.0 * n ** 3 / 3.0

@@ -28,7 +28,7 @@ def hpd(n, seed):
     return B @ B.conj().T / n + 0.5 * np.eye(n), rng.normal(size=n) + 1j * rng.normal(size=n)
 
 
-@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 64, 100, 257, 600])
+@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 64, 100, 129, 160, 257, 600, 1000])
 def test_cholesky_solve_matches_numpy(n):
     print(f"[cholesky] n={n}", flush=True)
     A, b = hpd(n, n)
