@@ -25,39 +25,39 @@ RBM_DI cx<double> cmul_conj(cx<double> a, cx<double> b) {   // a * conj(b)
     return mk(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
 }
 
-// One workgroup of NB x NB threads factors A[k0:k0+nb, k0:k0+nb] in LDS.
-// A non-positive pivot sets *info = global pivot index + 1.
+// One workgroup of NB x NB threads factors A[k0:k0+nb, k0:k0+nb] in LDS, one
+// barrier per column: at step j every thread reads the pivot d = A_jj,
+// thread (i, j) writes the finished L_ij (A_ij / sqrt(d), or sqrt(d) on the
+// diagonal) straight to global memory, and threads (i, l) with j < l <= i
+// subtract (A_ij / sqrt d) conj(A_lj / sqrt d) in LDS.  Column j is only read
+// during step j, so the scaled values never need to go back to LDS.
+// A non-positive pivot sets *info = global pivot index + 1 (and is replaced
+// by 1 so that the factorisation runs to the end).
 __global__ __launch_bounds__(1024) void k_potrf_diag(int n, int k0, cx<double>* __restrict__ A, int* __restrict__ info) {
     __shared__ cx<double> L[NB][NB + 1];
-    __shared__ int bad;
     const int i = threadIdx.x / NB, l = threadIdx.x % NB;
     const int nb = n - k0 < NB ? n - k0 : NB;
-    if (threadIdx.x == 0) bad = 0;
     L[i][l] = (i < nb && l < nb && l <= i) ? A[(int64_t)(k0 + i) * n + k0 + l] : mk(0.0, 0.0);
     __syncthreads();
+    int bad = 0;
     for (int j = 0; j < nb; j++) {
-        if (i == j && l == j) {
-            const double d = L[j][j].re;
-            if (!(d > 0.0)) {
-                bad = k0 + j + 1;
-                L[j][j] = mk(1.0, 0.0);
-            } else {
-                L[j][j] = mk(sqrt(d), 0.0);
+        double d = L[j][j].re;
+        if (!(d > 0.0)) {
+            if (!bad) bad = k0 + j + 1;
+            d = 1.0;
+        }
+        const double sd = sqrt(d), rs = 1.0 / sd;
+        if (i < nb && i >= j) {
+            if (l == j) {
+                A[(int64_t)(k0 + i) * n + k0 + j] = i == j ? mk(sd, 0.0) : mk(L[i][j].re * rs, L[i][j].im * rs);
+            } else if (l > j && l <= i) {
+                const cx<double> a = L[i][j], c = L[l][j];
+                const cx<double> u = cmul_conj(mk(a.re * rs, a.im * rs), mk(c.re * rs, c.im * rs));
+                L[i][l] = mk(L[i][l].re - u.re, L[i][l].im - u.im);
             }
         }
         __syncthreads();
-        if (l == j && i > j && i < nb) {
-            const double r = 1.0 / L[j][j].re;
-            L[i][j] = mk(L[i][j].re * r, L[i][j].im * r);
-        }
-        __syncthreads();
-        if (i > j && i < nb && l > j && l <= i) {
-            const cx<double> u = cmul_conj(L[i][j], L[l][j]);
-            L[i][l] = mk(L[i][l].re - u.re, L[i][l].im - u.im);
-        }
-        __syncthreads();
     }
-    if (i < nb && l < nb && l <= i) A[(int64_t)(k0 + i) * n + k0 + l] = L[i][l];
     if (threadIdx.x == 0 && bad && *info == 0) *info = bad;
 }
 
