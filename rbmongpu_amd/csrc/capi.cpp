@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -118,6 +119,16 @@ struct Ctx {
     };
     std::vector<Pending> pending;
     std::map<std::string, std::pair<double, int64_t>> totals;
+    // second stream (default priority) for work that overlaps the main one,
+    // with two events that order the streams; created on first use
+    hipStream_t side = nullptr;
+    hipEvent_t ev_main = nullptr, ev_side = nullptr;
+    void ensure_side() {
+        if (side) return;
+        RBM_HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        RBM_HIPCHK(hipEventCreateWithFlags(&ev_main, hipEventDisableTiming));
+        RBM_HIPCHK(hipEventCreateWithFlags(&ev_side, hipEventDisableTiming));
+    }
 
     void* buf(const char* name, size_t bytes) { return ws[name].ensure(bytes); }
 
@@ -962,6 +973,45 @@ static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const 
     }
 }
 
+// Overlap of sampling and evaluation for low-occupancy single-flip chains.
+// With at most one wavefront per SIMD (16 lanes per chain) the sampler's
+// per-proposal dependency chain leaves issue slots idle.  The chains then run
+// in two launches, and the first launch's samples are evaluated on the side
+// stream while the second launch samples.  The split is placed where one
+// launch would recompute the FP32 angles from the configuration anyway (a
+// multiple of the re-sync interval and of the draw block), so the second
+// launch, which starts from the first one's configurations with fresh angles
+// and the same Philox counters (McParams::k0), reproduces one launch's chains
+// bit for bit.  Returns the first launch's samples per chain, or 0 for one
+// launch.  RBMHIP_MC_SPLIT=0 in the environment turns it off.
+static int mc_split_point(const Ctx& c, const McParams& mp, int N, bool fp32) {
+    const char* env = std::getenv("RBMHIP_MC_SPLIT");
+    if (env && env[0] == '0') return 0;
+    if (!fp32 || mp.move != MC_SINGLE_FLIP || mp.resync <= 0 || mp.sweeps <= 0) return 0;
+    if ((int64_t)mp.chains > 16LL * c.num_cus) return 0;
+    const int S = mp.steps_per_chain;
+    if (S < 16) return 0;
+    uint64_t L = (uint64_t)mp.resync, g = (uint64_t)MC_DRAW_BLOCK, r = L;
+    while (r) { const uint64_t t = g % r; g = r; r = t; }
+    L = L / g * MC_DRAW_BLOCK;
+    const uint64_t base = (uint64_t)mp.therm * N, per = (uint64_t)mp.sweeps * N;
+    int best = 0;
+    for (int s0 = S / 4; s0 <= S - S / 4; s0++)
+        if ((base + (uint64_t)s0 * per) % L == 0 && (best == 0 || std::abs(s0 - S / 2) < std::abs(best - S / 2))) best = s0;
+    return best;
+}
+
+// Records of samples [off, ...) of a batch: per-sample pointers advanced.
+template <typename T>
+static Records records_from(const Records& r, int64_t off, int M) {
+    Records o = r;
+    if (o.spins) o.spins += 2 * off;
+    if (o.log_psi) o.log_psi += off;
+    if (o.e_loc) o.e_loc += off;
+    if (o.tanh) o.tanh = reinterpret_cast<cx<T>*>(o.tanh) + off * (int64_t)M;
+    return o;
+}
+
 // Draw the samples of an ensemble for psi and measure them.  For
 // ExactSummation the samples are the basis (or sector) states.
 template <typename T>
@@ -1008,6 +1058,7 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
         if (e->kind == 1 && (e->persistent || e->record_angles) && psi->kind != 0)
             throw Error(RBMHIP_ERR_UNSUPPORTED, "persistent chains and angle records are implemented for RBM wavefunctions");
         uint32_t* counts = reinterpret_cast<uint32_t*>(c.buf("counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
+        uint32_t* counts_extra = nullptr;   // the second launch's counts when the chains run in two
         const OpDev od = op ? op_dev(c, op) : empty_op();
         if (e->kind == 2) {
             // independent proposals do not depend on the chain: draw all of them,
@@ -1085,10 +1136,44 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
                 e->angles_chains = mp.chains;
                 e->angles_precision = sizeof(T) == 4 ? RBMHIP_FP32 : RBMHIP_FP64;
             }
-            if (mp.move == MC_SINGLE_FLIP)
-                c.launch("mc", [&] { return launch_mc<T>(md, mp, s.spins, counts, wlds, lds, c.stream); });
-            else
-                c.launch("mc", [&] { return launch_mc_moves<T>(md, mp, s.spins, counts, c.stream); });
+            Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
+            const int S0 = mc_split_point(c, mp, psi->N, sizeof(T) == 4);
+            if (S0 > 0) {
+                c.ensure_side();
+                const int64_t n0 = (int64_t)S0 * mp.chains;
+                uint64_t* st = mp.final_out ? mp.final_out
+                                            : reinterpret_cast<uint64_t*>(c.buf("mc_split_state", (size_t)mp.chains * 2 * sizeof(uint64_t)));
+                counts_extra = reinterpret_cast<uint32_t*>(c.buf("mc_split_counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
+                McParams a = mp, b = mp;
+                a.steps_per_chain = S0;
+                a.final_out = st;
+                a.angles_out = nullptr;
+                b.steps_per_chain = mp.steps_per_chain - S0;
+                b.therm = 0;
+                b.start = st;
+                b.k0 = (uint64_t)mp.therm * psi->N + (uint64_t)S0 * mp.sweeps * psi->N;
+                c.launch("mc", [&] { return launch_mc<T>(md, a, s.spins, counts, wlds, lds, c.stream); });
+                RBM_HIPCHK(hipEventRecord(c.ev_main, c.stream));
+                RBM_HIPCHK(hipStreamWaitEvent(c.side, c.ev_main, 0));
+                std::swap(c.stream, c.side);
+                try {
+                    eval_model<T>(c, psi, op, n0, s.spins, 0, rec);
+                } catch (...) {
+                    std::swap(c.stream, c.side);
+                    throw;
+                }
+                std::swap(c.stream, c.side);
+                c.launch("mc", [&] { return launch_mc<T>(md, b, s.spins + 2 * n0, counts_extra, wlds, lds, c.stream); });
+                eval_model<T>(c, psi, op, s.n - n0, s.spins + 2 * n0, 0, records_from<T>(rec, n0, psi->M));
+                RBM_HIPCHK(hipEventRecord(c.ev_side, c.side));
+                RBM_HIPCHK(hipStreamWaitEvent(c.stream, c.ev_side, 0));
+            } else {
+                if (mp.move == MC_SINGLE_FLIP)
+                    c.launch("mc", [&] { return launch_mc<T>(md, mp, s.spins, counts, wlds, lds, c.stream); });
+                else
+                    c.launch("mc", [&] { return launch_mc_moves<T>(md, mp, s.spins, counts, c.stream); });
+                eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
+            }
             if (e->persistent) {
                 e->state_device = c.device;
                 e->state_chains = mp.chains;
@@ -1096,8 +1181,6 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
                 e->state_move = mp.move;
                 e->state_sector = e->sector;
             }
-            Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
-            eval_model<T>(c, psi, op, s.n, s.spins, 0, rec);
         } else if (psi->precision == RBMHIP_BF16) {
             const DeepMfmaDev mm = deep_mfma_dev(c, psi);
             c.launch("deep_mfma_mc", [&] { return launch_deep_mfma_mc(mm, mp, s.spins, counts, c.num_cus, c.stream); });
@@ -1108,11 +1191,16 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
             const DeepDev<T> md = deep_dev<T>(c, psi);
             c.launch("deep_mc", [&] { return launch_deep_mc<T>(md, od, op != nullptr, mp, rec, counts, c.stream); });
         }
-        std::vector<uint32_t> h((size_t)mp.chains * 2);
+        std::vector<uint32_t> h((size_t)mp.chains * 2), h2;
         c.d2h(h.data(), counts, h.size() * sizeof(uint32_t));
+        if (counts_extra) {
+            h2.resize(h.size());
+            c.d2h(h2.data(), counts_extra, h2.size() * sizeof(uint32_t));
+        }
         c.sync();
         e->acc = e->rej = 0;
         for (int i = 0; i < mp.chains; i++) { e->acc += h[2 * i]; e->rej += h[2 * i + 1]; }
+        for (size_t i = 0; i + 1 < h2.size(); i += 2) { e->acc += h2[i]; e->rej += h2[i + 1]; }
         if (!op) s.el = nullptr;
     } else {
         s.lp = reinterpret_cast<cx<double>*>(c.buf("lp", (size_t)s.n * sizeof(cx<double>)));

@@ -162,7 +162,6 @@ RBM_DI void measure(const RbmDev<T>& md, const cx<T>* __restrict__ Wsrc, const O
 // current one is evaluated.  Rows past the last chain replay the last chain
 // and write nothing, so every wavefront reaches every barrier.
 // ---------------------------------------------------------------------------
-constexpr int MC_DRAW_BLOCK = 32;
 constexpr size_t MC_DRAW_LDS = 16 * MC_DRAW_BLOCK * sizeof(uint2);
 
 // exp(2 delta), the ratio |psi(s')/psi(s)|^2
@@ -301,7 +300,8 @@ __global__ __launch_bounds__(256) void k_mc_rbm(RbmDev<T> md, McParams mp, uint6
         __syncthreads();
         {
             // x = position * ld | position << 16 | (31 - position) << 24
-            const u32x4 r = philox((uint32_t)chain, (uint32_t)((kb >> 1) + lane), mp.call, 0u, mp.seed_lo, mp.seed_hi);
+            const u32x4 r = philox((uint32_t)chain, (uint32_t)(((mp.k0 + kb) >> 1) + lane), mp.call, 0u, mp.seed_lo,
+                                   mp.seed_hi);
             const uint32_t p0 = r.x % (uint32_t)N, p1 = r.z % (uint32_t)N;
             draws[2 * lane] = make_uint2(p0 * (uint32_t)ld | p0 << 16 | ((31u - p0) & 31u) << 24, pack_u(r.y, T(0)));
             draws[2 * lane + 1] = make_uint2(p1 * (uint32_t)ld | p1 << 16 | ((31u - p1) & 31u) << 24, pack_u(r.w, T(0)));

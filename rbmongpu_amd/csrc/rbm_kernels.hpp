@@ -54,7 +54,15 @@ struct McParams {
     uint64_t* final_out = nullptr;
     void* angles_out = nullptr;
     int resync = 0;
+    //   k0         index of the chain's first proposal in this launch (single-flip
+    //              RBM chains): a chain continued from `start` at k0 draws what
+    //              one launch would draw from proposal k0 on.  A multiple of
+    //              MC_DRAW_BLOCK and, with resync, of resync.
+    uint64_t k0 = 0;
 };
+
+// proposals per Philox draw block of the single-flip RBM sampler
+constexpr int MC_DRAW_BLOCK = 32;
 
 // FP32 Metropolis chains recompute their angles from the configuration every
 // MC_RESYNC_SWEEPS sweeps (McParams::resync = MC_RESYNC_SWEEPS * N proposals)

@@ -138,3 +138,30 @@ def test_persistent_chains_need_an_rbm():
     with pytest.raises(NotImplementedError):
         R.ExpectationValue(True)(deep, R.Operator(*HAM.tfim_1d(N), True), mc)
 
+
+
+def test_two_launch_sampling_matches_one_launch(monkeypatch):
+    """Low-occupancy complex64 chains (at most one wavefront per SIMD) run in
+    two launches split where one launch re-synchronises the angles, the first
+    launch's samples evaluated on a second stream while the second launch
+    samples: samples, log psi, local energies, acceptance and the gradient
+    equal one launch's bit for bit (RBMHIP_MC_SPLIT=0 forces one launch)."""
+    N, M, chains, per = 20, 80, 1024, 64
+    b, W, _ = random_rbm(N, M, seed=75, scale=0.1)
+    psi = R.Psi(np.zeros(N), np.zeros(N), b, W, 1.0, False, True, precision=32)
+    op = R.Operator(*HAM.tfim_1d(N), True)
+    out = {}
+    for split in ("0", "1"):
+        monkeypatch.setenv("RBMHIP_MC_SPLIT", split)
+        mc = R.MonteCarloLoop(chains * per, 1, 10, chains, True, seed=11)
+        R.timing_enable(True)
+        R.timing_reset()
+        g, E = R.ExpectationValue(True).gradient(psi, op, mc)
+        R.timing_enable(False)
+        launches = R.timing_query("mc")[1]
+        spins, lp, el = mc.last_samples()
+        out[split] = (launches, g, E, spins, lp, el, mc.acceptance_rate)
+        print(f"RBMHIP_MC_SPLIT={split}: {launches} sampler launches, E = {E}", flush=True)
+    assert out["0"][0] == 1 and out["1"][0] == 2
+    for x, y in zip(out["0"][1:], out["1"][1:]):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
