@@ -976,29 +976,39 @@ static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const 
 // Overlap of sampling and evaluation for low-occupancy single-flip chains.
 // With at most one wavefront per SIMD (16 lanes per chain) the sampler's
 // per-proposal dependency chain leaves issue slots idle.  The chains then run
-// in two launches, and the first launch's samples are evaluated on the side
-// stream while the second launch samples.  The split is placed where one
-// launch would recompute the FP32 angles from the configuration anyway (a
-// multiple of the re-sync interval and of the draw block), so the second
-// launch, which starts from the first one's configurations with fresh angles
-// and the same Philox counters (McParams::k0), reproduces one launch's chains
-// bit for bit.  Returns the first launch's samples per chain, or 0 for one
+// in up to MC_SPLIT_CHUNKS launches, and each launch's samples but the last's
+// are evaluated on the side stream while the next launch samples.  Each split
+// is placed where one launch would recompute the FP32 angles from the
+// configuration anyway (a multiple of the re-sync interval and of the draw
+// block), so a launch that starts from the previous one's configurations with
+// fresh angles and the same Philox counters (McParams::k0) reproduces one
+// launch's chains bit for bit.  Returns the sample counts per chain at which
+// launches end (the last is steps_per_chain), or an empty list for one
 // launch.  RBMHIP_MC_SPLIT=0 in the environment turns it off.
-static int mc_split_point(const Ctx& c, const McParams& mp, int N, bool fp32) {
+constexpr int MC_SPLIT_CHUNKS = 4;
+static std::vector<int> mc_split_points(const Ctx& c, const McParams& mp, int N, bool fp32) {
+    std::vector<int> ends;
     const char* env = std::getenv("RBMHIP_MC_SPLIT");
-    if (env && env[0] == '0') return 0;
-    if (!fp32 || mp.move != MC_SINGLE_FLIP || mp.resync <= 0 || mp.sweeps <= 0) return 0;
-    if ((int64_t)mp.chains > 16LL * c.num_cus) return 0;
+    if (env && env[0] == '0') return ends;
+    if (!fp32 || mp.move != MC_SINGLE_FLIP || mp.resync <= 0 || mp.sweeps <= 0) return ends;
+    if ((int64_t)mp.chains > 16LL * c.num_cus) return ends;
     const int S = mp.steps_per_chain;
-    if (S < 16) return 0;
+    if (S < 16) return ends;
     uint64_t L = (uint64_t)mp.resync, g = (uint64_t)MC_DRAW_BLOCK, r = L;
     while (r) { const uint64_t t = g % r; g = r; r = t; }
     L = L / g * MC_DRAW_BLOCK;
     const uint64_t base = (uint64_t)mp.therm * N, per = (uint64_t)mp.sweeps * N;
-    int best = 0;
-    for (int s0 = S / 4; s0 <= S - S / 4; s0++)
-        if ((base + (uint64_t)s0 * per) % L == 0 && (best == 0 || std::abs(s0 - S / 2) < std::abs(best - S / 2))) best = s0;
-    return best;
+    for (int q = 1; q < MC_SPLIT_CHUNKS; q++) {
+        const int target = (int)((int64_t)S * q / MC_SPLIT_CHUNKS);
+        int best = 0;
+        for (int s0 = (ends.empty() ? 0 : ends.back()) + 1; s0 < S; s0++)
+            if ((base + (uint64_t)s0 * per) % L == 0 && (best == 0 || std::abs(s0 - target) < std::abs(best - target)))
+                best = s0;
+        if (best > 0 && std::abs(best - target) <= S / (2 * MC_SPLIT_CHUNKS)) ends.push_back(best);
+    }
+    if (ends.empty()) return ends;
+    ends.push_back(S);
+    return ends;
 }
 
 // Records of samples [off, ...) of a batch: per-sample pointers advanced.
@@ -1058,7 +1068,8 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
         if (e->kind == 1 && (e->persistent || e->record_angles) && psi->kind != 0)
             throw Error(RBMHIP_ERR_UNSUPPORTED, "persistent chains and angle records are implemented for RBM wavefunctions");
         uint32_t* counts = reinterpret_cast<uint32_t*>(c.buf("counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
-        uint32_t* counts_extra = nullptr;   // the second launch's counts when the chains run in two
+        uint32_t* counts_extra = nullptr;   // later launches' counts when the chains run in several
+        size_t counts_extra_n = 0;
         const OpDev od = op ? op_dev(c, op) : empty_op();
         if (e->kind == 2) {
             // independent proposals do not depend on the chain: draw all of them,
@@ -1137,34 +1148,48 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
                 e->angles_precision = sizeof(T) == 4 ? RBMHIP_FP32 : RBMHIP_FP64;
             }
             Records rec{nullptr, s.lp, op ? s.el : nullptr, s.tanh};
-            const int S0 = mc_split_point(c, mp, psi->N, sizeof(T) == 4);
-            if (S0 > 0) {
+            const std::vector<int> ends = mc_split_points(c, mp, psi->N, sizeof(T) == 4);
+            if (!ends.empty()) {
                 c.ensure_side();
-                const int64_t n0 = (int64_t)S0 * mp.chains;
+                const size_t K = ends.size();
                 uint64_t* st = mp.final_out ? mp.final_out
                                             : reinterpret_cast<uint64_t*>(c.buf("mc_split_state", (size_t)mp.chains * 2 * sizeof(uint64_t)));
-                counts_extra = reinterpret_cast<uint32_t*>(c.buf("mc_split_counts", (size_t)mp.chains * 2 * sizeof(uint32_t)));
-                McParams a = mp, b = mp;
-                a.steps_per_chain = S0;
-                a.final_out = st;
-                a.angles_out = nullptr;
-                b.steps_per_chain = mp.steps_per_chain - S0;
-                b.therm = 0;
-                b.start = st;
-                b.k0 = (uint64_t)mp.therm * psi->N + (uint64_t)S0 * mp.sweeps * psi->N;
-                c.launch("mc", [&] { return launch_mc<T>(md, a, s.spins, counts, wlds, lds, c.stream); });
-                RBM_HIPCHK(hipEventRecord(c.ev_main, c.stream));
-                RBM_HIPCHK(hipStreamWaitEvent(c.side, c.ev_main, 0));
-                std::swap(c.stream, c.side);
-                try {
-                    eval_model<T>(c, psi, op, n0, s.spins, 0, rec);
-                } catch (...) {
-                    std::swap(c.stream, c.side);
-                    throw;
+                counts_extra = reinterpret_cast<uint32_t*>(c.buf("mc_split_counts", (K - 1) * mp.chains * 2 * sizeof(uint32_t)));
+                counts_extra_n = (K - 1) * mp.chains * 2;
+                int begin = 0;
+                for (size_t q = 0; q < K; q++) {
+                    const int64_t off = (int64_t)begin * mp.chains, cnt = (int64_t)(ends[q] - begin) * mp.chains;
+                    McParams a = mp;
+                    a.steps_per_chain = ends[q] - begin;
+                    if (q > 0) {
+                        a.therm = 0;
+                        a.start = st;
+                        a.k0 = (uint64_t)mp.therm * psi->N + (uint64_t)begin * mp.sweeps * psi->N;
+                    }
+                    if (q + 1 < K) {
+                        a.final_out = st;
+                        a.angles_out = nullptr;
+                    }
+                    uint32_t* cq = q == 0 ? counts : counts_extra + (q - 1) * mp.chains * 2;
+                    c.launch("mc", [&] { return launch_mc<T>(md, a, s.spins + 2 * off, cq, wlds, lds, c.stream); });
+                    const Records rq = records_from<T>(rec, off, psi->M);
+                    if (q + 1 < K) {
+                        // evaluate beside the next launch
+                        RBM_HIPCHK(hipEventRecord(c.ev_main, c.stream));
+                        RBM_HIPCHK(hipStreamWaitEvent(c.side, c.ev_main, 0));
+                        std::swap(c.stream, c.side);
+                        try {
+                            eval_model<T>(c, psi, op, cnt, s.spins + 2 * off, 0, rq);
+                        } catch (...) {
+                            std::swap(c.stream, c.side);
+                            throw;
+                        }
+                        std::swap(c.stream, c.side);
+                    } else {
+                        eval_model<T>(c, psi, op, cnt, s.spins + 2 * off, 0, rq);
+                    }
+                    begin = ends[q];
                 }
-                std::swap(c.stream, c.side);
-                c.launch("mc", [&] { return launch_mc<T>(md, b, s.spins + 2 * n0, counts_extra, wlds, lds, c.stream); });
-                eval_model<T>(c, psi, op, s.n - n0, s.spins + 2 * n0, 0, records_from<T>(rec, n0, psi->M));
                 RBM_HIPCHK(hipEventRecord(c.ev_side, c.side));
                 RBM_HIPCHK(hipStreamWaitEvent(c.stream, c.ev_side, 0));
             } else {
@@ -1194,7 +1219,7 @@ static Samples sample(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, rbmhip_ens* e, boo
         std::vector<uint32_t> h((size_t)mp.chains * 2), h2;
         c.d2h(h.data(), counts, h.size() * sizeof(uint32_t));
         if (counts_extra) {
-            h2.resize(h.size());
+            h2.resize(counts_extra_n);
             c.d2h(h2.data(), counts_extra, h2.size() * sizeof(uint32_t));
         }
         c.sync();

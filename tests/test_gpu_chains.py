@@ -142,10 +142,11 @@ def test_persistent_chains_need_an_rbm():
 
 def test_two_launch_sampling_matches_one_launch(monkeypatch):
     """Low-occupancy complex64 chains (at most one wavefront per SIMD) run in
-    two launches split where one launch re-synchronises the angles, the first
-    launch's samples evaluated on a second stream while the second launch
-    samples: samples, log psi, local energies, acceptance and the gradient
-    equal one launch's bit for bit (RBMHIP_MC_SPLIT=0 forces one launch)."""
+    several launches split where one launch re-synchronises the angles, each
+    launch's samples but the last's evaluated on a second stream while the
+    next launch samples: samples, log psi, local energies, acceptance and the
+    gradient equal one launch's bit for bit (RBMHIP_MC_SPLIT=0 forces one
+    launch)."""
     N, M, chains, per = 20, 80, 1024, 64
     b, W, _ = random_rbm(N, M, seed=75, scale=0.1)
     psi = R.Psi(np.zeros(N), np.zeros(N), b, W, 1.0, False, True, precision=32)
@@ -162,6 +163,6 @@ def test_two_launch_sampling_matches_one_launch(monkeypatch):
         spins, lp, el = mc.last_samples()
         out[split] = (launches, g, E, spins, lp, el, mc.acceptance_rate)
         print(f"RBMHIP_MC_SPLIT={split}: {launches} sampler launches, E = {E}", flush=True)
-    assert out["0"][0] == 1 and out["1"][0] == 2
+    assert out["0"][0] == 1 and out["1"][0] > 1
     for x, y in zip(out["0"][1:], out["1"][1:]):
         assert np.array_equal(np.asarray(x), np.asarray(y))
