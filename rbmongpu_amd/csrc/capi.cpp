@@ -985,7 +985,7 @@ static void eval_model(Ctx& c, rbmhip_psi* psi, rbmhip_op* op, int64_t n, const 
 // launch's chains bit for bit.  Returns the sample counts per chain at which
 // launches end (the last is steps_per_chain), or an empty list for one
 // launch.  RBMHIP_MC_SPLIT=0 in the environment turns it off.
-constexpr int MC_SPLIT_CHUNKS = 4;
+constexpr int MC_SPLIT_CHUNKS = 8;
 static std::vector<int> mc_split_points(const Ctx& c, const McParams& mp, int N, bool fp32) {
     std::vector<int> ends;
     const char* env = std::getenv("RBMHIP_MC_SPLIT");
